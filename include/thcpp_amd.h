@@ -1,0 +1,186 @@
+/*
+ * thcpp_amd: MI355X (gfx950) ISDF factorisation hot path behind a C ABI.
+ *
+ * This is the boundary a host (the C++ drop-in classes, Python via ctypes, or
+ * any FFI) binds.  Every array argument is a DEVICE pointer on the context's
+ * device unless marked "host".  No C++ types, exceptions or allocations cross
+ * the boundary; every call returns an int status (THC_OK = 0).  A context owns
+ * one HIP stream and a device workspace; it is not internally synchronised
+ * (one context per host thread).  Calls are stream-ordered on the context's
+ * stream and return after the stream has drained unless stated otherwise.
+ *
+ * Reference interfaces replaced (paths relative to JoonhoLee-Group/thcpp):
+ *   KMeans::KMeansSolver::kernel            src/kmeans.cpp:187-264
+ *   InterpolatingVectors::IVecs ctor        src/interpolating_vectors.cpp:17-57
+ *   InterpolatingVectors::IVecs::kernel     src/interpolating_vectors.cpp:406-455
+ * and the per-stage routines they call (cited on each entry point below).
+ */
+#ifndef THCPP_AMD_H
+#define THCPP_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Errors stop the call; warnings (>= 100) mean the outputs were
+ * written, mirroring where the reference prints a warning and continues. */
+enum thc_status {
+  THC_OK = 0,
+  THC_ERR_ARG = 1,              /* bad argument (null pointer, size, unsupported shape) */
+  THC_ERR_HIP = 2,              /* HIP runtime error; thc_last_error() has the text */
+  THC_ERR_EMPTY_CLUSTER = 3,    /* a k-means cluster weight < 1e-8 (kmeans.cpp:132,137-143, the reference exits) */
+  THC_ERR_NOT_POSDEF = 4,       /* FP64 Cholesky of CC^T failed (the reference's QR solve would return garbage) */
+  THC_ERR_NO_MEMORY = 5,        /* device workspace allocation failed */
+  THC_ERR_UNSUPPORTED = 6,      /* FFT length with a prime factor > 5, or > 4096 clusters, etc. */
+  THC_WARN_DUPLICATE_INDICES = 100, /* map_to_grid found repeated indices (kmeans.cpp:151-155) */
+  THC_WARN_LUV_NOT_POSDEF = 101,    /* chol(Muv) failed; Luv still written (interpolating_vectors.cpp:296-304) */
+  THC_WARN_FFT_GRID_MISMATCH = 102  /* prod(fft_grid) != N (interpolating_vectors.cpp:368-370) */
+};
+
+/* Arithmetic used by the K^2 N stages (solve application, G contraction). */
+enum thc_precision {
+  THC_PREC_FP64 = 0,  /* FP64 MFMA throughout */
+  THC_PREC_FP32 = 1,  /* FP32 MFMA for ZC^T and the G contraction, FP32 FFT; FP64 Cholesky/solve/V sum */
+  THC_PREC_BF16 = 2   /* BF16 split operands with FP32 accumulation for the G contraction */
+};
+
+typedef struct thc_ctx thc_ctx;
+
+/* Per-run k-means report. */
+typedef struct thc_kmeans_stats {
+  int iterations;          /* iterations executed */
+  int converged;           /* 1 if diff < threshold ended the loop */
+  int duplicates;          /* adjacent repeated indices after sorting */
+  int interp_zeroed;       /* 1 if the loop ended with diff == threshold (indices left 0, as the reference does) */
+  double last_diff;        /* normed difference of the final iteration */
+  double *diffs;           /* host, optional: max_it entries, normed difference per iteration */
+  double *pcts;            /* host, optional: max_it entries, % of grid points that changed cluster */
+  double elapsed_ms;       /* device time of the call */
+} thc_kmeans_stats;
+
+typedef struct thc_isdf_opts {
+  int precision;           /* enum thc_precision */
+  int refine_steps;        /* FP64 iterative-refinement steps of the least-squares solve (0 = none) */
+  int want_luv;            /* 1: also factor V = Luv Luv^T */
+} thc_isdf_opts;
+
+typedef struct thc_isdf_stats {
+  int cct_chol_info;       /* 0, or the 1-based order of the first non-positive leading minor of CC^T */
+  int luv_chol_info;       /* same for V */
+  double stage_ms[8];      /* device time: 0 ZC^T, 1 CC^T+chol+inverse, 2 solve, 3 FFT, 4 V, 5 Luv, 6 total, 7 refinement */
+} thc_isdf_stats;
+
+/* ---------------------------------------------------------------- context */
+
+/* Create a context on HIP device 'device'.  'stream' may be NULL (the context
+ * creates a default-priority stream of its own) or a hipStream_t to use. */
+int thc_ctx_create(int device, void *stream, thc_ctx **out);
+int thc_ctx_destroy(thc_ctx *ctx);
+/* Use HIP graphs for the k-means iteration loop (default 1). */
+int thc_ctx_set_graphs(thc_ctx *ctx, int enable);
+const char *thc_status_string(int status);
+/* Text of the last HIP error seen on this thread. */
+const char *thc_last_error(void);
+/* Library version string. */
+const char *thc_version(void);
+
+/* ---------------------------------------------------------------- k-means */
+
+/* Initial interpolation indices: iota(N), std::shuffle with std::mt19937(seed),
+ * first K sorted ascending.  Host arrays.  Replaces
+ * KMeansSolver::guess_initial_centroids (src/kmeans.cpp:159-175). */
+int thc_kmeans_initial_indices(long n, int k, unsigned int seed, int *host_idx_out);
+
+/* Nearest centroid per grid point, ties to the first index, bit-identical to
+ * KMeansSolver::classify_grid_points (src/kmeans.cpp:88-106).
+ * grid: n x 3 f64 (xyz interleaved), centroids: k x 3 f64, grid_map: n int out. */
+int thc_classify_grid_points(thc_ctx *ctx, const double *grid, long n,
+                             const double *centroids, int k, int *grid_map);
+
+/* Weighted centroids from an assignment, summed in ascending grid index and
+ * divided as KMeansSolver::update_centroids does (src/kmeans.cpp:108-144).
+ * Returns THC_ERR_EMPTY_CLUSTER when a weight is < 1e-8 (centroids still written). */
+int thc_update_centroids(thc_ctx *ctx, const double *rho, const double *grid, long n,
+                         const int *grid_map, int k, double *centroids);
+
+/* Nearest grid point per centroid, sorted ascending (KMeansSolver::map_to_grid,
+ * src/kmeans.cpp:146-157).  interp_idx: k int out.  Returns
+ * THC_WARN_DUPLICATE_INDICES when indices repeat. */
+int thc_map_to_grid(thc_ctx *ctx, const double *grid, long n, const double *centroids,
+                    int k, int *interp_idx);
+
+/* Full weighted k-means (KMeansSolver::kernel, src/kmeans.cpp:187-264) at one
+ * rank: initial centroids from host_init_idx (host, k ints) or, if NULL, from
+ * thc_kmeans_initial_indices(n, k, seed); then max_it Lloyd iterations that stop
+ * when the normed centroid difference drops below 'threshold'; then map_to_grid.
+ * interp_idx: k int out (sorted).  grid_map (n int) and centroids (k x 3 f64)
+ * are optional outputs (the final assignment and final centroids). */
+int thc_kmeans(thc_ctx *ctx, const double *grid, const double *rho, long n, int k,
+               int max_it, double threshold, unsigned int seed, const int *host_init_idx,
+               int *interp_idx, int *grid_map, double *centroids, thc_kmeans_stats *stats);
+
+/* ---------------------------------------------------------------- ISDF stages */
+
+/* ZC^T = (Phi_mu Phi^T)^2 elementwise, k x n row-major f64 (IVecs::setup_pseudo_dm
+ * + setup_CZt, src/interpolating_vectors.cpp:59-141).  aoR: n x m row-major f64
+ * (the orbital file's (N, M) layout), idx: k grid indices. */
+int thc_czt(thc_ctx *ctx, const double *aoR, long n, int m, const int *idx, int k, double *czt);
+
+/* CC^T = ZC^T[:, idx], k x k (IVecs::setup_CCt, src/interpolating_vectors.cpp:163-190). */
+int thc_cct(thc_ctx *ctx, const double *czt, long n, const int *idx, int k, double *cct);
+
+/* In-place lower Cholesky of a k x k row-major SPD matrix (strict upper part
+ * zeroed).  *info = 0 or the 1-based failing leading minor (pzpotrf 'L',
+ * src/matrix_operations.h:259-267). */
+int thc_cholesky(thc_ctx *ctx, double *a, int k, int *info);
+
+/* X = CC^T^-1 ZC^T, k x n: FP64 Cholesky of CC^T, explicit FP64 inverse from
+ * the factor, one MFMA GEMM, plus 'refine_steps' FP64 residual corrections.
+ * Replaces MatrixOperations::least_squares / pzgels (src/matrix_operations.h:184-202,
+ * called at src/interpolating_vectors.cpp:416). */
+int thc_least_squares(thc_ctx *ctx, const double *cct, const double *czt, long n, int k,
+                      int refine_steps, double *x, int *info);
+
+/* Coulomb-weighted half spectrum of the interpolating vectors: for each row mu
+ * of x (k x n, real), the unnormalised forward 3-D DFT over fft_grid (C order),
+ * kept for kz <= n2/2 and written as out[mu][g] (re, im interleaved) scaled by
+ * sqrt(w(G) coulG(G)) with w = 1 on self-conjugate planes and 2 elsewhere, so
+ * that V = out out^T.  out is k x (2 n0 n1 (n2/2+1)) f64.  Replaces
+ * IVecs::fft_vectors + the sqrt(coulG) scaling of construct_muv
+ * (src/interpolating_vectors.cpp:331-404, 258-263). */
+int thc_fft_vectors(thc_ctx *ctx, const double *x, int k, const int fft_grid[3],
+                    const double *coulG, double *out);
+
+/* V = B B^T (k x k, full symmetric) from the weighted half spectrum B
+ * (k x g2).  Replaces the pzherk of IVecs::construct_muv
+ * (src/interpolating_vectors.cpp:278). */
+int thc_construct_muv(thc_ctx *ctx, const double *b, int k, long g2, double *v);
+
+/* ---------------------------------------------------------------- whole path */
+
+/* IVecs ctor + kernel at one rank: ZC^T, CC^T, least squares, FFT, V and
+ * (opts->want_luv) Luv = chol(V) lower with zero upper triangle
+ * (src/interpolating_vectors.cpp:17-57, 406-455, 283-315).
+ *   aoR n x m f64, idx k ints (sorted interpolation points), coulG n f64.
+ *   v k x k out; luv k x k out (nullable); xi k x n out (nullable: the
+ *   interpolating vectors X = CC^T^-1 ZC^T, row mu = xi_mu(r)).
+ * Returns THC_ERR_NOT_POSDEF if CC^T is not numerically positive definite,
+ * THC_WARN_LUV_NOT_POSDEF if V is not (Luv still written). */
+int thc_isdf(thc_ctx *ctx, const double *aoR, long n, int m, const int *idx, int k,
+             const int fft_grid[3], const double *coulG, const thc_isdf_opts *opts,
+             double *v, double *luv, double *xi, thc_isdf_stats *stats);
+
+/* One complete factorisation: thc_kmeans then thc_isdf on its points
+ * (src/main.cpp:50-60 at one rank).  interp_idx k ints out. */
+int thc_factorize(thc_ctx *ctx, const double *grid, const double *rho, const double *aoR,
+                  long n, int m, int thc_cfac, int max_it, double threshold, unsigned int seed,
+                  const int fft_grid[3], const double *coulG, const thc_isdf_opts *opts,
+                  int *interp_idx, double *v, double *luv,
+                  thc_kmeans_stats *kstats, thc_isdf_stats *istats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* THCPP_AMD_H */

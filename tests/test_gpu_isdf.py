@@ -1,0 +1,146 @@
+"""ISDF stages on the GPU against closed forms and the oracle's restatement
+of src/interpolating_vectors.cpp."""
+import numpy as np
+import pytest
+import scipy.linalg
+import torch
+
+from oracle import isdf_oracle as IO
+from thcpp_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _rand_orbitals(n, m, seed):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, m)))
+    return q
+
+
+@pytest.mark.parametrize("n,m,k", [(4096, 12, 40), (3375, 7, 130), (20000, 64, 300)])
+def test_czt_cct(ctx, n, m, k):
+    a = _rand_orbitals(n, m, n)
+    idx = np.sort(np.random.default_rng(k).choice(n, k, replace=False)).astype(np.int32)
+    z = ctx.czt(a, idx)
+    zr = IO.czt(a, idx)
+    assert rel(z.cpu(), zr) < 1e-14
+    c = ctx.cct(z, idx).cpu().numpy()
+    np.testing.assert_array_equal(c, z.cpu().numpy()[:, idx])
+
+
+@pytest.mark.parametrize("k", [1, 64, 200, 777, 1100])
+def test_cholesky(ctx, k):
+    rng = np.random.default_rng(k)
+    b = rng.standard_normal((k, k + 10))
+    a = b @ b.T / k + np.eye(k) * 0.1
+    l, info = ctx.cholesky(a)
+    assert info == 0
+    lr = np.linalg.cholesky(a)
+    assert rel(l.cpu(), lr) < 1e-12
+    assert not np.triu(l.cpu().numpy(), 1).any()
+
+
+def test_cholesky_not_posdef(ctx):
+    a = np.eye(100)
+    a[70, 70] = -1.0
+    _, info = ctx.cholesky(a)
+    assert info == 71
+
+
+@pytest.mark.parametrize("k,n", [(100, 5000), (513, 20000)])
+def test_least_squares(ctx, k, n):
+    rng = np.random.default_rng(k)
+    b = rng.standard_normal((k, 3 * k))
+    c = b @ b.T / k + np.eye(k)
+    z = rng.standard_normal((k, n))
+    x = ctx.least_squares(c, z).cpu().numpy()
+    xr = scipy.linalg.solve(c, z, assume_a="pos")
+    assert rel(x, xr) < 1e-12
+
+
+def _half_spectrum_ref(x, n, coulg):
+    k = x.shape[0]
+    f = np.fft.rfftn(x.reshape(k, *n), axes=(1, 2, 3))
+    kz = np.arange(n[2] // 2 + 1)
+    w = np.where((kz == 0) | (2 * kz == n[2]), 1.0, 2.0)
+    cg = coulg.reshape(n)[:, :, : n[2] // 2 + 1] * w
+    f = f * np.sqrt(cg)[None]
+    return np.stack([f.real, f.imag], axis=-1).reshape(k, -1)
+
+
+@pytest.mark.parametrize("grid", [(8, 8, 8), (12, 10, 9), (16, 16, 16), (32, 32, 32), (24, 20, 30),
+                                  (27, 25, 45), (64, 64, 64), (5, 3, 2), (1, 4, 6)])
+def test_fft_vectors(ctx, grid):
+    from thcpp_amd import synthetic
+    rng = np.random.default_rng(sum(grid))
+    k = 5
+    n = int(np.prod(grid))
+    x = rng.standard_normal((k, n))
+    cg = synthetic.coulomb_kernel(7.0, grid)
+    got = ctx.fft_vectors(x, grid, cg).cpu().numpy()
+    ref = _half_spectrum_ref(x, grid, cg)
+    assert rel(got, ref) < 1e-13
+
+
+def test_fft_vectors_matches_oracle_muv(ctx):
+    """sum over the full grid of conj(A_mu) coulG A_nu (oracle, complex, reference
+    order) equals the real Gram of the weighted half spectrum."""
+    from thcpp_amd import synthetic
+    grid = (12, 10, 8)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((6, int(np.prod(grid))))
+    cg = synthetic.coulomb_kernel(6.0, grid)
+    b = ctx.fft_vectors(x, grid, cg).cpu().numpy()
+    muv = IO.construct_muv(IO.fft_vectors(x, grid), cg)
+    assert np.abs(muv.imag).max() < 1e-10 * np.abs(muv).max()
+    assert rel(b @ b.T, muv.real) < 1e-13
+
+
+@pytest.mark.parametrize("k,g2", [(50, 1000), (300, 70000), (1030, 5000)])
+def test_construct_muv(ctx, k, g2):
+    rng = np.random.default_rng(k)
+    b = rng.standard_normal((k, g2))
+    v = ctx.construct_muv(b).cpu().numpy()
+    assert rel(v, b @ b.T) < 1e-13
+    np.testing.assert_array_equal(v, v.T)
+
+
+def test_isdf_cfg0_vs_oracle(ctx, cfg0, cfg0_kmeans):
+    s = cfg0
+    idx = cfg0_kmeans["interp_idx"]
+    ref = IO.isdf(s.aoR, idx, s.n, s.coulG, return_all=True)
+    out = ctx.isdf(s.aoR, idx, s.n, s.coulG, want_xi=True)
+    assert out["status"] == L.THC_OK
+    ev = rel(out["V"].cpu(), ref["V"])
+    ex = rel(out["xi"].cpu(), ref["X"])
+    print("cfg0 FP64 rel V %.3e rel xi %.3e" % (ev, ex))
+    # CC^T has condition number ~4.5e11 here; two FP64 solvers (QR vs Cholesky)
+    # already differ by ~7e-8 in V.
+    assert ev < 1e-6 and ex < 1e-5
+    l = out["Luv"].cpu().numpy()
+    assert rel(l @ l.T, out["V"].cpu()) < 1e-13
+
+
+def test_isdf_fp32_cfg0(ctx, cfg0, cfg0_kmeans):
+    s = cfg0
+    idx = cfg0_kmeans["interp_idx"]
+    ref = ctx.isdf(s.aoR, idx, s.n, s.coulG)
+    out = ctx.isdf(s.aoR, idx, s.n, s.coulG, precision=L.PREC_FP32)
+    e = rel(out["V"].cpu(), ref["V"].cpu())
+    print("cfg0 FP32 FFT+contraction rel V vs FP64 %.3e" % e)
+    assert e < 1e-5
+
+
+def test_factorize_cfg0(ctx, cfg0, cfg0_kmeans):
+    s = cfg0
+    out = ctx.factorize(s.grid, s.rho, s.aoR, s.thc_cfac, s.max_it, s.n, s.coulG)
+    assert out["status"] == L.THC_OK
+    np.testing.assert_array_equal(out["interp_idx"].cpu().numpy(), cfg0_kmeans["interp_idx"])
+    ref = IO.isdf(s.aoR, cfg0_kmeans["interp_idx"], s.n, s.coulG)
+    assert rel(out["V"].cpu(), ref["V"]) < 1e-6

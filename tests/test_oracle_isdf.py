@@ -1,0 +1,126 @@
+"""Pins for the ISDF oracle (oracle/isdf_oracle.py), CPU only.
+
+Reference-held data: tests/golden/etaPm.npy and orb_prod.npy (from the
+reference's tests/data/eri_thc.h5) fix the Hadamard-squared Gram CC^T =
+(eta eta^T)^2 = orb_prod orb_prod^T and its numerical rank under the
+reference's rank rule.  Mathematics fixes the rest: a brute-force DFT, the
+Coulomb energy of a plane wave, invariance of V under a constant shift of a
+vector (G = 0 is excluded), interpolation xi_mu(r_nu) = delta, and
+Luv Luv^T = V.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import isdf_oracle as IO
+from thcpp_amd import synthetic
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def test_cct_fixture_hadamard_gram():
+    eta = np.load(os.path.join(GOLDEN, "etaPm.npy"))
+    prod = np.load(os.path.join(GOLDEN, "orb_prod.npy"))
+    idx = np.arange(eta.shape[0])
+    z = IO.czt(eta, idx)
+    assert rel(z, prod @ prod.T) < 1e-14
+    c = IO.cct(z, idx)
+    np.testing.assert_array_equal(c, z)
+    # M = 10 real orbitals: rank <= M (M + 1) / 2 = 55; the reference's rule gives 49
+    assert IO.rank(c) == 49
+
+
+def test_czt_is_square_of_pseudo_dm():
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((300, 7))
+    idx = np.array([3, 10, 77, 200])
+    p = IO.pseudo_dm(a, idx)
+    np.testing.assert_allclose(p, a[idx] @ a.T, rtol=0, atol=1e-14)
+    np.testing.assert_array_equal(IO.czt(a, idx), p * p)
+
+
+def _dft_bruteforce(x, n):
+    n0, n1, n2 = n
+    k0, k1, k2 = np.meshgrid(np.arange(n0), np.arange(n1), np.arange(n2), indexing="ij")
+    r0, r1, r2 = k0.reshape(-1), k1.reshape(-1), k2.reshape(-1)
+    ph = np.exp(-2j * np.pi * (np.outer(r0, r0) / n0 + np.outer(r1, r1) / n1 + np.outer(r2, r2) / n2))
+    return x @ ph.T  # rows: vectors; columns: G (C order)
+
+
+@pytest.mark.parametrize("n", [(4, 3, 5), (2, 6, 4), (3, 3, 3)])
+def test_fft_vectors_bruteforce(n):
+    rng = np.random.default_rng(sum(n))
+    x = rng.standard_normal((3, int(np.prod(n))))
+    ivg = IO.fft_vectors(x, n)
+    assert rel(ivg.T, _dft_bruteforce(x, n)) < 1e-13
+    # real input: X(-G) = conj X(G)
+    a = ivg[:, 0].reshape(n)
+    neg = a[(-np.arange(n[0])) % n[0]][:, (-np.arange(n[1])) % n[1]][:, :, (-np.arange(n[2])) % n[2]]
+    assert np.abs(neg - a.conj()).max() < 1e-12
+
+
+def test_plane_wave_coulomb_closed_form():
+    """xi(r) = cos(G0 . r): its transform is N/2 at +-G0, so
+    V = 2 (N/2)^2 coulG(G0) exactly."""
+    L, n = 7.3, (8, 6, 10)
+    N = int(np.prod(n))
+    grid = synthetic.uniform_grid(L, n)
+    cg = synthetic.coulomb_kernel(L, n)
+    m = np.array([1, 2, 3])
+    g0 = 2 * np.pi / L * m
+    x = np.cos(grid @ g0)[None, :]
+    v = IO.construct_muv(IO.fft_vectors(x, n), cg).real[0, 0]
+    gidx = (m[0] * n[1] + m[1]) * n[2] + m[2]
+    want = 2 * (N / 2) ** 2 * cg[gidx]
+    assert abs(v - want) < 1e-12 * want
+    assert abs(cg[gidx] - 4 * np.pi / g0.dot(g0) * L ** 3 / N ** 2) < 1e-15 * cg[gidx]
+
+
+def test_g0_excluded_shift_invariance():
+    rng = np.random.default_rng(3)
+    n = (6, 5, 4)
+    cg = synthetic.coulomb_kernel(5.0, n)
+    assert cg[0] == 0.0
+    x = rng.standard_normal((4, int(np.prod(n))))
+    v1 = IO.construct_muv(IO.fft_vectors(x, n), cg)
+    v2 = IO.construct_muv(IO.fft_vectors(x + rng.standard_normal((4, 1)), n), cg)
+    assert rel(v2, v1) < 1e-13
+    assert np.allclose(v1, v1.conj().T)
+    assert np.linalg.eigvalsh(v1.real).min() > -1e-12 * np.abs(v1).max()
+
+
+def test_least_squares_square_system():
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal((40, 60))
+    c = b @ b.T
+    z = rng.standard_normal((40, 500))
+    x = IO.least_squares(c, z)
+    assert rel(x, scipy.linalg.solve(c, z)) < 1e-10
+    assert rel(c @ x, z) < 1e-12
+
+
+def test_isdf_interpolates_at_its_points():
+    """X = CC^T^-1 ZC^T and CC^T = ZC^T[:, idx] give X[:, idx] = I."""
+    s = synthetic.make_system(reps=1, n=12, M=8, thc_cfac=2, max_it=1)
+    idx = np.sort(np.random.default_rng(1).choice(s.N, s.K, replace=False))
+    out = IO.isdf(s.aoR, idx, s.n, s.coulG, return_all=True)
+    assert np.abs(out["X"][:, idx] - np.eye(s.K)).max() < 1e-8
+    assert out["imag_max"] < 1e-10 * np.abs(out["V"]).max()
+
+
+def test_cholesky_luv():
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal((30, 50))
+    v = b @ b.T
+    l, info = IO.cholesky_luv(v)
+    assert info == 0 and rel(l @ l.T, v) < 1e-14 and not np.triu(l, 1).any()
+    bad = np.eye(10)
+    bad[6, 6] = -2
+    _, info = IO.cholesky_luv(bad)
+    assert info == 7

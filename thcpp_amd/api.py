@@ -1,0 +1,178 @@
+"""Python host over the C ABI, for tests and benchmarks.
+
+Device memory and streams come from PyTorch (plumbing only); every
+computation is a call into libthcpp_amd.so.  Method names follow the
+reference's functions (src/kmeans.cpp, src/interpolating_vectors.cpp).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev(t, dtype, device):
+    if isinstance(t, np.ndarray):
+        t = torch.from_numpy(np.ascontiguousarray(t))
+    return t.to(device=device, dtype=dtype).contiguous()
+
+
+class Context:
+    """One thc_ctx on one GPU (device index 'device')."""
+
+    def __init__(self, device: int = 0, graphs: bool = True, stream=None):
+        self.device = torch.device("cuda", device)
+        h = ctypes.c_void_p()
+        L.check(L.lib().thc_ctx_create(device, stream, ctypes.byref(h)), "thc_ctx_create")
+        self.h = h
+        L.check(L.lib().thc_ctx_set_graphs(self.h, 1 if graphs else 0), "thc_ctx_set_graphs")
+
+    def close(self):
+        if self.h:
+            L.lib().thc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def f64(self, a):
+        return _dev(a, torch.float64, self.device)
+
+    def i32(self, a):
+        return _dev(a, torch.int32, self.device)
+
+    # ------------------------------------------------------------ k-means
+    @staticmethod
+    def initial_indices(n: int, k: int, seed: int) -> np.ndarray:
+        out = np.empty(k, dtype=np.int32)
+        L.check(L.lib().thc_kmeans_initial_indices(n, k, seed, out.ctypes.data_as(ctypes.c_void_p)),
+                "thc_kmeans_initial_indices")
+        return out
+
+    def classify_grid_points(self, grid, centroids):
+        g, c = self.f64(grid), self.f64(centroids)
+        out = torch.empty(g.shape[0], dtype=torch.int32, device=self.device)
+        L.check(L.lib().thc_classify_grid_points(self.h, _p(g), g.shape[0], _p(c), c.shape[0], _p(out)),
+                "thc_classify_grid_points")
+        return out
+
+    def update_centroids(self, rho, grid, grid_map, k):
+        r, g, m = self.f64(rho).reshape(-1), self.f64(grid), self.i32(grid_map)
+        out = torch.empty((k, 3), dtype=torch.float64, device=self.device)
+        st = L.lib().thc_update_centroids(self.h, _p(r), _p(g), g.shape[0], _p(m), k, _p(out))
+        return out, st
+
+    def map_to_grid(self, grid, centroids):
+        g, c = self.f64(grid), self.f64(centroids)
+        out = torch.empty(c.shape[0], dtype=torch.int32, device=self.device)
+        st = L.lib().thc_map_to_grid(self.h, _p(g), g.shape[0], _p(c), c.shape[0], _p(out))
+        L.check(st, "thc_map_to_grid", allow=(L.THC_WARN_DUPLICATE_INDICES,))
+        return out, st
+
+    def kmeans(self, grid, rho, k, max_it, threshold=-1.0, seed=0, init_idx=None, want_map=False,
+               want_centroids=False):
+        g, r = self.f64(grid), self.f64(rho).reshape(-1)
+        n = g.shape[0]
+        idx = torch.empty(k, dtype=torch.int32, device=self.device)
+        gm = torch.empty(n, dtype=torch.int32, device=self.device) if want_map else None
+        cen = torch.empty((k, 3), dtype=torch.float64, device=self.device) if want_centroids else None
+        diffs = np.zeros(max_it)
+        pcts = np.zeros(max_it)
+        stats = L.KmeansStats()
+        stats.diffs = diffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        stats.pcts = pcts.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        init = None
+        if init_idx is not None:
+            init = np.ascontiguousarray(init_idx, dtype=np.int32)
+        st = L.lib().thc_kmeans(self.h, _p(g), _p(r), n, k, max_it, threshold, seed,
+                                init.ctypes.data_as(ctypes.c_void_p) if init is not None else None,
+                                _p(idx), _p(gm), _p(cen), ctypes.byref(stats))
+        it = stats.iterations
+        return {"status": st, "interp_idx": idx, "grid_map": gm, "centroids": cen, "iters": it,
+                "diffs": diffs[:it].copy(), "pcts": pcts[:it].copy(), "converged": bool(stats.converged),
+                "duplicates": stats.duplicates, "interp_zeroed": bool(stats.interp_zeroed),
+                "elapsed_ms": stats.elapsed_ms}
+
+    # ------------------------------------------------------------ ISDF stages
+    def czt(self, aoR, idx):
+        a, i = self.f64(aoR), self.i32(idx)
+        out = torch.empty((i.shape[0], a.shape[0]), dtype=torch.float64, device=self.device)
+        L.check(L.lib().thc_czt(self.h, _p(a), a.shape[0], a.shape[1], _p(i), i.shape[0], _p(out)), "thc_czt")
+        return out
+
+    def cct(self, czt, idx):
+        z, i = self.f64(czt), self.i32(idx)
+        out = torch.empty((i.shape[0], i.shape[0]), dtype=torch.float64, device=self.device)
+        L.check(L.lib().thc_cct(self.h, _p(z), z.shape[1], _p(i), i.shape[0], _p(out)), "thc_cct")
+        return out
+
+    def cholesky(self, a):
+        m = self.f64(a).clone()
+        info = ctypes.c_int(0)
+        L.check(L.lib().thc_cholesky(self.h, _p(m), m.shape[0], ctypes.byref(info)), "thc_cholesky")
+        return m, info.value
+
+    def least_squares(self, cct, czt, refine=0):
+        c, z = self.f64(cct), self.f64(czt)
+        out = torch.empty_like(z)
+        info = ctypes.c_int(0)
+        st = L.lib().thc_least_squares(self.h, _p(c), _p(z), z.shape[1], z.shape[0], refine, _p(out),
+                                       ctypes.byref(info))
+        L.check(st, "thc_least_squares")
+        return out
+
+    def fft_vectors(self, x, fft_grid, coulG):
+        xx, cg = self.f64(x), self.f64(coulG).reshape(-1)
+        n = (ctypes.c_int * 3)(*[int(v) for v in fft_grid])
+        g2 = 2 * int(fft_grid[0]) * int(fft_grid[1]) * (int(fft_grid[2]) // 2 + 1)
+        out = torch.empty((xx.shape[0], g2), dtype=torch.float64, device=self.device)
+        L.check(L.lib().thc_fft_vectors(self.h, _p(xx), xx.shape[0], n, _p(cg), _p(out)), "thc_fft_vectors")
+        return out
+
+    def construct_muv(self, b):
+        bb = self.f64(b)
+        out = torch.empty((bb.shape[0], bb.shape[0]), dtype=torch.float64, device=self.device)
+        L.check(L.lib().thc_construct_muv(self.h, _p(bb), bb.shape[0], bb.shape[1], _p(out)), "thc_construct_muv")
+        return out
+
+    def isdf(self, aoR, idx, fft_grid, coulG, precision=L.PREC_FP64, refine=0, want_luv=True, want_xi=False):
+        a, i, cg = self.f64(aoR), self.i32(idx), self.f64(coulG).reshape(-1)
+        n, m = a.shape
+        k = i.shape[0]
+        v = torch.empty((k, k), dtype=torch.float64, device=self.device)
+        luv = torch.empty((k, k), dtype=torch.float64, device=self.device) if want_luv else None
+        xi = torch.empty((k, n), dtype=torch.float64, device=self.device) if want_xi else None
+        opts = L.IsdfOpts(precision, refine, 1 if want_luv else 0)
+        stats = L.IsdfStats()
+        fg = (ctypes.c_int * 3)(*[int(q) for q in fft_grid])
+        st = L.lib().thc_isdf(self.h, _p(a), n, m, _p(i), k, fg, _p(cg), ctypes.byref(opts), _p(v), _p(luv),
+                              _p(xi), ctypes.byref(stats))
+        return {"status": st, "V": v, "Luv": luv, "xi": xi, "cct_info": stats.cct_chol_info,
+                "luv_info": stats.luv_chol_info, "stage_ms": list(stats.stage_ms)}
+
+    def factorize(self, grid, rho, aoR, thc_cfac, max_it, fft_grid, coulG, threshold=-1.0, seed=0,
+                  precision=L.PREC_FP64, refine=0, want_luv=True):
+        g, r, a, cg = self.f64(grid), self.f64(rho).reshape(-1), self.f64(aoR), self.f64(coulG).reshape(-1)
+        n, m = a.shape
+        k = thc_cfac * m
+        idx = torch.empty(k, dtype=torch.int32, device=self.device)
+        v = torch.empty((k, k), dtype=torch.float64, device=self.device)
+        luv = torch.empty((k, k), dtype=torch.float64, device=self.device) if want_luv else None
+        opts = L.IsdfOpts(precision, refine, 1 if want_luv else 0)
+        ks, ist = L.KmeansStats(), L.IsdfStats()
+        fg = (ctypes.c_int * 3)(*[int(q) for q in fft_grid])
+        st = L.lib().thc_factorize(self.h, _p(g), _p(r), _p(a), n, m, thc_cfac, max_it, threshold, seed, fg,
+                                   _p(cg), ctypes.byref(opts), _p(idx), _p(v), _p(luv), ctypes.byref(ks),
+                                   ctypes.byref(ist))
+        return {"status": st, "interp_idx": idx, "V": v, "Luv": luv, "kmeans_ms": ks.elapsed_ms,
+                "iters": ks.iterations, "stage_ms": list(ist.stage_ms)}

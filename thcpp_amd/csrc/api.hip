@@ -1,0 +1,493 @@
+// C ABI of libthcpp_amd: context, workspace, the ISDF pipeline (IVecs ctor +
+// kernel at one rank, src/interpolating_vectors.cpp:17-57 and 406-455) and the
+// k-means entry points (kmeans.hip).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "common.h"
+#include "kmeans.h"
+
+extern "C" int thc_seed_indices_impl(long n, int k, unsigned int seed, int* out);
+
+namespace thc {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const char* what, const char* file, int line) {
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "%s (%s:%d)", what, file, line);
+  g_last_error = buf;
+}
+
+int Arena::reserve(size_t bytes) {
+  if (bytes <= cap) return THC_OK;
+  release();
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_last_error(hipGetErrorString(e), __FILE__, __LINE__);
+    return THC_ERR_NO_MEMORY;
+  }
+  base = reinterpret_cast<char*>(p);
+  cap = bytes;
+  used = 0;
+  return THC_OK;
+}
+
+void Arena::release() {
+  if (base) (void)hipFree(base);
+  base = nullptr;
+  cap = 0;
+  used = 0;
+}
+
+int ensure_pinned(thc_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->pinned_bytes) return THC_OK;
+  if (ctx->pinned) THC_HIP(hipHostFree(ctx->pinned));
+  ctx->pinned = nullptr;
+  ctx->pinned_bytes = 0;
+  bytes = align_up(bytes, 4096);
+  THC_HIP(hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault));
+  ctx->pinned_bytes = bytes;
+  return THC_OK;
+}
+
+namespace {
+
+// C[mu][nu] = Z[mu][idx[nu]]
+__global__ __launch_bounds__(256) void gather_cols(const double* z, long ldz, const int* idx, int k, double* c) {
+  const long total = (long)k * k;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += 256L * gridDim.x) {
+    const long r = e / k, q = e % k;
+    const long col = idx[q];
+    c[e] = (col >= 0 && col < ldz) ? z[r * ldz + col] : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void to_f32(const double* x, float* y, long count) {
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < count; e += 256L * gridDim.x) y[e] = (float)x[e];
+}
+
+__global__ __launch_bounds__(256) void copy_f64(const double* x, double* y, long count) {
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < count; e += 256L * gridDim.x) y[e] = x[e];
+}
+
+__global__ void check_idx(const int* idx, int k, long n, int* bad) {
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < k; j += 256 * gridDim.x)
+    if (idx[j] < 0 || idx[j] >= n) atomicOr(bad, 1);
+}
+
+inline int grid_for(long count) { return (int)std::min<long>(8192, std::max<long>(1, (count + 255) / 256)); }
+
+struct IsdfBufs {
+  double *z, *x, *c, *l, *ci, *r, *v;
+  void* b;      // weighted half spectrum (f64 or f32)
+  float* x32;   // FP32 copy of X for the FP32 FFT
+  void *chol_w, *inv_w, *fft_w, *syrk_w;
+  int* info;
+};
+
+size_t isdf_bytes(long n, int k, const int fft[3], int prec, int refine, bool own_x) {
+  const long g2 = half_spectrum_len(fft);
+  const size_t kk = (size_t)k * k;
+  size_t b = 0;
+  b += carve_bytes<double>((size_t)k * n);                   // z
+  if (own_x) b += carve_bytes<double>((size_t)k * n);        // x
+  b += 4 * carve_bytes<double>(kk);                          // c, l, ci, v
+  if (refine > 0) b += carve_bytes<double>((size_t)k * n);   // r
+  b += prec == 0 ? carve_bytes<double>((size_t)k * g2) : carve_bytes<float>((size_t)k * g2);
+  if (prec != 0) b += carve_bytes<float>((size_t)k * n);
+  b += chol_work_bytes(k) + 256 + spd_inverse_work_bytes(k) + 256 + fft_work_bytes(k, fft, prec) + 256 +
+       syrk_work_bytes(k, g2, prec) + 256;
+  b += carve_bytes<int>(16);
+  return b;
+}
+
+bool isdf_carve(Arena& a, long n, int k, const int fft[3], int prec, int refine, double* xi, IsdfBufs& B) {
+  const long g2 = half_spectrum_len(fft);
+  const size_t kk = (size_t)k * k;
+  B = IsdfBufs{};
+  B.z = a.take<double>((size_t)k * n);
+  B.x = xi ? xi : a.take<double>((size_t)k * n);
+  B.c = a.take<double>(kk);
+  B.l = a.take<double>(kk);
+  B.ci = a.take<double>(kk);
+  B.v = a.take<double>(kk);
+  B.r = refine > 0 ? a.take<double>((size_t)k * n) : nullptr;
+  if (prec == 0) B.b = a.take<double>((size_t)k * g2);
+  else B.b = a.take<float>((size_t)k * g2);
+  B.x32 = prec != 0 ? a.take<float>((size_t)k * n) : nullptr;
+  B.chol_w = a.take<char>(chol_work_bytes(k));
+  B.inv_w = a.take<char>(spd_inverse_work_bytes(k));
+  B.fft_w = a.take<char>(fft_work_bytes(k, fft, prec));
+  B.syrk_w = a.take<char>(syrk_work_bytes(k, g2, prec));
+  B.info = a.take<int>(16);
+  return B.z && B.x && B.c && B.l && B.ci && B.v && (refine <= 0 || B.r) && B.b && (prec == 0 || B.x32) &&
+         B.chol_w && B.inv_w && B.fft_w && B.syrk_w && B.info;
+}
+
+__global__ void zero_ints(int* p, int count) {
+  if (threadIdx.x < count) p[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+// IVecs ctor + kernel at one rank.
+int isdf_run(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int k, const int fft[3],
+             const double* coulG, const thc_isdf_opts* opts, double* v_out, double* luv_out, double* xi_out,
+             thc_isdf_stats* stats) {
+  if (!ctx || !aoR || !idx || !fft || !coulG || !v_out || n < 1 || m < 1 || k < 1 || k > n) return THC_ERR_ARG;
+  const int prec = opts ? opts->precision : THC_PREC_FP64;
+  const int refine = opts ? std::max(0, opts->refine_steps) : 0;
+  const int want_luv = opts ? opts->want_luv : 1;
+  if (prec != THC_PREC_FP64 && prec != THC_PREC_FP32) return THC_ERR_UNSUPPORTED;
+  const long nfft = (long)fft[0] * fft[1] * fft[2];
+  if (fft[0] < 1 || fft[1] < 1 || fft[2] < 1) return THC_ERR_ARG;
+  // The reference only warns when prod(fft_grid) != N (interpolating_vectors.cpp:368-370) and then
+  // transforms N-point columns with the wrong shape; here that case is refused.
+  if (nfft != n) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  THC_TRY(ctx->ws.reserve(isdf_bytes(n, k, fft, prec, refine, xi_out == nullptr)));
+  ctx->ws.reset();
+  IsdfBufs B;
+  if (!isdf_carve(ctx->ws, n, k, fft, prec, refine, xi_out, B)) return THC_ERR_NO_MEMORY;
+  const long g2 = half_spectrum_len(fft);
+  hipEvent_t* ev = ctx->ev;
+
+  hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, B.info, 16);
+  hipLaunchKernelGGL(check_idx, dim3(grid_for(k)), dim3(256), 0, s, idx, k, n, B.info + 2);
+  THC_HIP(hipEventRecord(ev[0], s));
+  // 1. ZC^T = (Phi_mu Phi^T)^2: A rows gathered by idx, epilogue squares
+  {
+    GemmArgs g = {};
+    g.m = k; g.n = n; g.k = m;
+    g.a = aoR; g.lda = m; g.a_trans = 0; g.a_rows = idx; g.a_nrows = n;
+    g.b = aoR; g.ldb = m; g.b_trans = 1;
+    g.c = B.z; g.ldc = n;
+    g.alpha = 1.0; g.beta = 0.0; g.epi_square = 1; g.batch = 1;
+    THC_TRY(gemm_f64(g, s));
+  }
+  THC_HIP(hipEventRecord(ev[1], s));
+  // 2. CC^T = ZC^T[:, idx]; Cholesky; explicit inverse
+  hipLaunchKernelGGL(gather_cols, dim3(grid_for((long)k * k)), dim3(256), 0, s, B.z, n, idx, k, B.c);
+  hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * k)), dim3(256), 0, s, B.c, B.l, (long)k * k);
+  THC_HIP(hipGetLastError());
+  THC_TRY(cholesky_f64(B.l, k, k, B.info, B.chol_w, s));
+  THC_TRY(spd_inverse_from_chol(B.l, k, k, B.ci, B.inv_w, s));
+  THC_HIP(hipEventRecord(ev[2], s));
+  // 3. X = CC^T^-1 ZC^T
+  {
+    GemmArgs g = {};
+    g.m = k; g.n = n; g.k = k;
+    g.a = B.ci; g.lda = k; g.a_trans = 0;
+    g.b = B.z; g.ldb = n; g.b_trans = 0;
+    g.c = B.x; g.ldc = n;
+    g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
+    THC_TRY(gemm_f64(g, s));
+  }
+  THC_HIP(hipEventRecord(ev[3], s));
+  // 3b. FP64 iterative refinement: R = Z - C X; X += C^-1 R
+  for (int it = 0; it < refine; ++it) {
+    hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * n)), dim3(256), 0, s, B.z, B.r, (long)k * n);
+    GemmArgs g = {};
+    g.m = k; g.n = n; g.k = k;
+    g.a = B.c; g.lda = k; g.a_trans = 0;
+    g.b = B.x; g.ldb = n; g.b_trans = 0;
+    g.c = B.r; g.ldc = n;
+    g.alpha = -1.0; g.beta = 1.0; g.batch = 1;
+    THC_TRY(gemm_f64(g, s));
+    GemmArgs h = g;
+    h.a = B.ci; h.b = B.r; h.c = B.x; h.alpha = 1.0; h.beta = 1.0;
+    THC_TRY(gemm_f64(h, s));
+  }
+  THC_HIP(hipEventRecord(ev[4], s));
+  // 4. weighted half spectrum
+  if (prec == THC_PREC_FP64) {
+    THC_TRY(fft_coulomb(B.x, k, fft, coulG, B.b, 0, B.fft_w, s));
+  } else {
+    hipLaunchKernelGGL(to_f32, dim3(grid_for((long)k * n)), dim3(256), 0, s, B.x, B.x32, (long)k * n);
+    THC_TRY(fft_coulomb(B.x32, k, fft, coulG, B.b, 1, B.fft_w, s));
+  }
+  THC_HIP(hipEventRecord(ev[5], s));
+  // 5. V = B B^T, FP64 sum of the split-K partials
+  THC_TRY(syrk_splitk(B.b, k, g2, g2, prec == THC_PREC_FP64 ? 0 : 1, v_out, k, B.syrk_w, s));
+  THC_HIP(hipEventRecord(ev[6], s));
+  // 6. Luv = chol(V)
+  if (want_luv && luv_out) {
+    hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * k)), dim3(256), 0, s, v_out, luv_out, (long)k * k);
+    THC_TRY(cholesky_f64(luv_out, k, k, B.info + 1, B.chol_w, s));
+  }
+  THC_HIP(hipEventRecord(ev[7], s));
+  int info[4] = {0, 0, 0, 0};
+  THC_HIP(hipMemcpyAsync(info, B.info, sizeof(info), hipMemcpyDeviceToHost, s));
+  THC_HIP(hipStreamSynchronize(s));
+  if (stats) {
+    stats->cct_chol_info = info[0];
+    stats->luv_chol_info = info[1];
+    float t;
+    const int map[6][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {5, 6}, {6, 7}};
+    for (int q = 0; q < 6; ++q) {
+      THC_HIP(hipEventElapsedTime(&t, ev[map[q][0]], ev[map[q][1]]));
+      stats->stage_ms[q] = t;
+    }
+    THC_HIP(hipEventElapsedTime(&t, ev[0], ev[7]));
+    stats->stage_ms[6] = t;
+    THC_HIP(hipEventElapsedTime(&t, ev[3], ev[4]));
+    stats->stage_ms[7] = t;
+  }
+  if (info[2]) return THC_ERR_ARG;
+  if (info[0]) return THC_ERR_NOT_POSDEF;
+  if (info[1] && want_luv && luv_out) return THC_WARN_LUV_NOT_POSDEF;
+  return THC_OK;
+}
+
+}  // namespace thc
+
+// ======================================================================== C ABI
+
+using namespace thc;
+
+extern "C" {
+
+const char* thc_version(void) { return "thcpp_amd 0.1 (gfx950)"; }
+
+const char* thc_last_error(void) { return g_last_error.c_str(); }
+
+const char* thc_status_string(int st) {
+  switch (st) {
+    case THC_OK: return "ok";
+    case THC_ERR_ARG: return "bad argument";
+    case THC_ERR_HIP: return "HIP runtime error";
+    case THC_ERR_EMPTY_CLUSTER: return "CRITICAL ERROR: Found centroid with weight ~ 0";
+    case THC_ERR_NOT_POSDEF: return "CC^T is not numerically positive definite";
+    case THC_ERR_NO_MEMORY: return "device workspace allocation failed";
+    case THC_ERR_UNSUPPORTED: return "unsupported size";
+    case THC_WARN_DUPLICATE_INDICES: return "WARNING: Found repeated interpolation indices";
+    case THC_WARN_LUV_NOT_POSDEF: return "WARNING: Cholesky decomposition of Muv failed";
+    case THC_WARN_FFT_GRID_MISMATCH: return "WARNING: FFT grid not consistent with number of real space grid points";
+    default: return "unknown status";
+  }
+}
+
+int thc_ctx_create(int device, void* stream, thc_ctx** out) {
+  if (!out) return THC_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  THC_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(device));
+  thc_ctx* c = new thc_ctx();
+  c->device = device;
+  if (stream) {
+    c->stream = reinterpret_cast<hipStream_t>(stream);
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      set_last_error(hipGetErrorString(e), __FILE__, __LINE__);
+      return THC_ERR_HIP;
+    }
+    c->own_stream = true;
+  }
+  for (auto& e : c->ev) {
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) {
+      set_last_error(hipGetErrorString(r), __FILE__, __LINE__);
+      return THC_ERR_HIP;
+    }
+  }
+  *out = c;
+  return THC_OK;
+}
+
+int thc_ctx_destroy(thc_ctx* c) {
+  if (!c) return THC_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->kmgraph) {
+    c->kmgraph->reset();
+    delete c->kmgraph;
+  }
+  c->ws.release();
+  c->kmws.release();
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return THC_OK;
+}
+
+int thc_ctx_set_graphs(thc_ctx* c, int enable) {
+  if (!c) return THC_ERR_ARG;
+  c->use_graphs = enable != 0;
+  return THC_OK;
+}
+
+int thc_kmeans_initial_indices(long n, int k, unsigned int seed, int* host_idx_out) {
+  if (thc_seed_indices_impl(n, k, seed, host_idx_out)) return THC_ERR_ARG;
+  return THC_OK;
+}
+
+int thc_classify_grid_points(thc_ctx* ctx, const double* grid, long n, const double* centroids, int k,
+                             int* grid_map) {
+  return kmeans_classify(ctx, grid, n, centroids, k, grid_map);
+}
+
+int thc_update_centroids(thc_ctx* ctx, const double* rho, const double* grid, long n, const int* grid_map, int k,
+                         double* centroids) {
+  return kmeans_update(ctx, rho, grid, n, grid_map, k, centroids);
+}
+
+int thc_map_to_grid(thc_ctx* ctx, const double* grid, long n, const double* centroids, int k, int* interp_idx) {
+  return kmeans_map_to_grid(ctx, grid, n, centroids, k, interp_idx);
+}
+
+int thc_kmeans(thc_ctx* ctx, const double* grid, const double* rho, long n, int k, int max_it, double threshold,
+               unsigned int seed, const int* host_init_idx, int* interp_idx, int* grid_map, double* centroids,
+               thc_kmeans_stats* stats) {
+  if (!ctx || n < 1 || k < 1 || k > n) return THC_ERR_ARG;
+  const int* init = host_init_idx;
+  std::vector<int> tmp;
+  if (!init) {
+    tmp.resize(k);
+    if (thc_seed_indices_impl(n, k, seed, tmp.data())) return THC_ERR_ARG;
+    init = tmp.data();
+  }
+  return kmeans_run(ctx, grid, rho, n, k, max_it, threshold, init, interp_idx, grid_map, centroids, stats);
+}
+
+int thc_czt(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int k, double* czt) {
+  if (!ctx || !aoR || !idx || !czt || n < 1 || m < 1 || k < 1) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  GemmArgs g = {};
+  g.m = k; g.n = n; g.k = m;
+  g.a = aoR; g.lda = m; g.a_trans = 0; g.a_rows = idx; g.a_nrows = n;
+  g.b = aoR; g.ldb = m; g.b_trans = 1;
+  g.c = czt; g.ldc = n;
+  g.alpha = 1.0; g.beta = 0.0; g.epi_square = 1; g.batch = 1;
+  THC_TRY(gemm_f64(g, ctx->stream));
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  return THC_OK;
+}
+
+int thc_cct(thc_ctx* ctx, const double* czt, long n, const int* idx, int k, double* cct) {
+  if (!ctx || !czt || !idx || !cct || n < 1 || k < 1) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(gather_cols, dim3(grid_for((long)k * k)), dim3(256), 0, ctx->stream, czt, n, idx, k, cct);
+  THC_HIP(hipGetLastError());
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  return THC_OK;
+}
+
+int thc_cholesky(thc_ctx* ctx, double* a, int k, int* info) {
+  if (!ctx || !a || k < 1) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  THC_TRY(ctx->ws.reserve(chol_work_bytes(k) + carve_bytes<int>(4)));
+  ctx->ws.reset();
+  void* w = ctx->ws.take<char>(chol_work_bytes(k));
+  int* d_info = ctx->ws.take<int>(4);
+  if (!w || !d_info) return THC_ERR_NO_MEMORY;
+  hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, ctx->stream, d_info, 4);
+  THC_TRY(cholesky_f64(a, k, k, d_info, w, ctx->stream));
+  int h = 0;
+  THC_HIP(hipMemcpyAsync(&h, d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  if (info) *info = h;
+  return THC_OK;
+}
+
+int thc_least_squares(thc_ctx* ctx, const double* cct, const double* czt, long n, int k, int refine_steps,
+                      double* x, int* info) {
+  if (!ctx || !cct || !czt || !x || n < 1 || k < 1) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t kk = (size_t)k * k;
+  size_t need = 2 * carve_bytes<double>(kk) + chol_work_bytes(k) + spd_inverse_work_bytes(k) + 1024 +
+                carve_bytes<int>(4);
+  if (refine_steps > 0) need += carve_bytes<double>((size_t)k * n);
+  THC_TRY(ctx->ws.reserve(need));
+  ctx->ws.reset();
+  double* l = ctx->ws.take<double>(kk);
+  double* ci = ctx->ws.take<double>(kk);
+  void* cw = ctx->ws.take<char>(chol_work_bytes(k));
+  void* iw = ctx->ws.take<char>(spd_inverse_work_bytes(k));
+  int* d_info = ctx->ws.take<int>(4);
+  double* r = refine_steps > 0 ? ctx->ws.take<double>((size_t)k * n) : nullptr;
+  if (!l || !ci || !cw || !iw || !d_info || (refine_steps > 0 && !r)) return THC_ERR_NO_MEMORY;
+  hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, d_info, 4);
+  hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)kk)), dim3(256), 0, s, cct, l, (long)kk);
+  THC_TRY(cholesky_f64(l, k, k, d_info, cw, s));
+  THC_TRY(spd_inverse_from_chol(l, k, k, ci, iw, s));
+  GemmArgs g = {};
+  g.m = k; g.n = n; g.k = k;
+  g.a = ci; g.lda = k; g.b = czt; g.ldb = n; g.c = x; g.ldc = n;
+  g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
+  THC_TRY(gemm_f64(g, s));
+  for (int it = 0; it < refine_steps; ++it) {
+    hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * n)), dim3(256), 0, s, czt, r, (long)k * n);
+    GemmArgs q = g;
+    q.a = cct; q.b = x; q.c = r; q.alpha = -1.0; q.beta = 1.0;
+    THC_TRY(gemm_f64(q, s));
+    GemmArgs u = g;
+    u.a = ci; u.b = r; u.c = x; u.alpha = 1.0; u.beta = 1.0;
+    THC_TRY(gemm_f64(u, s));
+  }
+  int h = 0;
+  THC_HIP(hipMemcpyAsync(&h, d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+  THC_HIP(hipStreamSynchronize(s));
+  if (info) *info = h;
+  return h ? THC_ERR_NOT_POSDEF : THC_OK;
+}
+
+int thc_fft_vectors(thc_ctx* ctx, const double* x, int k, const int fft_grid[3], const double* coulG,
+                    double* out) {
+  if (!ctx || !x || !fft_grid || !coulG || !out || k < 1) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  THC_TRY(ctx->ws.reserve(fft_work_bytes(k, fft_grid, 0) + 256));
+  ctx->ws.reset();
+  void* w = ctx->ws.take<char>(fft_work_bytes(k, fft_grid, 0));
+  if (!w) return THC_ERR_NO_MEMORY;
+  THC_TRY(fft_coulomb(x, k, fft_grid, coulG, out, 0, w, ctx->stream));
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  return THC_OK;
+}
+
+int thc_construct_muv(thc_ctx* ctx, const double* b, int k, long g2, double* v) {
+  if (!ctx || !b || !v || k < 1 || g2 < 1) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  THC_TRY(ctx->ws.reserve(syrk_work_bytes(k, g2, 0) + 256));
+  ctx->ws.reset();
+  void* w = ctx->ws.take<char>(syrk_work_bytes(k, g2, 0));
+  if (!w) return THC_ERR_NO_MEMORY;
+  THC_TRY(syrk_splitk(b, k, g2, g2, 0, v, k, w, ctx->stream));
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  return THC_OK;
+}
+
+int thc_isdf(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int k, const int fft_grid[3],
+             const double* coulG, const thc_isdf_opts* opts, double* v, double* luv, double* xi,
+             thc_isdf_stats* stats) {
+  return isdf_run(ctx, aoR, n, m, idx, k, fft_grid, coulG, opts, v, luv, xi, stats);
+}
+
+int thc_factorize(thc_ctx* ctx, const double* grid, const double* rho, const double* aoR, long n, int m,
+                  int thc_cfac, int max_it, double threshold, unsigned int seed, const int fft_grid[3],
+                  const double* coulG, const thc_isdf_opts* opts, int* interp_idx, double* v, double* luv,
+                  thc_kmeans_stats* kstats, thc_isdf_stats* istats) {
+  if (!ctx || thc_cfac < 1 || m < 1) return THC_ERR_ARG;
+  const long k = (long)thc_cfac * m;
+  if (k > n || k > 0x7fffffffL) return THC_ERR_ARG;
+  const int st = thc_kmeans(ctx, grid, rho, n, (int)k, max_it, threshold, seed, nullptr, interp_idx, nullptr,
+                            nullptr, kstats);
+  if (st != THC_OK && st < 100) return st;
+  const int st2 = isdf_run(ctx, aoR, n, m, interp_idx, (int)k, fft_grid, coulG, opts, v, luv, nullptr, istats);
+  if (st2 != THC_OK) return st2;
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Internal definitions shared by the HIP translation units of libthcpp_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/thcpp_amd.h"
+
+namespace thc {
+
+void set_last_error(const char* what, const char* file, int line);
+
+#define THC_HIP(expr)                                                   \
+  do {                                                                  \
+    hipError_t e_ = (expr);                                             \
+    if (e_ != hipSuccess) {                                             \
+      ::thc::set_last_error(hipGetErrorString(e_), __FILE__, __LINE__); \
+      return THC_ERR_HIP;                                               \
+    }                                                                   \
+  } while (0)
+
+#define THC_TRY(expr)          \
+  do {                         \
+    int s_ = (expr);           \
+    if (s_ != THC_OK && s_ < 100) return s_; \
+  } while (0)
+
+// Bump allocator over one device allocation.  reset() forgets every carve;
+// reserve() grows the backing store (freeing the old one) when a call needs
+// more than it holds.  Carves are 256-byte aligned.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  int reserve(size_t bytes);
+  void reset() { used = 0; }
+  template <class T>
+  T* take(size_t count) {
+    size_t off = (used + 255) & ~size_t(255);
+    size_t bytes = count * sizeof(T);
+    if (off + bytes > cap) return nullptr;
+    used = off + bytes;
+    return reinterpret_cast<T*>(base + off);
+  }
+  void release();
+};
+
+}  // namespace thc
+
+struct KmGraphCache;
+
+struct thc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  bool use_graphs = true;
+  thc::Arena ws;       // per-call scratch
+  thc::Arena kmws;     // k-means state (kept across calls for graph replay)
+  KmGraphCache* kmgraph = nullptr;
+  hipEvent_t ev[10] = {};
+  void* pinned = nullptr;  // small pinned host buffer for status read-back
+  size_t pinned_bytes = 0;
+};
+
+namespace thc {
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Byte count a carve of 'count' T's takes in an arena (with alignment slack).
+template <class T>
+inline size_t carve_bytes(size_t count) { return align_up(count * sizeof(T), 256) + 256; }
+
+int ensure_pinned(thc_ctx* ctx, size_t bytes);
+
+// ---- GEMM (gemm.hip) ------------------------------------------------------
+// C[m, n] = alpha * sum_k op(A)[m, k] op(B)[k, n] + beta * C[m, n]
+//   a_trans = 0: A[m * lda + k]  (row-major m x k)   1: A[k * lda + m]
+//   b_trans = 0: B[k * ldb + n]  (row-major k x n)   1: B[n * ldb + k]
+//   a_rows: optional row gather for a_trans = 0 (row m of op(A) = row a_rows[m] of A)
+//   epi_square: C = (alpha * acc)^2 (beta ignored)
+//   lower: only tiles on/below the diagonal are computed (m, n square)
+//   batch: blockIdx.z, strides in elements
+struct GemmArgs {
+  long m, n, k;
+  const void* a; long lda; int a_trans; const int* a_rows;
+  long a_nrows;  // rows of A addressable through a_rows (gathered rows outside [0, a_nrows) read as 0)
+  const void* b; long ldb; int b_trans;
+  void* c; long ldc;
+  double alpha, beta;
+  int epi_square;
+  int lower;
+  int batch;
+  long stride_a, stride_b, stride_c;
+};
+int gemm_f64(const GemmArgs& g, hipStream_t s);
+int gemm_f32(const GemmArgs& g, hipStream_t s);
+// Lower-triangle SYRK with split inner dimension and FP64 reduction:
+// V = A A^T for A (m x k row-major, lda), written full symmetric into v (ldv).
+// A is double (prec 0) or float (prec 1); partials live in 'work'.
+size_t syrk_work_bytes(long m, long k, int prec);
+int syrk_splitk(const void* a, long m, long k, long lda, int prec, double* v, long ldv,
+                void* work, hipStream_t s);
+
+// ---- dense FP64 factorisations (linalg.hip) --------------------------------
+size_t chol_work_bytes(int k);
+// In-place lower Cholesky (row-major, lda); info written to device int.
+int cholesky_f64(double* a, int k, long lda, int* d_info, void* work, hipStream_t s);
+size_t spd_inverse_work_bytes(int k);
+// ainv = A^-1 from its lower Cholesky factor l (k x k, lda).
+int spd_inverse_from_chol(const double* l, int k, long lda, double* ainv, void* work, hipStream_t s);
+int zero_upper(double* a, int k, long lda, hipStream_t s);
+int symmetrize_from_lower(double* a, int k, long lda, hipStream_t s);
+
+// ---- FFT (fft.hip) ----------------------------------------------------------
+size_t fft_work_bytes(int k, const int n[3], int prec);
+// x: k x n0 n1 n2 (double if prec 0, float if prec 1); out: k x g2 weighted half spectrum
+int fft_coulomb(const void* x, int k, const int n[3], const double* coulG, void* out, int prec,
+                void* work, hipStream_t s);
+inline long half_spectrum_len(const int n[3]) { return 2L * n[0] * n[1] * (n[2] / 2 + 1); }
+
+}  // namespace thc

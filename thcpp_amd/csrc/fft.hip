@@ -1,0 +1,263 @@
+// Batched 3-D real-to-complex FFT of the interpolating vectors with the
+// Coulomb weighting fused into the last pass (replaces the per-column FFTW
+// loop and the sqrt(coulG) scaling of the reference,
+// src/interpolating_vectors.cpp:371-383 and 258-263).
+//
+// Convention: unnormalised forward transform, X(k) = sum_r x(r) e^{-2 pi i k.r/n},
+// over the row-major grid (n0, n1, n2), z fastest.  Only kz <= n2/2 is kept
+// (x is real).  Output row mu holds, for every kept G = (kx, ky, kz) in C
+// order, (Re, Im) * sqrt(w coulG(G)) with w = 1 when kz = 0 or 2 kz = n2
+// (those planes hold both members of each conjugate pair) and 2 otherwise, so
+// sum_G conj(A_mu(G)) coulG(G) A_nu(G) over the full grid equals the real dot
+// product of rows mu and nu.
+//
+// Pass A, one workgroup per (mu, x-plane): two real z-rows are packed as one
+// complex row, transformed along z, unpacked into the half spectrum, then
+// transformed along y.  Pass B, one workgroup per (mu, y-index): transform along
+// x, weight, store.  Both passes run a Stockham autosort FFT in LDS with radix
+// 4, 2, 3 and 5 stages; twiddles come from a table of e^{-2 pi i j/n}.
+#include <algorithm>
+
+#include "common.h"
+
+namespace thc {
+namespace {
+
+template <class T> struct C2 { T x, y; };
+
+template <class T>
+__device__ inline C2<T> cmul(C2<T> a, C2<T> b) {
+  return C2<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <class T>
+__device__ inline C2<T> cadd(C2<T> a, C2<T> b) { return C2<T>{a.x + b.x, a.y + b.y}; }
+template <class T>
+__device__ inline C2<T> csub(C2<T> a, C2<T> b) { return C2<T>{a.x - b.x, a.y - b.y}; }
+
+constexpr int MAXR = 8;  // max stages
+
+struct Plan1D {
+  int n;
+  int nst;
+  int rad[MAXR];
+};
+
+struct FftParams {
+  int k, n0, n1, n2, n2h;
+  Plan1D pz, py, px;
+  const void* twz; const void* twy; const void* twx;
+};
+
+// One Stockham stage over 'nlines' lines of length n, read src write dst.
+template <class T, int R>
+__device__ inline void stage(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, int n, int ns, int nlines,
+                             int es, int ls, const C2<T>* __restrict__ tw) {
+  const int nb = n / R;
+  const int total = nlines * nb;
+  const int tstep = n / (ns * R);
+  for (int w = threadIdx.x; w < total; w += blockDim.x) {
+    const int line = w / nb, t = w % nb;
+    const int base = line * ls;
+    const int j = t % ns;
+    C2<T> x[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) x[q] = src[base + (t + q * nb) * es];
+    if (ns > 1) {
+#pragma unroll
+      for (int q = 1; q < R; ++q) x[q] = cmul(x[q], tw[j * q * tstep]);
+    }
+    C2<T> y[R];
+    if (R == 2) {
+      y[0] = cadd(x[0], x[1]);
+      y[1] = csub(x[0], x[1]);
+    } else if (R == 4) {
+      const C2<T> a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+      const C2<T> c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
+      y[0] = cadd(a, c);
+      y[2] = csub(a, c);
+      y[1] = C2<T>{b.x + d.y, b.y - d.x};  // b - i d
+      y[3] = C2<T>{b.x - d.y, b.y + d.x};  // b + i d
+    } else {
+      const int step = n / R;
+#pragma unroll
+      for (int p = 0; p < R; ++p) {
+        C2<T> acc = x[0];
+#pragma unroll
+        for (int q = 1; q < R; ++q) acc = cadd(acc, cmul(x[q], tw[((p * q) % R) * step]));
+        y[p] = acc;
+      }
+    }
+    const int d0 = (t / ns) * ns * R + j;
+#pragma unroll
+    for (int q = 0; q < R; ++q) dst[base + (d0 + q * ns) * es] = y[q];
+  }
+}
+
+// Full 1-D transforms of 'nlines' lines; returns the buffer holding the result.
+template <class T>
+__device__ C2<T>* fft_lines(C2<T>* a, C2<T>* b, const Plan1D& pl, int nlines, int es, int ls, const C2<T>* tw) {
+  int ns = 1;
+  for (int s = 0; s < pl.nst; ++s) {
+    const int r = pl.rad[s];
+    if (r == 4) stage<T, 4>(a, b, pl.n, ns, nlines, es, ls, tw);
+    else if (r == 2) stage<T, 2>(a, b, pl.n, ns, nlines, es, ls, tw);
+    else if (r == 3) stage<T, 3>(a, b, pl.n, ns, nlines, es, ls, tw);
+    else stage<T, 5>(a, b, pl.n, ns, nlines, es, ls, tw);
+    __syncthreads();
+    ns *= r;
+    C2<T>* t = a; a = b; b = t;
+  }
+  return a;
+}
+
+// x: (k, n0, n1, n2) real; t: (k, n0, n1, n2h) complex.
+template <class T>
+__global__ __launch_bounds__(256) void pass_a(const T* __restrict__ x, C2<T>* __restrict__ t, FftParams p) {
+  extern __shared__ unsigned char smem_raw[];
+  const int n1 = p.n1, n2 = p.n2, n2h = p.n2h;
+  const int n1p = (n1 + 1) / 2;
+  const int cap = max(n1p * n2, n1 * n2h);
+  C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
+  C2<T>* bufb = bufa + cap;
+  const long mu = blockIdx.y, ix = blockIdx.x;
+  const T* src = x + (mu * p.n0 + ix) * (long)n1 * n2;
+  // pack rows (2r, 2r+1) as re/im
+  for (int e = threadIdx.x; e < n1p * n2; e += blockDim.x) {
+    const int r = e / n2, z = e % n2;
+    const int y0 = 2 * r, y1 = 2 * r + 1;
+    const T re = src[(long)y0 * n2 + z];
+    const T im = (y1 < n1) ? src[(long)y1 * n2 + z] : T(0);
+    bufa[e] = C2<T>{re, im};
+  }
+  __syncthreads();
+  const C2<T>* twz = reinterpret_cast<const C2<T>*>(p.twz);
+  C2<T>* res = fft_lines<T>(bufa, bufb, p.pz, n1p, 1, n2, twz);
+  C2<T>* h = (res == bufa) ? bufb : bufa;
+  // unpack: A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / (2i)
+  for (int e = threadIdx.x; e < n1p * n2h; e += blockDim.x) {
+    const int r = e / n2h, kz = e % n2h;
+    const C2<T> z1 = res[r * n2 + kz];
+    const C2<T> z2 = res[r * n2 + (n2 - kz) % n2];
+    const C2<T> a = C2<T>{(z1.x + z2.x) * T(0.5), (z1.y - z2.y) * T(0.5)};
+    const C2<T> bb = C2<T>{(z1.y + z2.y) * T(0.5), (z2.x - z1.x) * T(0.5)};
+    h[(2 * r) * n2h + kz] = a;
+    if (2 * r + 1 < n1) h[(2 * r + 1) * n2h + kz] = bb;
+  }
+  __syncthreads();
+  const C2<T>* twy = reinterpret_cast<const C2<T>*>(p.twy);
+  C2<T>* other = (h == bufa) ? bufb : bufa;
+  C2<T>* fin = fft_lines<T>(h, other, p.py, n2h, n2h, 1, twy);
+  C2<T>* dst = t + (mu * p.n0 + ix) * (long)n1 * n2h;
+  for (int e = threadIdx.x; e < n1 * n2h; e += blockDim.x) dst[e] = fin[e];
+}
+
+// t: (k, n0, n1, n2h) complex -> out (k, n0, n1, n2h, 2) weighted.
+template <class T>
+__global__ __launch_bounds__(256) void pass_b(const C2<T>* __restrict__ t, const double* __restrict__ coulG,
+                                              T* __restrict__ out, FftParams p) {
+  extern __shared__ unsigned char smem_raw[];
+  const int n0 = p.n0, n1 = p.n1, n2 = p.n2, n2h = p.n2h;
+  C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
+  C2<T>* bufb = bufa + n0 * n2h;
+  const long mu = blockIdx.y;
+  const int iy = blockIdx.x;
+  const C2<T>* src = t + mu * (long)n0 * n1 * n2h;
+  for (int e = threadIdx.x; e < n0 * n2h; e += blockDim.x) {
+    const int ix = e / n2h, kz = e % n2h;
+    bufa[e] = src[((long)ix * n1 + iy) * n2h + kz];
+  }
+  __syncthreads();
+  const C2<T>* twx = reinterpret_cast<const C2<T>*>(p.twx);
+  C2<T>* fin = fft_lines<T>(bufa, bufb, p.px, n2h, n2h, 1, twx);
+  T* dst = out + mu * (long)n0 * n1 * n2h * 2;
+  for (int e = threadIdx.x; e < n0 * n2h; e += blockDim.x) {
+    const int kx = e / n2h, kz = e % n2h;
+    const long g = ((long)kx * n1 + iy) * n2 + kz;
+    const double w = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
+    const T f = (T)sqrt(w * coulG[g]);
+    const C2<T> v = fin[e];
+    const long o = (((long)kx * n1 + iy) * n2h + kz) * 2;
+    dst[o] = v.x * f;
+    dst[o + 1] = v.y * f;
+  }
+}
+
+bool make_plan(int n, Plan1D& pl) {
+  pl.n = n;
+  pl.nst = 0;
+  int m = n;
+  while (m % 4 == 0 && pl.nst < MAXR) { pl.rad[pl.nst++] = 4; m /= 4; }
+  while (m % 2 == 0 && pl.nst < MAXR) { pl.rad[pl.nst++] = 2; m /= 2; }
+  while (m % 3 == 0 && pl.nst < MAXR) { pl.rad[pl.nst++] = 3; m /= 3; }
+  while (m % 5 == 0 && pl.nst < MAXR) { pl.rad[pl.nst++] = 5; m /= 5; }
+  return m == 1;
+}
+
+// e^{-2 pi i j / n}, j < n, from sincospi (about 1 ulp in double).
+template <class T>
+__global__ void twiddle_kernel(int n, C2<T>* tw) {
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double sn, cs;
+    sincospi(-2.0 * (double)j / (double)n, &sn, &cs);
+    tw[j] = C2<T>{(T)cs, (T)sn};
+  }
+}
+
+constexpr int MAXN = 128;
+
+template <class T>
+size_t lds_a(int n1, int n2) {
+  const int n2h = n2 / 2 + 1, n1p = (n1 + 1) / 2;
+  return 2 * sizeof(C2<T>) * (size_t)std::max(n1p * n2, n1 * n2h);
+}
+template <class T>
+size_t lds_b(int n0, int n2) {
+  return 2 * sizeof(C2<T>) * (size_t)n0 * (n2 / 2 + 1);
+}
+
+template <class T>
+int run(const void* x, int k, const int n[3], const double* coulG, void* out, void* work, hipStream_t s) {
+  FftParams p;
+  p.k = k; p.n0 = n[0]; p.n1 = n[1]; p.n2 = n[2]; p.n2h = n[2] / 2 + 1;
+  if (!make_plan(n[2], p.pz) || !make_plan(n[1], p.py) || !make_plan(n[0], p.px)) return THC_ERR_UNSUPPORTED;
+  const size_t la = lds_a<T>(n[1], n[2]), lb = lds_b<T>(n[0], n[2]);
+  if (la > 160 * 1024 || lb > 160 * 1024) return THC_ERR_UNSUPPORTED;
+  // twiddle tables at the head of the workspace, then the intermediate spectrum
+  char* w = reinterpret_cast<char*>(work);
+  const size_t tb = align_up(sizeof(T) * 2 * MAXN, 256);
+  C2<T>* dz = reinterpret_cast<C2<T>*>(w);
+  C2<T>* dy = reinterpret_cast<C2<T>*>(w + tb);
+  C2<T>* dx = reinterpret_cast<C2<T>*>(w + 2 * tb);
+  C2<T>* tmid = reinterpret_cast<C2<T>*>(w + 3 * tb);
+  hipLaunchKernelGGL(twiddle_kernel<T>, dim3(1), dim3(128), 0, s, n[2], dz);
+  hipLaunchKernelGGL(twiddle_kernel<T>, dim3(1), dim3(128), 0, s, n[1], dy);
+  hipLaunchKernelGGL(twiddle_kernel<T>, dim3(1), dim3(128), 0, s, n[0], dx);
+  THC_HIP(hipGetLastError());
+  p.twz = dz; p.twy = dy; p.twx = dx;
+  THC_HIP(hipFuncSetAttribute((const void*)pass_a<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la));
+  THC_HIP(hipFuncSetAttribute((const void*)pass_b<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
+  hipLaunchKernelGGL(pass_a<T>, dim3(n[0], k), dim3(256), la, s, reinterpret_cast<const T*>(x), tmid, p);
+  THC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(pass_b<T>, dim3(n[1], k), dim3(256), lb, s, tmid, coulG, reinterpret_cast<T*>(out), p);
+  THC_HIP(hipGetLastError());
+  return THC_OK;
+}
+
+}  // namespace
+
+size_t fft_work_bytes(int k, const int n[3], int prec) {
+  const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
+  const size_t tb = align_up(es * 2 * MAXN, 256);
+  return 3 * tb + align_up(2 * es * (size_t)k * n[0] * n[1] * (n[2] / 2 + 1), 256) + 256;
+}
+
+int fft_coulomb(const void* x, int k, const int n[3], const double* coulG, void* out, int prec, void* work,
+                hipStream_t s) {
+  if (k <= 0) return THC_OK;
+  for (int d = 0; d < 3; ++d)
+    if (n[d] < 1 || n[d] > MAXN) return THC_ERR_UNSUPPORTED;
+  if (prec == 0) return run<double>(x, k, n, coulG, out, work, s);
+  return run<float>(x, k, n, coulG, out, work, s);
+}
+
+}  // namespace thc

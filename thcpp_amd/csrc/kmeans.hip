@@ -1,0 +1,845 @@
+// Weighted k-means (centroidal Voronoi) interpolation points on gfx950,
+// bit-identical to the reference's sequential FP64 loop (src/kmeans.cpp).
+//
+// Exactness rules kept by every kernel here:
+//  * distances are sqrt((dx*dx + dy*dy) + dz*dz) with dx = centroid - point,
+//    every multiply and add rounded separately (no FMA contraction, below), and
+//    sqrt correctly rounded (the device sqrt is; tests check it);
+//  * the argmin takes the first index among equal rounded distances
+//    (std::min_element), evaluated as a lexicographic (distance, index) minimum;
+//    sqrt is monotone, so a sqrt is only needed when d^2 improves;
+//  * centroid sums run per cluster in ascending grid index with the product
+//    rounded before the add (a stable counting sort supplies that order), and
+//    are divided once (IEEE double division);
+//  * pruning of centroids (or grid tiles) is conservative: a candidate is
+//    dropped only if its distance bound exceeds the best bound by a relative
+//    1e-8 on squares, far above the few-ulp rounding of any distance.
+#pragma clang fp contract(off)
+#include <cfloat>
+#include <climits>
+
+#include "common.h"
+#include "kmeans.h"
+
+namespace thc {
+namespace km {
+
+__device__ inline double dist2(double cx, double cy, double cz, double gx, double gy, double gz) {
+  const double dx = cx - gx;
+  const double dy = cy - gy;
+  const double dz = cz - gz;
+  return (dx * dx + dy * dy) + dz * dz;
+}
+
+// Lexicographic (rounded distance, index) update for candidates visited in
+// any order.  bd2 is the exact d^2 of the current best.
+__device__ inline void consider_any(double d2, int idx, double& bd2, double& bs, int& bi) {
+  if (d2 < bd2) {
+    const double s = sqrt(d2);
+    if (s < bs || (s == bs && idx < bi)) { bs = s; bd2 = d2; bi = idx; }
+  } else if (d2 == bd2) {
+    if (idx < bi) bi = idx;
+  } else if (idx < bi && d2 <= bd2 * (1.0 + 0x1p-50)) {
+    // d^2 slightly above the best may still round to the same distance
+    const double s = sqrt(d2);
+    if (s == bs) bi = idx;
+  }
+}
+
+__device__ inline double lb2_box(double cx, double cy, double cz, const double* b) {
+  const double gx = fmax(0.0, fmax(b[0] - cx, cx - b[3]));
+  const double gy = fmax(0.0, fmax(b[1] - cy, cy - b[4]));
+  const double gz = fmax(0.0, fmax(b[2] - cz, cz - b[5]));
+  return gx * gx + gy * gy + gz * gz;
+}
+__device__ inline double ub2_box(double cx, double cy, double cz, const double* b) {
+  const double ex = fmax(fabs(cx - b[0]), fabs(cx - b[3]));
+  const double ey = fmax(fabs(cy - b[1]), fabs(cy - b[4]));
+  const double ez = fmax(fabs(cz - b[2]), fabs(cz - b[5]));
+  return ex * ex + ey * ey + ez * ez;
+}
+constexpr double PRUNE = 1.0 + 1e-8;
+
+__device__ inline double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline int wave_sum(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// ------------------------------------------------------------------ setup
+
+__global__ __launch_bounds__(256) void bbox_partial(const double* grid, long n, double* part) {
+  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x)
+    for (int d = 0; d < 3; ++d) {
+      const double v = grid[i * 3 + d];
+      lo[d] = fmin(lo[d], v);
+      hi[d] = fmax(hi[d], v);
+    }
+  __shared__ double s[6][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int d = 0; d < 3; ++d) {
+    double a = wave_min(lo[d]), b = -wave_min(-hi[d]);
+    if (lane == 0) { s[d][w] = a; s[3 + d][w] = b; }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int d = threadIdx.x;
+    double r = s[d][0];
+    for (int q = 1; q < 4; ++q) r = d < 3 ? fmin(r, s[d][q]) : fmax(r, s[d][q]);
+    part[blockIdx.x * 6 + d] = r;
+  }
+}
+
+__global__ void bbox_final(const double* part, int nb, double* box) {
+  const int d = threadIdx.x;
+  if (d >= 6) return;
+  double r = part[d];
+  for (int b = 1; b < nb; ++b) r = d < 3 ? fmin(r, part[b * 6 + d]) : fmax(r, part[b * 6 + d]);
+  box[d] = r;
+}
+
+__device__ inline unsigned spread3(unsigned v) {
+  unsigned r = 0;
+  for (int b = 0; b < 10; ++b) r |= ((v >> b) & 1u) << (3 * b);
+  return r;
+}
+
+__global__ __launch_bounds__(256) void morton_keys(const double* grid, long n, const double* box, int cells,
+                                                   int* keys) {
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i >= n) return;
+  unsigned c[3];
+  for (int d = 0; d < 3; ++d) {
+    const double lo = box[d], ext = box[3 + d] - box[d];
+    int v = 0;
+    if (ext > 0) v = (int)((grid[i * 3 + d] - lo) / ext * cells);
+    c[d] = (unsigned)min(max(v, 0), cells - 1);
+  }
+  keys[i] = (int)((spread3(c[0]) << 2) | (spread3(c[1]) << 1) | spread3(c[2]));
+}
+
+__global__ __launch_bounds__(256) void gather_sorted(const double* grid, const int* perm, long n, double* sx,
+                                                     double* sy, double* sz) {
+  const long p = blockIdx.x * 256L + threadIdx.x;
+  if (p >= n) return;
+  const long i = perm[p];
+  sx[p] = grid[i * 3 + 0];
+  sy[p] = grid[i * 3 + 1];
+  sz[p] = grid[i * 3 + 2];
+}
+
+__global__ __launch_bounds__(256) void tile_boxes(const double* sx, const double* sy, const double* sz, long n,
+                                                  double* boxes) {
+  const long p = blockIdx.x * (long)TP + threadIdx.x;
+  double v[3];
+  const bool ok = p < n;
+  v[0] = ok ? sx[p] : 0;
+  v[1] = ok ? sy[p] : 0;
+  v[2] = ok ? sz[p] : 0;
+  __shared__ double s[6][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int d = 0; d < 3; ++d) {
+    double a = wave_min(ok ? v[d] : DBL_MAX), b = -wave_min(ok ? -v[d] : DBL_MAX);
+    if (lane == 0) { s[d][w] = a; s[3 + d][w] = b; }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int d = threadIdx.x;
+    double r = s[d][0];
+    for (int q = 1; q < 4; ++q) r = d < 3 ? fmin(r, s[d][q]) : fmax(r, s[d][q]);
+    boxes[blockIdx.x * 6 + d] = r;
+  }
+}
+
+__global__ __launch_bounds__(256) void init_centroids(const double* grid, const int* idx, int k, double* cen) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= k) return;
+  const long i = idx[j];
+  cen[j * 3 + 0] = grid[i * 3 + 0];
+  cen[j * 3 + 1] = grid[i * 3 + 1];
+  cen[j * 3 + 2] = grid[i * 3 + 2];
+}
+
+__global__ void init_state(State* st, int mode) {
+  if (threadIdx.x == 0) {
+    st->done = 0;
+    st->final_buf = 0;
+    st->error = 0;
+    st->changed = 0;
+    st->iters = 0;
+    st->mode = mode;
+    st->dups = 0;
+    st->last_diff = 0.0;
+  }
+}
+
+// ------------------------------------------------------------------ stable counting sort
+
+// Per index-order tile of CS_TILE points: histogram of keys.
+__global__ __launch_bounds__(256) void cs_hist(const int* keys, long n, int nbins, int* tile_count,
+                                               const State* st) {
+  if (st && st->done) return;
+  __shared__ int h[MAXB];
+  for (int b = threadIdx.x; b < nbins; b += 256) h[b] = 0;
+  __syncthreads();
+  const long base = (long)blockIdx.x * CS_TILE;
+  for (int e = threadIdx.x; e < CS_TILE; e += 256) {
+    const long i = base + e;
+    if (i < n) {
+      const int key = keys[i];
+      if ((unsigned)key < (unsigned)nbins) atomicAdd(&h[key], 1);
+    }
+  }
+  __syncthreads();
+  int* out = tile_count + (long)blockIdx.x * nbins;
+  for (int b = threadIdx.x; b < nbins; b += 256) out[b] = h[b];
+}
+
+// Column scan over tiles, in place: tile_count[t][b] -> exclusive offset of
+// tile t within bin b; total[b] = count of bin b.
+__global__ __launch_bounds__(256) void cs_colscan(int* tile_count, int ntiles, int nbins, int* total,
+                                                  const State* st) {
+  if (st && st->done) return;
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nbins) return;
+  int run = 0;
+  for (int t = 0; t < ntiles; ++t) {
+    const long o = (long)t * nbins + b;
+    const int c = tile_count[o];
+    tile_count[o] = run;
+    run += c;
+  }
+  total[b] = run;
+}
+
+// Exclusive scan of total[0..nbins) into start (one workgroup of 1024).
+__global__ __launch_bounds__(1024) void cs_totscan(const int* total, int nbins, int* start, const State* st) {
+  if (st && st->done) return;
+  __shared__ int s[1024];
+  const int t = threadIdx.x;
+  const int per = (nbins + 1023) / 1024;
+  int loc = 0;
+  for (int q = 0; q < per; ++q) {
+    const int b = t * per + q;
+    if (b < nbins) loc += total[b];
+  }
+  s[t] = loc;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int run = s[t] - loc;  // exclusive
+  for (int q = 0; q < per; ++q) {
+    const int b = t * per + q;
+    if (b < nbins) {
+      start[b] = run;
+      run += total[b];
+    }
+  }
+}
+
+// Stable scatter: out[start[key] + rank] = index, rank counting earlier
+// indices with the same key.  Chunks of 256 go in order; inside a chunk the
+// four waves go in order; inside a wave, lanes with equal keys are ranked by
+// lane through a 'nbits'-ballot match.
+__global__ __launch_bounds__(256) void cs_scatter(const int* keys, long n, int nbins, int nbits,
+                                                  const int* tile_off, const int* start, int* out,
+                                                  const State* st) {
+  if (st && st->done) return;
+  __shared__ int cnt[MAXB];
+  __shared__ int slot[256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int* toff = tile_off + (long)blockIdx.x * nbins;
+  for (int b = tid; b < nbins; b += 256) cnt[b] = start[b] + toff[b];
+  __syncthreads();
+  const long base = (long)blockIdx.x * CS_TILE;
+  for (int ch = 0; ch < CS_TILE / 256; ++ch) {
+    const long i = base + ch * 256 + tid;
+    int key = i < n ? keys[i] : 0;
+    const bool valid = i < n && (unsigned)key < (unsigned)nbins;
+    if (!valid) key = 0;
+    unsigned long long peers = __ballot(valid);
+    for (int bit = 0; bit < nbits; ++bit) {
+      const bool on = (key >> bit) & 1;
+      const unsigned long long bal = __ballot(valid && on);
+      peers &= on ? bal : ~bal;
+    }
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int rank = __popcll(peers & below);
+    const int leader = valid ? (__ffsll((long long)peers) - 1) : 0;
+    for (int w = 0; w < 4; ++w) {
+      if (wave == w && valid && rank == 0) {
+        const int b0 = cnt[key];
+        cnt[key] = b0 + __popcll(peers);
+        slot[tid] = b0;
+      }
+      __syncthreads();
+    }
+    if (valid) out[slot[wave * 64 + leader] + rank] = (int)i;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ Lloyd iteration
+
+__global__ __launch_bounds__(256) void assign(const double* sx, const double* sy, const double* sz,
+                                              const int* perm, long n, const double* boxes,
+                                              const double* cen, int k, int* map_new, const int* map_old,
+                                              State* st) {
+  if (st && st->done) return;
+  __shared__ int cidx[MAXB];
+  __shared__ double ccx[CAND_CAP], ccy[CAND_CAP], ccz[CAND_CAP];
+  __shared__ double redd[4];
+  __shared__ int redi[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double* b = boxes + (long)blockIdx.x * 6;
+  double box[6];
+  for (int d = 0; d < 6; ++d) box[d] = b[d];
+
+  double u = DBL_MAX;
+  for (int j = tid; j < k; j += 256) u = fmin(u, ub2_box(cen[j * 3], cen[j * 3 + 1], cen[j * 3 + 2], box));
+  u = wave_min(u);
+  if (lane == 0) redd[wave] = u;
+  __syncthreads();
+  u = fmin(fmin(redd[0], redd[1]), fmin(redd[2], redd[3]));
+  const double thr = u * PRUNE;
+
+  int ncand = 0;
+  for (int j0 = 0; j0 < k; j0 += 256) {
+    const int j = j0 + tid;
+    double cx = 0, cy = 0, cz = 0;
+    bool flag = false;
+    if (j < k) {
+      cx = cen[j * 3]; cy = cen[j * 3 + 1]; cz = cen[j * 3 + 2];
+      flag = lb2_box(cx, cy, cz, box) <= thr;
+    }
+    const unsigned long long bal = __ballot(flag);
+    __syncthreads();
+    if (lane == 0) redi[wave] = __popcll(bal);
+    __syncthreads();
+    int off = ncand;
+    for (int w = 0; w < wave; ++w) off += redi[w];
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    off += __popcll(bal & below);
+    if (flag) {
+      cidx[off] = j;
+      if (off < CAND_CAP) { ccx[off] = cx; ccy[off] = cy; ccz[off] = cz; }
+    }
+    ncand += redi[0] + redi[1] + redi[2] + redi[3];
+  }
+  __syncthreads();
+
+  const long p = (long)blockIdx.x * TP + tid;
+  int changed = 0;
+  if (p < n) {
+    const double gx = sx[p], gy = sy[p], gz = sz[p];
+    // ascending candidate order: only a strictly smaller rounded distance wins
+    int bj = cidx[0];
+    double bd2 = dist2(ccx[0], ccy[0], ccz[0], gx, gy, gz);
+    double bs = sqrt(bd2);
+    for (int c = 1; c < ncand; ++c) {
+      double cx, cy, cz;
+      const int j = cidx[c];
+      if (c < CAND_CAP) { cx = ccx[c]; cy = ccy[c]; cz = ccz[c]; }
+      else { cx = cen[j * 3]; cy = cen[j * 3 + 1]; cz = cen[j * 3 + 2]; }
+      const double d2 = dist2(cx, cy, cz, gx, gy, gz);
+      if (d2 < bd2) {
+        const double s = sqrt(d2);
+        if (s < bs) { bs = s; bd2 = d2; bj = j; }
+      }
+    }
+    const long orig = perm[p];
+    if (map_old) changed = (map_old[orig] != bj);
+    map_new[orig] = bj;
+  }
+  if (st && map_old) {
+    changed = wave_sum(changed);
+    if (lane == 0 && changed) atomicAdd(&st->changed, changed);
+  }
+}
+
+__global__ __launch_bounds__(256) void centroid_sums(const int* order, const int* start, const int* total,
+                                                     const double* rho, const double* grid, int k,
+                                                     double* cen_out, State* st) {
+  if (st && st->done) return;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= k) return;
+  const int s = start[c], cnt = total[c];
+  double w = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+  for (int t = 0; t < cnt; ++t) {
+    const long i = order[s + t];
+    const double r = rho[i];
+    w += r;
+    ax += r * grid[i * 3 + 0];
+    ay += r * grid[i * 3 + 1];
+    az += r * grid[i * 3 + 2];
+  }
+  if (w < 1e-8) atomicOr(&st->error, 1);
+  cen_out[c * 3 + 0] = ax / w;
+  cen_out[c * 3 + 1] = ay / w;
+  cen_out[c * 3 + 2] = az / w;
+}
+
+// normed_difference(new, cur) with the reference's sequential sum, % changed,
+// and the loop control of KMeansSolver::kernel (kmeans.cpp:233-257).
+__global__ __launch_bounds__(1024) void diff_control(const double* nxt, const double* cur, int k, long n,
+                                                     double threshold, int it, int max_it, int nxt_buf,
+                                                     State* st, double* diffs, double* pcts) {
+  if (st->done) return;
+  extern __shared__ double terms[];
+  __shared__ double red[4][16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int len = 3 * k;
+  if (st->error) {
+    if (tid == 0) { st->done = 1; st->iters = it + 1; }
+    return;
+  }
+  double mxa = -DBL_MAX, mna = DBL_MAX, mxb = -DBL_MAX, mnb = DBL_MAX;
+  for (int i = tid; i < len; i += 1024) {
+    mxa = fmax(mxa, nxt[i]); mna = fmin(mna, nxt[i]);
+    mxb = fmax(mxb, cur[i]); mnb = fmin(mnb, cur[i]);
+  }
+  mxa = -wave_min(-mxa); mna = wave_min(mna); mxb = -wave_min(-mxb); mnb = wave_min(mnb);
+  if (lane == 0) { red[0][wave] = mxa; red[1][wave] = mna; red[2][wave] = mxb; red[3][wave] = mnb; }
+  __syncthreads();
+  mxa = red[0][0]; mna = red[1][0]; mxb = red[2][0]; mnb = red[3][0];
+  for (int w = 1; w < 16; ++w) {
+    mxa = fmax(mxa, red[0][w]); mna = fmin(mna, red[1][w]);
+    mxb = fmax(mxb, red[2][w]); mnb = fmin(mnb, red[3][w]);
+  }
+  const double scale = fmax(fabs(mxa), fabs(mna)) + fmax(fabs(mxb), fabs(mnb));
+  for (int i = tid; i < len; i += 1024) {
+    const double t = (nxt[i] - cur[i]) / scale;
+    terms[i] = t * t;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double acc = 0.0;
+    for (int i = 0; i < len; ++i) acc += terms[i];
+    const double diff = scale * sqrt(acc);
+    const double pct = 100 * ((double)st->changed) / (double)n;
+    diffs[it] = diff;
+    pcts[it] = pct;
+    st->iters = it + 1;
+    st->last_diff = diff;
+    st->changed = 0;
+    if (diff < threshold) {
+      st->done = 1;
+      st->mode = MODE_FINAL;
+      st->final_buf = nxt_buf;
+    } else if (it == max_it - 1) {
+      st->mode = (diff > threshold) ? MODE_LATEST : MODE_ZEROS;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ map_to_grid
+
+// Nearest grid point per centroid (lexicographic distance/index minimum over
+// the grid tiles the bounds cannot exclude).
+__global__ __launch_bounds__(256) void map_to_grid(const double* sx, const double* sy, const double* sz,
+                                                   const int* perm, long n, const double* boxes, int ntiles,
+                                                   const double* cen_a, const double* cen_b, int latest_buf,
+                                                   const State* st, int* out) {
+  const int c = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int mode = st ? st->mode : MODE_LATEST;
+  if (mode == MODE_ZEROS || (st && st->error)) {
+    if (tid == 0) out[c] = 0;
+    return;
+  }
+  const int buf = (mode == MODE_FINAL) ? st->final_buf : latest_buf;
+  const double* cen = buf ? cen_b : cen_a;
+  const double cx = cen[c * 3], cy = cen[c * 3 + 1], cz = cen[c * 3 + 2];
+  __shared__ double redd[4];
+  __shared__ int redi[4];
+  __shared__ int tl[MAXB];
+  double u = DBL_MAX;
+  for (int t = tid; t < ntiles; t += 256) u = fmin(u, ub2_box(cx, cy, cz, boxes + (long)t * 6));
+  u = wave_min(u);
+  if (lane == 0) redd[wave] = u;
+  __syncthreads();
+  u = fmin(fmin(redd[0], redd[1]), fmin(redd[2], redd[3]));
+  const double thr = u * PRUNE;
+  double bd2 = DBL_MAX, bs = DBL_MAX;
+  int bi = INT_MAX;
+  __shared__ int cntw;
+  // candidate tiles, collected in windows of MAXB
+  for (int t0 = 0; t0 < ntiles; t0 += MAXB) {
+    const int tend = min(ntiles, t0 + MAXB);
+    if (tid == 0) cntw = 0;
+    __syncthreads();
+    for (int t = t0 + tid; t < tend; t += 256)
+      if (lb2_box(cx, cy, cz, boxes + (long)t * 6) <= thr) tl[atomicAdd(&cntw, 1)] = t;
+    __syncthreads();
+    const int nt = cntw;
+    for (int q = 0; q < nt; ++q) {
+      const long p = (long)tl[q] * TP + tid;
+      if (p < n) consider_any(dist2(sx[p], sy[p], sz[p], cx, cy, cz), perm[p], bd2, bs, bi);
+    }
+    __syncthreads();
+  }
+  // lexicographic reduction of (bs, bi)
+  for (int o = 32; o > 0; o >>= 1) {
+    const double os = __shfl_xor(bs, o);
+    const int oi = __shfl_xor(bi, o);
+    if (os < bs || (os == bs && oi < bi)) { bs = os; bi = oi; }
+  }
+  if (lane == 0) { redd[wave] = bs; redi[wave] = bi; }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < 4; ++w)
+      if (redd[w] < bs || (redd[w] == bs && redi[w] < bi)) { bs = redd[w]; bi = redi[w]; }
+    // redd[0]/redi[0] already folded through bs/bi of wave 0
+    out[c] = bi;
+  }
+}
+
+// Sort the k indices ascending (bitonic in LDS) and count adjacent repeats.
+__global__ __launch_bounds__(1024) void sort_indices(int* idx, int k, State* st) {
+  __shared__ int s[MAXB];
+  int p2 = 1;
+  while (p2 < k) p2 <<= 1;
+  for (int i = threadIdx.x; i < p2; i += 1024) s[i] = i < k ? idx[i] : INT_MAX;
+  __syncthreads();
+  for (int size = 2; size <= p2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < p2; i += 1024) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const int a = s[i], b = s[j];
+          if ((a > b) == up) { s[i] = b; s[j] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  int d = 0;
+  for (int i = threadIdx.x; i < k; i += 1024) {
+    idx[i] = s[i];
+    if (i + 1 < k && s[i] == s[i + 1]) ++d;
+  }
+  d = wave_sum(d);
+  if ((threadIdx.x & 63) == 0 && d && st) atomicAdd(&st->dups, d);
+}
+
+}  // namespace km
+}  // namespace thc
+
+// ====================================================================== host
+
+namespace thc {
+namespace km {
+
+__global__ void copy_final(const int* map0, const int* map1, long n, const double* cen0, const double* cen1,
+                           int k, int latest_buf, const State* st, int* map_out, double* cen_out) {
+  const int iters = st->iters;
+  const int mbuf = iters > 0 ? (iters - 1) & 1 : 0;
+  const int cbuf = st->mode == MODE_FINAL ? st->final_buf : latest_buf;
+  const int* m = mbuf ? map1 : map0;
+  const double* c = cbuf ? cen1 : cen0;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x) {
+    if (map_out) map_out[i] = m[i];
+    if (cen_out && i < 3L * k) cen_out[i] = c[i];
+  }
+  if (cen_out && n < 3L * k)
+    for (long i = n + blockIdx.x * 256L + threadIdx.x; i < 3L * k; i += 256L * gridDim.x) cen_out[i] = c[i];
+}
+
+__global__ void check_keys(const int* keys, long n, int nbins, State* st) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x)
+    if ((unsigned)keys[i] >= (unsigned)nbins) atomicOr(&st->error, 2);
+}
+
+struct Layout {
+  long n;
+  int k, max_it;
+  int cells, nbins_sp, nbits_sp, nbits_k;
+  int ntiles, ncs, nbb;
+};
+
+static int bits_for(int nb) {
+  int b = 1;
+  while ((1 << b) < nb) ++b;
+  return b;
+}
+
+static Layout plan(long n, int k, int max_it) {
+  Layout L;
+  L.n = n; L.k = k; L.max_it = max_it;
+  int cells = 1;
+  while (cells < 16 && (long)(2 * cells) * (2 * cells) * (2 * cells) * 64 <= n) cells *= 2;
+  L.cells = cells;
+  L.nbins_sp = cells * cells * cells;
+  L.nbits_sp = bits_for(L.nbins_sp);
+  L.nbits_k = bits_for(k);
+  L.ntiles = ceil_div(n, TP);
+  L.ncs = ceil_div(n, CS_TILE);
+  L.nbb = (int)std::min<long>(256, (n + 255) / 256);
+  return L;
+}
+
+struct Bufs {
+  double *part, *box, *sx, *sy, *sz, *boxes, *cen[2], *diffs, *pcts;
+  int *keys, *perm, *map[2], *order, *tc, *total, *start, *idx0;
+  State* st;
+};
+
+static size_t bytes_for(const Layout& L, bool full) {
+  const long n = L.n;
+  const int nb = std::max(L.nbins_sp, L.k);
+  size_t b = 0;
+  b += carve_bytes<double>(L.nbb * 6) + carve_bytes<double>(6);
+  b += 3 * carve_bytes<double>(n) + carve_bytes<double>((size_t)L.ntiles * 6);
+  b += 2 * carve_bytes<int>(n);                       // keys, perm
+  b += carve_bytes<int>((size_t)L.ncs * nb) + 2 * carve_bytes<int>(nb);
+  b += carve_bytes<State>(1);
+  if (full) {
+    b += 2 * carve_bytes<double>(3L * L.k) + 2 * carve_bytes<double>(std::max(1, L.max_it));
+    b += 3 * carve_bytes<int>(n) + carve_bytes<int>(L.k);
+  }
+  return b;
+}
+
+static bool carve(Arena& a, const Layout& L, Bufs& b, bool full) {
+  const long n = L.n;
+  const int nb = std::max(L.nbins_sp, L.k);
+  b = Bufs{};
+  b.part = a.take<double>(L.nbb * 6);
+  b.box = a.take<double>(6);
+  b.sx = a.take<double>(n);
+  b.sy = a.take<double>(n);
+  b.sz = a.take<double>(n);
+  b.boxes = a.take<double>((size_t)L.ntiles * 6);
+  b.keys = a.take<int>(n);
+  b.perm = a.take<int>(n);
+  b.tc = a.take<int>((size_t)L.ncs * nb);
+  b.total = a.take<int>(nb);
+  b.start = a.take<int>(nb);
+  b.st = a.take<State>(1);
+  bool ok = b.part && b.box && b.sx && b.sy && b.sz && b.boxes && b.keys && b.perm && b.tc && b.total &&
+            b.start && b.st;
+  if (full) {
+    b.cen[0] = a.take<double>(3L * L.k);
+    b.cen[1] = a.take<double>(3L * L.k);
+    b.diffs = a.take<double>(std::max(1, L.max_it));
+    b.pcts = a.take<double>(std::max(1, L.max_it));
+    b.map[0] = a.take<int>(n);
+    b.map[1] = a.take<int>(n);
+    b.order = a.take<int>(n);
+    b.idx0 = a.take<int>(L.k);
+    ok = ok && b.cen[0] && b.cen[1] && b.diffs && b.pcts && b.map[0] && b.map[1] && b.order && b.idx0;
+  }
+  return ok;
+}
+
+// Stable counting sort of keys[0..n) (values < nbins) into out (indices).
+static void enqueue_sort(const Layout& L, const Bufs& b, const int* keys, int nbins, int nbits, int* out,
+                         const State* st, hipStream_t s) {
+  hipLaunchKernelGGL(cs_hist, dim3(L.ncs), dim3(256), 0, s, keys, L.n, nbins, b.tc, st);
+  hipLaunchKernelGGL(cs_colscan, dim3(ceil_div(nbins, 256)), dim3(256), 0, s, b.tc, L.ncs, nbins, b.total, st);
+  hipLaunchKernelGGL(cs_totscan, dim3(1), dim3(1024), 0, s, b.total, nbins, b.start, st);
+  hipLaunchKernelGGL(cs_scatter, dim3(L.ncs), dim3(256), 0, s, keys, L.n, nbins, nbits, b.tc, b.start, out, st);
+}
+
+// Spatially sorted copy of the grid (Morton cells, stable) and tile boxes.
+static void enqueue_setup(const Layout& L, const Bufs& b, const double* grid, hipStream_t s) {
+  hipLaunchKernelGGL(bbox_partial, dim3(L.nbb), dim3(256), 0, s, grid, L.n, b.part);
+  hipLaunchKernelGGL(bbox_final, dim3(1), dim3(64), 0, s, b.part, L.nbb, b.box);
+  hipLaunchKernelGGL(morton_keys, dim3(ceil_div(L.n, 256)), dim3(256), 0, s, grid, L.n, b.box, L.cells, b.keys);
+  enqueue_sort(L, b, b.keys, L.nbins_sp, L.nbits_sp, b.perm, nullptr, s);
+  hipLaunchKernelGGL(gather_sorted, dim3(ceil_div(L.n, 256)), dim3(256), 0, s, grid, b.perm, L.n, b.sx, b.sy, b.sz);
+  hipLaunchKernelGGL(tile_boxes, dim3(L.ntiles), dim3(256), 0, s, b.sx, b.sy, b.sz, L.n, b.boxes);
+}
+
+static void enqueue_kmeans(const Layout& L, const Bufs& b, const double* grid, const double* rho, double threshold,
+                           int* interp_idx, int* map_out, double* cen_out, hipStream_t s) {
+  hipLaunchKernelGGL(init_state, dim3(1), dim3(64), 0, s, b.st, (int)MODE_LATEST);
+  enqueue_setup(L, b, grid, s);
+  hipLaunchKernelGGL(init_centroids, dim3(ceil_div(L.k, 256)), dim3(256), 0, s, grid, b.idx0, L.k, b.cen[0]);
+  const size_t dyn = (size_t)3 * L.k * sizeof(double);
+  for (int it = 0; it < L.max_it; ++it) {
+    const int cur = it & 1, nxt = cur ^ 1;
+    int* mnew = b.map[it & 1];
+    const int* mold = it == 0 ? nullptr : b.map[(it + 1) & 1];
+    hipLaunchKernelGGL(assign, dim3(L.ntiles), dim3(256), 0, s, b.sx, b.sy, b.sz, b.perm, L.n, b.boxes,
+                       b.cen[cur], L.k, mnew, mold, b.st);
+    enqueue_sort(L, b, mnew, L.k, L.nbits_k, b.order, b.st, s);
+    hipLaunchKernelGGL(centroid_sums, dim3(ceil_div(L.k, 256)), dim3(256), 0, s, b.order, b.start, b.total, rho,
+                       grid, L.k, b.cen[nxt], b.st);
+    hipLaunchKernelGGL(diff_control, dim3(1), dim3(1024), dyn, s, b.cen[nxt], b.cen[cur], L.k, L.n, threshold,
+                       it, L.max_it, nxt, b.st, b.diffs, b.pcts);
+  }
+  const int latest = L.max_it & 1;
+  hipLaunchKernelGGL(map_to_grid, dim3(L.k), dim3(256), 0, s, b.sx, b.sy, b.sz, b.perm, L.n, b.boxes, L.ntiles,
+                     b.cen[0], b.cen[1], latest, b.st, interp_idx);
+  hipLaunchKernelGGL(sort_indices, dim3(1), dim3(1024), 0, s, interp_idx, L.k, b.st);
+  if (map_out || cen_out)
+    hipLaunchKernelGGL(copy_final, dim3(256), dim3(256), 0, s, b.map[0], b.map[1], L.n, b.cen[0], b.cen[1], L.k,
+                       latest, b.st, map_out, cen_out);
+}
+
+}  // namespace km
+
+static int km_attrs_done = 0;
+static int km_set_attrs() {
+  if (km_attrs_done) return THC_OK;
+  THC_HIP(hipFuncSetAttribute((const void*)km::diff_control, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              3 * km::MAXB * (int)sizeof(double)));
+  km_attrs_done = 1;
+  return THC_OK;
+}
+
+int kmeans_run(thc_ctx* ctx, const double* grid, const double* rho, long n, int k, int max_it, double threshold,
+               const int* host_init_idx, int* interp_idx, int* grid_map_out, double* centroids_out,
+               thc_kmeans_stats* stats) {
+  using namespace km;
+  if (!ctx || !grid || !rho || !interp_idx || !host_init_idx) return THC_ERR_ARG;
+  if (n < 1 || k < 1 || k > n || max_it < 1 || n > 0x7fffffffL) return THC_ERR_ARG;
+  if (k > MAXB) return THC_ERR_UNSUPPORTED;
+  for (int j = 0; j < k; ++j)
+    if (host_init_idx[j] < 0 || host_init_idx[j] >= n) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  THC_TRY(km_set_attrs());
+  const Layout L = plan(n, k, max_it);
+  const size_t need = bytes_for(L, true);
+  if (need > ctx->kmws.cap) {
+    if (ctx->kmgraph) ctx->kmgraph->reset();
+    THC_TRY(ctx->kmws.reserve(need));
+  }
+  ctx->kmws.reset();
+  Bufs b;
+  if (!carve(ctx->kmws, L, b, true)) return THC_ERR_NO_MEMORY;
+  hipStream_t s = ctx->stream;
+  THC_TRY(ensure_pinned(ctx, sizeof(int) * k + sizeof(State) + 2 * sizeof(double) * max_it + 64));
+  int* hidx = reinterpret_cast<int*>(ctx->pinned);
+  std::copy(host_init_idx, host_init_idx + k, hidx);
+  THC_HIP(hipEventRecord(ctx->ev[0], s));
+  THC_HIP(hipMemcpyAsync(b.idx0, hidx, sizeof(int) * k, hipMemcpyHostToDevice, s));
+  if (ctx->use_graphs) {
+    if (!ctx->kmgraph) ctx->kmgraph = new KmGraphCache();
+    KmGraphCache& gc = *ctx->kmgraph;
+    const void* key[5] = {grid, rho, interp_idx, grid_map_out, centroids_out};
+    bool hit = gc.exec && gc.n == n && gc.k == k && gc.max_it == max_it && gc.threshold == threshold &&
+               gc.base == ctx->kmws.base;
+    for (int q = 0; q < 5 && hit; ++q) hit = gc.key_ptr[q] == key[q];
+    if (!hit) {
+      gc.reset();
+      THC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      enqueue_kmeans(L, b, grid, rho, threshold, interp_idx, grid_map_out, centroids_out, s);
+      hipError_t e = hipStreamEndCapture(s, &gc.graph);
+      THC_HIP(e);
+      THC_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
+      for (int q = 0; q < 5; ++q) gc.key_ptr[q] = key[q];
+      gc.n = n; gc.k = k; gc.max_it = max_it; gc.threshold = threshold; gc.base = ctx->kmws.base;
+    }
+    THC_HIP(hipGraphLaunch(gc.exec, s));
+  } else {
+    enqueue_kmeans(L, b, grid, rho, threshold, interp_idx, grid_map_out, centroids_out, s);
+    THC_HIP(hipGetLastError());
+  }
+  char* hp = reinterpret_cast<char*>(ctx->pinned);
+  State* hst = reinterpret_cast<State*>(hp + align_up(sizeof(int) * k, 64));
+  double* hd = reinterpret_cast<double*>(reinterpret_cast<char*>(hst) + align_up(sizeof(State), 64));
+  double* hpct = hd + max_it;
+  THC_HIP(hipMemcpyAsync(hst, b.st, sizeof(State), hipMemcpyDeviceToHost, s));
+  THC_HIP(hipMemcpyAsync(hd, b.diffs, sizeof(double) * max_it, hipMemcpyDeviceToHost, s));
+  THC_HIP(hipMemcpyAsync(hpct, b.pcts, sizeof(double) * max_it, hipMemcpyDeviceToHost, s));
+  THC_HIP(hipEventRecord(ctx->ev[1], s));
+  THC_HIP(hipStreamSynchronize(s));
+  if (stats) {
+    stats->iterations = hst->iters;
+    stats->converged = hst->mode == MODE_FINAL;
+    stats->duplicates = hst->dups;
+    stats->interp_zeroed = hst->mode == MODE_ZEROS;
+    stats->last_diff = hst->last_diff;
+    if (stats->diffs) std::copy(hd, hd + hst->iters, stats->diffs);
+    if (stats->pcts) std::copy(hpct, hpct + hst->iters, stats->pcts);
+    float ms = 0;
+    THC_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    stats->elapsed_ms = ms;
+  }
+  if (hst->error) return THC_ERR_EMPTY_CLUSTER;
+  if (hst->dups) return THC_WARN_DUPLICATE_INDICES;
+  return THC_OK;
+}
+
+int kmeans_classify(thc_ctx* ctx, const double* grid, long n, const double* centroids, int k, int* grid_map) {
+  using namespace km;
+  if (!ctx || !grid || !centroids || !grid_map || n < 1 || k < 1 || n > 0x7fffffffL) return THC_ERR_ARG;
+  if (k > MAXB) return THC_ERR_UNSUPPORTED;
+  THC_HIP(hipSetDevice(ctx->device));
+  const Layout L = plan(n, k, 1);
+  THC_TRY(ctx->ws.reserve(bytes_for(L, false)));
+  ctx->ws.reset();
+  Bufs b;
+  if (!carve(ctx->ws, L, b, false)) return THC_ERR_NO_MEMORY;
+  hipStream_t s = ctx->stream;
+  enqueue_setup(L, b, grid, s);
+  hipLaunchKernelGGL(assign, dim3(L.ntiles), dim3(256), 0, s, b.sx, b.sy, b.sz, b.perm, L.n, b.boxes, centroids, k,
+                     grid_map, (const int*)nullptr, (State*)nullptr);
+  THC_HIP(hipGetLastError());
+  THC_HIP(hipStreamSynchronize(s));
+  return THC_OK;
+}
+
+int kmeans_update(thc_ctx* ctx, const double* rho, const double* grid, long n, const int* grid_map, int k,
+                  double* centroids) {
+  using namespace km;
+  if (!ctx || !rho || !grid || !grid_map || !centroids || n < 1 || k < 1 || n > 0x7fffffffL) return THC_ERR_ARG;
+  if (k > MAXB) return THC_ERR_UNSUPPORTED;
+  THC_HIP(hipSetDevice(ctx->device));
+  const Layout L = plan(n, k, 1);
+  THC_TRY(ctx->ws.reserve(bytes_for(L, false) + carve_bytes<int>(n)));
+  ctx->ws.reset();
+  Bufs b;
+  if (!carve(ctx->ws, L, b, false)) return THC_ERR_NO_MEMORY;
+  int* order = ctx->ws.take<int>(n);
+  if (!order) return THC_ERR_NO_MEMORY;
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(init_state, dim3(1), dim3(64), 0, s, b.st, (int)MODE_LATEST);
+  hipLaunchKernelGGL(check_keys, dim3(256), dim3(256), 0, s, grid_map, n, k, b.st);
+  enqueue_sort(L, b, grid_map, k, L.nbits_k, order, nullptr, s);
+  hipLaunchKernelGGL(centroid_sums, dim3(ceil_div(k, 256)), dim3(256), 0, s, order, b.start, b.total, rho, grid, k,
+                     centroids, b.st);
+  THC_HIP(hipGetLastError());
+  State hst;
+  THC_HIP(hipMemcpyAsync(&hst, b.st, sizeof(State), hipMemcpyDeviceToHost, s));
+  THC_HIP(hipStreamSynchronize(s));
+  if (hst.error & 2) return THC_ERR_ARG;
+  if (hst.error) return THC_ERR_EMPTY_CLUSTER;
+  return THC_OK;
+}
+
+int kmeans_map_to_grid(thc_ctx* ctx, const double* grid, long n, const double* centroids, int k, int* interp_idx) {
+  using namespace km;
+  if (!ctx || !grid || !centroids || !interp_idx || n < 1 || k < 1 || n > 0x7fffffffL) return THC_ERR_ARG;
+  if (k > MAXB) return THC_ERR_UNSUPPORTED;
+  THC_HIP(hipSetDevice(ctx->device));
+  const Layout L = plan(n, k, 1);
+  THC_TRY(ctx->ws.reserve(bytes_for(L, false)));
+  ctx->ws.reset();
+  Bufs b;
+  if (!carve(ctx->ws, L, b, false)) return THC_ERR_NO_MEMORY;
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(init_state, dim3(1), dim3(64), 0, s, b.st, (int)MODE_LATEST);
+  enqueue_setup(L, b, grid, s);
+  hipLaunchKernelGGL(map_to_grid, dim3(k), dim3(256), 0, s, b.sx, b.sy, b.sz, b.perm, L.n, b.boxes, L.ntiles,
+                     centroids, centroids, 0, (const State*)nullptr, interp_idx);
+  hipLaunchKernelGGL(sort_indices, dim3(1), dim3(1024), 0, s, interp_idx, k, b.st);
+  THC_HIP(hipGetLastError());
+  State hst;
+  THC_HIP(hipMemcpyAsync(&hst, b.st, sizeof(State), hipMemcpyDeviceToHost, s));
+  THC_HIP(hipStreamSynchronize(s));
+  return hst.dups ? THC_WARN_DUPLICATE_INDICES : THC_OK;
+}
+
+}  // namespace thc
