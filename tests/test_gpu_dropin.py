@@ -1,0 +1,82 @@
+"""The drop-in driver thcpp_amd.x: orbital file + input.json in the
+reference's formats, QMCPACK THC file out (src/main.cpp:18-77,
+src/interpolating_vectors.cpp:72-90, 210-245, 283-315)."""
+import ctypes
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import isdf_oracle as IO
+from oracle import kmeans_oracle as KO
+from thcpp_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(os.path.dirname(HERE), "thcpp_amd")
+vp = ctypes.c_void_p
+
+
+def host():
+    h = ctypes.CDLL(os.path.join(PKG, "libthcpp_amd_host.so"))
+    h.thc_host_write_orbital_file.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, vp, vp, vp, vp, vp, vp,
+                                              vp, ctypes.c_double]
+    h.thc_host_read_f64.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.c_long]
+    h.thc_host_read_f64.restype = ctypes.c_long
+    h.thc_host_read_int.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.c_long]
+    h.thc_host_read_int.restype = ctypes.c_long
+    return h
+
+
+def read(h, path, name, dtype=np.float64, cap=1 << 24):
+    out = np.empty(cap, dtype=dtype)
+    f = h.thc_host_read_f64 if dtype == np.float64 else h.thc_host_read_int
+    n = f(path.encode(), name.encode(), out.ctypes.data_as(vp), cap)
+    assert n >= 0, name
+    return out[:n]
+
+
+def test_dropin_driver(tmp_path):
+    h = host()
+    s = synthetic.make_system(reps=1, n=16, M=8, thc_cfac=3, max_it=7)
+    orb = str(tmp_path / "orbitals.h5")
+    thc = str(tmp_path / "thc.h5")
+    P = lambda a: a.ctypes.data_as(vp)
+    fg = np.array(s.n, dtype=np.int32)
+    hc = np.arange(s.M * s.M, dtype=np.float64).reshape(s.M, s.M)
+    ne = np.array([4, 4], dtype=np.int32)
+    assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(s.aoR), P(fg), P(s.coulG),
+                                         P(hc), P(ne), -2.5) == 0
+    inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac,
+           "interpolating_points": {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}}}
+    (tmp_path / "input.json").write_text(json.dumps(inp))
+    r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    ref = KO.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, 0)
+    idx = read(h, thc, "/Hamiltonian/THC/InterpIndx", np.int32)
+    np.testing.assert_array_equal(idx, ref["interp_idx"])
+    orbs = read(h, thc, "/Hamiltonian/THC/Orbitals").reshape(s.M, s.K, 2)
+    np.testing.assert_array_equal(orbs[:, :, 0], s.aoR[idx].T)
+    assert not orbs[:, :, 1].any()
+    luv = read(h, thc, "/Hamiltonian/THC/Luv").reshape(s.K, s.K, 2)[:, :, 0]
+    assert not np.triu(luv, 1).any()
+    v_ref = IO.isdf(s.aoR, idx, s.n, s.coulG)["V"]
+    err = np.linalg.norm(luv @ luv.T - v_ref) / np.linalg.norm(v_ref)
+    assert err < 1e-9, err
+    np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/dims", np.int32), [-1, 0, 0, s.M, 4, 4, 0, 0])
+    np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/THC/dims", np.int32), [s.M, s.K, 0])
+    np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/Energies"), [-2.5])
+    np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/occups", np.int32), np.zeros(8))
+    np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/hcore").reshape(s.M, s.M, 2)[:, :, 0], hc)
+    assert "Step" in r.stdout and "Dumping Luv" in r.stdout
+
+
+def test_dropin_missing_key_fails(tmp_path):
+    (tmp_path / "input.json").write_text(json.dumps({"orbital_file": "x.h5", "output_file": "y.h5"}))
+    r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode != 0 and "thc_cfac" in r.stderr

@@ -21,6 +21,14 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libthcpp_amd.so")
 ROOT = os.path.dirname(HERE)
 
+HOST = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "libthcpp_amd_host.so")
+EXE = os.path.join(HERE, "thcpp_amd.x")
+HOST_SOURCES = ["h5io.cpp", "thcpp.cpp", "orbital_file.cpp"]
+HOST_HEADERS = ["h5io.h", "json.h", "thcpp.h"]
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+              "-I" + os.path.join(ROOT, "include")]
+
 HIP_SOURCES = ["kmeans.hip", "gemm.hip", "linalg.hip", "fft.hip", "api.hip"]
 CXX_SOURCES = ["seed.cpp"]
 HEADERS = ["common.h", "kmeans.h"]
@@ -68,17 +76,53 @@ def build(force: bool = False, jobs: int = 8) -> str:
             cmds.append(["g++", "-O2", "-fPIC", "-std=c++17", "-c", s, "-o", o])
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
+    relinked = False
     if cmds or not os.path.exists(LIB):
         tmp = LIB + ".tmp"
         _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
         os.replace(tmp, LIB)
+        relinked = True
+    _build_host(force or relinked, jobs)
     return LIB
+
+
+def _host_stale(src: str, obj: str) -> bool:
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src] + [os.path.join(HOST, h) for h in HOST_HEADERS] + [os.path.join(ROOT, "include", "thcpp_amd.h")]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _build_host(force: bool, jobs: int) -> None:
+    """Drop-in host: libthcpp_amd_host.so (classes + HDF5 I/O) and thcpp_amd.x."""
+    cmds, objs = [], []
+    for src in HOST_SOURCES + ["main.cpp"]:
+        s = os.path.join(HOST, src)
+        o = os.path.join(OBJ, "host_" + src + ".o")
+        if src != "main.cpp":
+            objs.append(o)
+        if force or _host_stale(s, o):
+            cmds.append(["g++"] + HOST_FLAGS + ["-c", s, "-o", o])
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
+    if cmds or not os.path.exists(HOST_LIB) or not os.path.exists(EXE):
+        rp = ["-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+        tmp = HOST_LIB + ".tmp"
+        _run(["g++", "-shared", "-o", tmp] + objs + ["-L" + HERE, "-lthcpp_amd", "-L/opt/rocm/lib", "-lamdhip64",
+                                                     "-ldl"] + rp)
+        os.replace(tmp, HOST_LIB)
+        tmp = EXE + ".tmp"
+        _run(["g++", "-o", tmp, os.path.join(OBJ, "host_main.cpp.o"), "-L" + HERE, "-lthcpp_amd_host",
+              "-lthcpp_amd", "-L/opt/rocm/lib", "-lamdhip64"] + rp)
+        os.replace(tmp, EXE)
 
 
 def clean() -> None:
     shutil.rmtree(OBJ, ignore_errors=True)
-    if os.path.exists(LIB):
-        os.remove(LIB)
+    for p in (LIB, HOST_LIB, EXE):
+        if os.path.exists(p):
+            os.remove(p)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,438 @@
+// Drop-in host: the reference's single-rank driver sequence over the C ABI.
+#include "thcpp.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+
+#include "h5io.h"
+
+namespace {
+
+void hip_ok(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* dev_alloc(size_t count) {
+  void* p = nullptr;
+  hip_ok(hipMalloc(&p, count * sizeof(T) + 16), "hipMalloc");
+  return reinterpret_cast<T*>(p);
+}
+
+template <class T>
+T* to_dev(const std::vector<T>& h) {
+  T* d = dev_alloc<T>(h.size());
+  hip_ok(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+  return d;
+}
+
+template <class T>
+std::vector<T> to_host(const T* d, size_t count) {
+  std::vector<T> h(count);
+  hip_ok(hipMemcpy(h.data(), d, count * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy");
+  return h;
+}
+
+struct DevFree {
+  void* p;
+  explicit DevFree(void* q) : p(q) {}
+  ~DevFree() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+void status_ok(int st, const char* where) {
+  if (st == THC_OK || st >= 100) return;
+  throw std::runtime_error(std::string(where) + ": " + thc_status_string(st) + " " + thc_last_error());
+}
+
+// Real orbitals from the complex aoR the reference reads (src/h5helper.cpp:19-26).
+std::vector<double> real_part_checked(const std::vector<double>& z, const std::string& name) {
+  std::vector<double> r(z.size() / 2);
+  double im = 0.0, re = 0.0;
+  for (size_t i = 0; i < r.size(); ++i) {
+    r[i] = z[2 * i];
+    re = std::max(re, std::fabs(z[2 * i]));
+    im = std::max(im, std::fabs(z[2 * i + 1]));
+  }
+  if (im > 1e-12 * std::max(1.0, re))
+    throw std::runtime_error(name + " has imaginary parts; complex orbitals are not supported by this build");
+  return r;
+}
+
+}  // namespace
+
+namespace ContextHandler {
+BlacsHandler::BlacsHandler() {
+  const char* d = std::getenv("THC_DEVICE");
+  device = d ? std::atoi(d) : 0;
+  status_ok(thc_ctx_create(device, nullptr, &ctx), "thc_ctx_create");
+}
+BlacsHandler::~BlacsHandler() {
+  if (ctx) thc_ctx_destroy(ctx);
+}
+}  // namespace ContextHandler
+
+namespace UTILS {
+void parse_simple_opts(thc::Json& input, int rank, int& thc_cfac, int& thc_half_cfac, bool& half_rotated,
+                       bool& find_interp_vec) {
+  (void)rank;
+  thc_cfac = input.at("thc_cfac").as_int();
+  find_interp_vec = input.contains("find_interp_vec") ? input.at("find_interp_vec").as_bool() : true;
+  half_rotated = false;
+  thc_half_cfac = 0;
+  if (find_interp_vec) {
+    if (input.contains("half_rotated")) {
+      half_rotated = input.at("half_rotated").as_bool();
+    } else {
+      std::cout << " * Performing THC on full orbital basis." << std::endl;
+    }
+    if (half_rotated) {
+      if (input.contains("thc_half_cfac")) {
+        thc_half_cfac = input.at("thc_half_cfac").as_int();
+      } else {
+        std::cout << " * thc_half_cfac not specified." << std::endl;
+        std::cout << " * Setting to be the same as thc_cfac." << std::endl;
+        thc_half_cfac = thc_cfac;
+      }
+    }
+  }
+}
+}  // namespace UTILS
+
+namespace KMeans {
+
+KMeansSolver::KMeansSolver(thc::Json& input, ContextHandler::BlacsHandler& BH) {
+  (void)BH;
+  filename = input.at("orbital_file").as_string();
+  const thc::Json& km = input.at("interpolating_points").at("kmeans");
+  max_it = km.at("max_it").as_int();
+  threshold = km.at("threshold").as_double();
+  if (km.contains("rng_seed")) {
+    rng_seed = km.at("rng_seed").as_uint();
+  } else {
+    std::cout << " * RNG seed not set in input file." << std::endl;
+    rng_seed = (unsigned int)std::chrono::system_clock::now().time_since_epoch().count();
+    std::cout << " * Setting RNG seed to : " << rng_seed << std::endl;
+    std::cout << std::endl;
+  }
+  if (km.contains("density")) {
+    density_label = km.at("density").as_string();
+  } else {
+    density_label = "density";
+    std::cout << " * Density not specified." << std::endl;
+    std::cout << " * Using full basis set for density." << std::endl;
+  }
+}
+
+void KMeansSolver::kernel(ContextHandler::BlacsHandler& BH, std::vector<int>& interp_indxs, int thc_cfac) {
+  thc::h5::File f(filename, thc::h5::File::ReadOnly);
+  std::vector<uint64_t> gd, dd;
+  std::vector<double> grid = f.read_f64("real_space_grid", &gd);
+  std::vector<double> rho = f.read_f64(density_label, &dd);
+  const std::vector<uint64_t> ad = f.dims("aoR");
+  const long n = (long)gd[0];
+  const int k = thc_cfac * (int)ad[1];
+  if (gd.size() != 2 || gd[1] != 3 || (long)rho.size() != n) throw std::runtime_error("bad grid/density shapes");
+  std::cout << "#################################################" << std::endl;
+  std::cout << "## Finding interpolation points using K-Means. ##" << std::endl;
+  std::cout << "#################################################" << std::endl;
+  std::cout << std::endl;
+  std::cout << " * Number of interpolating points : " << k << std::endl;
+  std::cout << " * Number of grid points : " << n << std::endl;
+  double* d_grid = to_dev(grid);
+  DevFree f1(d_grid);
+  double* d_rho = to_dev(rho);
+  DevFree f2(d_rho);
+  int* d_idx = dev_alloc<int>(k);
+  DevFree f3(d_idx);
+  std::vector<double> diffs(std::max(1, max_it)), pcts(std::max(1, max_it));
+  thc_kmeans_stats st = {};
+  st.diffs = diffs.data();
+  st.pcts = pcts.data();
+  const auto t0 = std::chrono::steady_clock::now();
+  const int s = thc_kmeans(BH.ctx, d_grid, d_rho, n, k, max_it, threshold, rng_seed, nullptr, d_idx, nullptr,
+                           nullptr, &st);
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::cout << "  *   Step        Error     % change" << std::endl;
+  for (int i = 0; i < st.iterations; ++i)
+    if (i % 10 == 0)
+      std::cout << std::setw(10) << i << "  " << std::setprecision(5) << std::scientific << diffs[i] << "  "
+                << pcts[i] << std::endl;
+  std::cout.unsetf(std::ios::floatfield);
+  if (s == THC_ERR_EMPTY_CLUSTER) {
+    std::cout << " CRITICAL ERROR: Found centroid with weight ~ 0" << std::endl;
+    std::exit(1);
+  }
+  status_ok(s, "thc_kmeans");
+  if (!st.converged && st.last_diff > threshold)
+    std::cout << " * Threshold not breached. Final Error: " << st.last_diff << std::endl;
+  interp_indxs = to_host(d_idx, k);
+  if (st.duplicates)
+    for (int i = 0; i + 1 < k; ++i)
+      if (interp_indxs[i] == interp_indxs[i + 1]) std::cout << "ERROR: Found repeated indices: " << i << std::endl;
+  std::cout << " * Time taken for K-Means solution: " << secs << " seconds" << std::endl;
+  std::cout << std::endl;
+}
+
+}  // namespace KMeans
+
+namespace InterpolatingPoints {
+
+IPoints::IPoints(thc::Json& input, ContextHandler::BlacsHandler& BH) {
+  (void)BH;
+  std::cout << "#################################################" << std::endl;
+  std::cout << "##   Setting up interpolating point solver.    ##" << std::endl;
+  std::cout << "#################################################" << std::endl;
+  std::cout << std::endl;
+  const thc::Json& base = input.at("interpolating_points");
+  if (base.contains("kmeans")) {
+    solver = "kmeans";
+  } else if (base.contains("qrcp")) {
+    solver = "qrcp";
+  } else {
+    std::cout << "No solver specified." << std::endl;
+    solver = "kmeans";
+  }
+}
+
+std::vector<int> IPoints::kernel(thc::Json& input, ContextHandler::BlacsHandler& BH, int thc_cfac, bool half_rotate) {
+  std::vector<int> interp_indxs;
+  if (solver == "qrcp") {
+    throw std::runtime_error("the QRCP interpolating-point solver is not part of this build (k-means only)");
+  }
+  if (half_rotate) throw std::runtime_error("half-rotated interpolating points are not part of this build");
+  std::cout << " * Using K-Means solver." << std::endl;
+  std::cout << std::endl;
+  KMeans::KMeansSolver KM(input, BH);
+  KM.kernel(BH, interp_indxs, thc_cfac);
+  return interp_indxs;
+}
+
+}  // namespace InterpolatingPoints
+
+namespace InterpolatingVectors {
+
+IVecs::IVecs(thc::Json& input, ContextHandler::BlacsHandler& BH, std::vector<int>& interp_indxs, bool half_rotated,
+             bool append) {
+  input_file = input.at("orbital_file").as_string();
+  output_file = input.at("output_file").as_string();
+  std::cout << "#################################################" << std::endl;
+  std::cout << "##   Setting up interpolative vector solver.   ##" << std::endl;
+  std::cout << "#################################################" << std::endl;
+  std::cout << std::endl;
+  if (half_rotated) throw std::runtime_error("the half-rotated THC path is not part of this build");
+  {
+    thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
+    const auto d = fin.dims("aoR");
+    ngrid = (long)d[0];
+    nbasis = (int)d[1];
+    fft_grid = fin.read_int("fft_grid");
+  }
+  if (append) thc::h5::File(output_file, thc::h5::File::Truncate);
+  prefix = "";
+  nmu = (int)interp_indxs.size();
+  setup_CZt(interp_indxs, BH);
+  setup_CCt(interp_indxs, BH);
+  check_rank(BH);
+}
+
+IVecs::~IVecs() {
+  if (d_czt) (void)hipFree(d_czt);
+  if (d_cct) (void)hipFree(d_cct);
+  if (d_idx) (void)hipFree(d_idx);
+}
+
+void IVecs::setup_CZt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH) {
+  std::vector<double> aoR;
+  {
+    thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
+    aoR = real_part_checked(fin.read_complex("aoR"), "aoR");
+  }
+  const int k = nmu, m = nbasis;
+  const long n = ngrid;
+  // THC/Orbitals (M, K) and THC/InterpIndx (src/interpolating_vectors.cpp:72-90)
+  {
+    std::vector<double> orb(2 * (size_t)m * k, 0.0);
+    for (int i = 0; i < m; ++i)
+      for (int mu = 0; mu < k; ++mu) orb[2 * ((size_t)i * k + mu)] = aoR[(size_t)interp_indxs[mu] * m + i];
+    thc::h5::File fo(output_file, thc::h5::File::ReadWrite);
+    fo.ensure_group("/Hamiltonian/THC");
+    std::cout << " * Writing aoR_mu to file" << std::endl;
+    fo.write_complex_as_f64("/Hamiltonian/THC/" + prefix + "Orbitals", orb.data(), {(uint64_t)m, (uint64_t)k},
+                            true);
+    std::cout << " * Writing indices of interpolating points to file" << std::endl;
+    fo.write_int("/Hamiltonian/THC/" + prefix + "InterpIndx", interp_indxs.data(), {(uint64_t)k});
+  }
+  std::cout << " * Constructing CZt matrix" << std::endl;
+  std::cout << " * Matrix Shape: (" << k << ", " << n << ")" << std::endl;
+  double* d_ao = to_dev(aoR);
+  DevFree fa(d_ao);
+  d_idx = to_dev(interp_indxs);
+  d_czt = dev_alloc<double>((size_t)k * n);
+  status_ok(thc_czt(BH.ctx, d_ao, n, m, d_idx, k, d_czt), "thc_czt");
+}
+
+void IVecs::setup_CCt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH) {
+  (void)interp_indxs;
+  d_cct = dev_alloc<double>((size_t)nmu * nmu);
+  status_ok(thc_cct(BH.ctx, d_czt, ngrid, d_idx, nmu, d_cct), "thc_cct");
+}
+
+// The reference counts singular values of CC^T (src/interpolating_vectors.cpp:192-208); here the
+// Cholesky factorisation the solve needs reports whether CC^T is numerically positive definite.
+void IVecs::check_rank(ContextHandler::BlacsHandler& BH) {
+  std::cout << " * Checking rank of CCt matrix." << std::endl;
+  double* tmp = dev_alloc<double>((size_t)nmu * nmu);
+  DevFree ft(tmp);
+  hip_ok(hipMemcpy(tmp, d_cct, sizeof(double) * nmu * nmu, hipMemcpyDeviceToDevice), "hipMemcpy");
+  int info = 0;
+  status_ok(thc_cholesky(BH.ctx, tmp, nmu, &info), "thc_cholesky");
+  if (info != 0) {
+    std::cout << " * WARNING: CCt matrix is rank deficient. Leading minor " << info
+              << " is not positive definite." << std::endl;
+  } else {
+    std::cout << " * CCt matrix has full rank : " << nmu << std::endl;
+  }
+  std::cout << std::endl;
+}
+
+void IVecs::kernel(ContextHandler::BlacsHandler& BH) {
+  const int k = nmu;
+  const long n = ngrid;
+  std::cout << "#################################################" << std::endl;
+  std::cout << "##        Finding interpolating vectors.       ##" << std::endl;
+  std::cout << "#################################################" << std::endl;
+  std::cout << std::endl;
+  std::cout << " * Performing least squares solve." << std::endl;
+  auto t0 = std::chrono::steady_clock::now();
+  double* d_x = dev_alloc<double>((size_t)k * n);
+  DevFree fx(d_x);
+  int info = 0;
+  const int s = thc_least_squares(BH.ctx, d_cct, d_czt, n, k, 0, d_x, &info);
+  std::cout << " * Time for least squares solve : "
+            << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds"
+            << std::endl;
+  std::cout << std::endl;
+  if (s != THC_OK) {
+    std::cout << " * Parallel least squares failed." << std::endl;
+    std::cout << " * Error code: " << info << std::endl;
+    status_ok(s, "thc_least_squares");
+  }
+  // release ZC^T before the spectrum is formed
+  (void)hipFree(d_czt);
+  d_czt = nullptr;
+  std::cout << "#################################################" << std::endl;
+  std::cout << "##        Constructing Muv matrix.             ##" << std::endl;
+  std::cout << "#################################################" << std::endl;
+  std::cout << std::endl;
+  std::cout << " * Performing FFT on interpolating vectors." << std::endl;
+  std::cout << std::endl;
+  if ((long)fft_grid[0] * fft_grid[1] * fft_grid[2] != n)
+    std::cout << " * WARNING: FFT grid not consitent with number of real space grid points." << std::endl;
+  std::vector<double> coulG;
+  {
+    thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
+    coulG = fin.read_f64("fft_coulomb");
+  }
+  double* d_cg = to_dev(coulG);
+  DevFree fc(d_cg);
+  const long g2 = 2L * fft_grid[0] * fft_grid[1] * (fft_grid[2] / 2 + 1);
+  double* d_b = dev_alloc<double>((size_t)k * g2);
+  DevFree fb(d_b);
+  t0 = std::chrono::steady_clock::now();
+  status_ok(thc_fft_vectors(BH.ctx, d_x, k, fft_grid.data(), d_cg, d_b), "thc_fft_vectors");
+  std::cout << " * Time to FFT interpolating vectors : "
+            << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds"
+            << std::endl;
+  std::cout << std::endl;
+  std::cout << " * Constructing Muv." << std::endl;
+  double* d_v = dev_alloc<double>((size_t)k * k);
+  DevFree fv(d_v);
+  t0 = std::chrono::steady_clock::now();
+  status_ok(thc_construct_muv(BH.ctx, d_b, k, g2, d_v), "thc_construct_muv");
+  std::cout << "  * Time to construct Muv: "
+            << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds."
+            << std::endl;
+  // dump_thc_data (src/interpolating_vectors.cpp:283-315)
+  std::cout << " * Performing Cholesky decomposition on Muv." << std::endl;
+  t0 = std::chrono::steady_clock::now();
+  int cinfo = 0;
+  status_ok(thc_cholesky(BH.ctx, d_v, k, &cinfo), "thc_cholesky");
+  std::vector<double> l = to_host(d_v, (size_t)k * k);
+  std::cout << " * Time to perform cholesky decomposition on Muv: "
+            << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds."
+            << std::endl;
+  std::cout << " * Dumping Luv to: " << output_file << "." << std::endl;
+  if (cinfo != 0) {
+    std::cout << "  * WARNING: Cholesky decomposition failed!" << std::endl;
+    std::cout << "  * SCALAPACK Error code: " << cinfo << std::endl;
+    std::cout << "   * The leading minor of order " << cinfo << " is not positive definite." << std::endl;
+  }
+  std::cout << std::endl;
+  std::vector<double> luv(2 * (size_t)k * k, 0.0);
+  for (size_t i = 0; i < (size_t)k * k; ++i) luv[2 * i] = l[i];
+  thc::h5::File fo(output_file, thc::h5::File::ReadWrite);
+  fo.ensure_group("/Hamiltonian/THC");
+  fo.write_complex_as_f64("/Hamiltonian/THC/" + prefix + "Luv", luv.data(), {(uint64_t)k, (uint64_t)k}, true);
+}
+
+void IVecs::dump_qmcpack_data(int thc_cfac, int thc_half_cfac, ContextHandler::BlacsHandler& BH) {
+  (void)BH;
+  thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
+  thc::h5::File fo(output_file, thc::h5::File::ReadWrite);
+  fo.ensure_group("/Hamiltonian");
+  std::vector<uint64_t> hd;
+  std::vector<double> hcore = fin.read_complex("hcore", &hd);
+  fo.write_complex_as_f64("/Hamiltonian/hcore", hcore.data(), hd, false);
+  std::vector<int> ne = fin.read_int("num_electrons");
+  const int nup = ne.at(0), ndown = ne.at(1);
+  std::vector<double> e0 = fin.read_f64("constant_energy_factors");
+  fo.write_f64("/Hamiltonian/Energies", e0.data(), {(uint64_t)e0.size()});
+  std::vector<int> occups(nup + ndown, 0);
+  fo.write_int("/Hamiltonian/occups", occups.data(), {(uint64_t)occups.size()});
+  std::vector<int> dims = {-1, 0, 0, nbasis, nup, ndown, 0, 0};
+  fo.write_int("/Hamiltonian/dims", dims.data(), {8});
+  // thc_half_cfac is only defined on the half-rotated path (src/main.cpp:38); 0 otherwise
+  std::vector<int> thc_dims = {nbasis, thc_cfac * nbasis, thc_half_cfac * nbasis};
+  fo.ensure_group("/Hamiltonian/THC");
+  fo.write_int("/Hamiltonian/THC/dims", thc_dims.data(), {3});
+}
+
+}  // namespace InterpolatingVectors
+
+int thcpp_main(const std::string& input_path) {
+  ContextHandler::BlacsHandler BH;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::ifstream in(input_path);
+  if (!in) throw std::runtime_error("cannot read " + input_path);
+  std::stringstream ss;
+  ss << in.rdbuf();
+  thc::Json input = thc::Json::parse(ss.str());
+  std::cout << " * Running on 1 GPU (" << thc_version() << ")." << std::endl;
+  int thc_cfac = 0, thc_half_cfac = 0;
+  bool half_rotate = false, find_interp_vec = true;
+  UTILS::parse_simple_opts(input, BH.rank, thc_cfac, thc_half_cfac, half_rotate, find_interp_vec);
+  InterpolatingPoints::IPoints IPSolver(input, BH);
+  std::vector<int> interp_indxs = IPSolver.kernel(input, BH, thc_cfac, false);
+  if (find_interp_vec) {
+    InterpolatingVectors::IVecs IVSolver(input, BH, interp_indxs, false, true);
+    IVSolver.kernel(BH);
+    if (!half_rotate) IVSolver.dump_qmcpack_data(thc_cfac, thc_half_cfac, BH);
+  }
+  if (find_interp_vec && half_rotate)
+    throw std::runtime_error("the half-rotated THC path is not part of this build");
+  std::cout << " * Total simulation time : "
+            << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds."
+            << std::endl;
+  return 0;
+}
