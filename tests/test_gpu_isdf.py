@@ -144,3 +144,51 @@ def test_factorize_cfg0(ctx, cfg0, cfg0_kmeans):
     np.testing.assert_array_equal(out["interp_idx"].cpu().numpy(), cfg0_kmeans["interp_idx"])
     ref = IO.isdf(s.aoR, cfg0_kmeans["interp_idx"], s.n, s.coulG)
     assert rel(out["V"].cpu(), ref["V"]) < 1e-6
+
+
+def test_isdf_refinement_cfg0(ctx, cfg0, cfg0_kmeans):
+    """FP64 residual refinement (R = ZC^T - CC^T X, X += CC^T^-1 R) keeps V at the
+    FP64 conditioning floor and moves xi towards the oracle's QR solution."""
+    s = cfg0
+    idx = cfg0_kmeans["interp_idx"]
+    ref = IO.isdf(s.aoR, idx, s.n, s.coulG, return_all=True)
+    r0 = ctx.isdf(s.aoR, idx, s.n, s.coulG, want_xi=True)
+    r1 = ctx.isdf(s.aoR, idx, s.n, s.coulG, refine=1, want_xi=True)
+    assert r1["status"] == L.THC_OK
+    e0, e1 = rel(r0["V"].cpu(), ref["V"]), rel(r1["V"].cpu(), ref["V"])
+    print("cfg0 rel V no refinement %.3e, one step %.3e" % (e0, e1))
+    assert e1 < 1e-6
+    c = IO.cct(ref["Z"], idx)
+    x1 = r1["xi"].cpu().numpy()
+    assert rel(c @ x1, ref["Z"]) <= 1.5 * rel(c @ r0["xi"].cpu().numpy(), ref["Z"]) + 1e-14
+
+
+def test_isdf_mixed_int8_cfg0(ctx, cfg0, cfg0_kmeans):
+    """Mixed path: FP32 FFT and V from exact int8 digit products (5 digits per
+    value) stays within 1e-6 of the FP64 path."""
+    s = cfg0
+    idx = cfg0_kmeans["interp_idx"]
+    ref = ctx.isdf(s.aoR, idx, s.n, s.coulG)
+    out = ctx.isdf(s.aoR, idx, s.n, s.coulG, precision=L.PREC_MIXED)
+    assert out["status"] == L.THC_OK
+    e = rel(out["V"].cpu(), ref["V"].cpu())
+    print("cfg0 mixed (int8 V) rel V vs FP64 %.3e" % e)
+    assert e < 1e-6
+    v = out["V"].cpu().numpy()
+    np.testing.assert_array_equal(v, v.T)
+
+
+def test_isdf_mixed_uneven_sizes(ctx):
+    """K not a multiple of the 128 tile, grid with odd edges, spectrum length not
+    a multiple of the 4096 chunk."""
+    from thcpp_amd import synthetic
+    rng = np.random.default_rng(7)
+    n = (15, 12, 10)
+    N = int(np.prod(n))
+    a = np.linalg.qr(rng.standard_normal((N, 9)))[0]
+    idx = np.sort(rng.choice(N, 37, replace=False)).astype(np.int32)
+    cg = synthetic.coulomb_kernel(6.5, n)
+    ref = ctx.isdf(a, idx, n, cg)
+    out = ctx.isdf(a, idx, n, cg, precision=L.PREC_MIXED)
+    assert out["status"] in (L.THC_OK, L.THC_WARN_LUV_NOT_POSDEF)
+    assert rel(out["V"].cpu(), ref["V"].cpu()) < 1e-6

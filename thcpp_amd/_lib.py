@@ -22,7 +22,7 @@ THC_WARN_DUPLICATE_INDICES = 100
 THC_WARN_LUV_NOT_POSDEF = 101
 THC_WARN_FFT_GRID_MISMATCH = 102
 
-PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_MIXED = 0, 1, 2, 3
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int

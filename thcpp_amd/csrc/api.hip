@@ -67,6 +67,16 @@ __global__ __launch_bounds__(256) void gather_cols(const double* z, long ldz, co
   }
 }
 
+// C[mu][nu] = Zt[idx[nu]][mu] (ZC^T stored transposed, N x K)
+__global__ __launch_bounds__(256) void gather_rows_t(const double* zt, long n, const int* idx, int k, double* c) {
+  const long total = (long)k * k;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += 256L * gridDim.x) {
+    const long q = e / k, r = e % k;  // read row idx[q] of Zt contiguously
+    const long row = idx[q];
+    c[r * k + q] = (row >= 0 && row < n) ? zt[row * k + r] : 0.0;
+  }
+}
+
 __global__ __launch_bounds__(256) void to_f32(const double* x, float* y, long count) {
   for (long e = blockIdx.x * 256L + threadIdx.x; e < count; e += 256L * gridDim.x) y[e] = (float)x[e];
 }
@@ -100,8 +110,9 @@ size_t isdf_bytes(long n, int k, const int fft[3], int prec, int refine, bool ow
   if (refine > 0) b += carve_bytes<double>((size_t)k * n);   // r
   b += prec == 0 ? carve_bytes<double>((size_t)k * g2) : carve_bytes<float>((size_t)k * g2);
   if (prec != 0) b += carve_bytes<float>((size_t)k * n);
-  b += chol_work_bytes(k) + 256 + spd_inverse_work_bytes(k) + 256 + fft_work_bytes(k, fft, prec) + 256 +
-       syrk_work_bytes(k, g2, prec) + 256;
+  const int fprec = prec == 0 ? 0 : 1;  // FFT / spectrum precision
+  b += chol_work_bytes(k) + 256 + spd_inverse_work_bytes(k) + 256 + fft_work_bytes(k, fft, fprec) + 256 +
+       (prec == THC_PREC_MIXED ? oz_syrk_work_bytes(k, g2) : syrk_work_bytes(k, g2, fprec)) + 256;
   b += carve_bytes<int>(16);
   return b;
 }
@@ -122,8 +133,9 @@ bool isdf_carve(Arena& a, long n, int k, const int fft[3], int prec, int refine,
   B.x32 = prec != 0 ? a.take<float>((size_t)k * n) : nullptr;
   B.chol_w = a.take<char>(chol_work_bytes(k));
   B.inv_w = a.take<char>(spd_inverse_work_bytes(k));
-  B.fft_w = a.take<char>(fft_work_bytes(k, fft, prec));
-  B.syrk_w = a.take<char>(syrk_work_bytes(k, g2, prec));
+  const int fprec = prec == 0 ? 0 : 1;
+  B.fft_w = a.take<char>(fft_work_bytes(k, fft, fprec));
+  B.syrk_w = a.take<char>(prec == THC_PREC_MIXED ? oz_syrk_work_bytes(k, g2) : syrk_work_bytes(k, g2, fprec));
   B.info = a.take<int>(16);
   return B.z && B.x && B.c && B.l && B.ci && B.v && (refine <= 0 || B.r) && B.b && (prec == 0 || B.x32) &&
          B.chol_w && B.inv_w && B.fft_w && B.syrk_w && B.info;
@@ -143,7 +155,7 @@ int isdf_run(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int
   const int prec = opts ? opts->precision : THC_PREC_FP64;
   const int refine = opts ? std::max(0, opts->refine_steps) : 0;
   const int want_luv = opts ? opts->want_luv : 1;
-  if (prec != THC_PREC_FP64 && prec != THC_PREC_FP32) return THC_ERR_UNSUPPORTED;
+  if (prec != THC_PREC_FP64 && prec != THC_PREC_FP32 && prec != THC_PREC_MIXED) return THC_ERR_UNSUPPORTED;
   const long nfft = (long)fft[0] * fft[1] * fft[2];
   if (fft[0] < 1 || fft[1] < 1 || fft[2] < 1) return THC_ERR_ARG;
   // The reference only warns when prod(fft_grid) != N (interpolating_vectors.cpp:368-370) and then
@@ -161,19 +173,20 @@ int isdf_run(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int
   hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, B.info, 16);
   hipLaunchKernelGGL(check_idx, dim3(grid_for(k)), dim3(256), 0, s, idx, k, n, B.info + 2);
   THC_HIP(hipEventRecord(ev[0], s));
-  // 1. ZC^T = (Phi_mu Phi^T)^2: A rows gathered by idx, epilogue squares
+  // 1. ZC^T = (Phi_mu Phi^T)^2: A rows gathered by idx, epilogue squares; stored
+  //    transposed (N x K, Zt[r][mu]) so the solve reads both operands along k
   {
     GemmArgs g = {};
     g.m = k; g.n = n; g.k = m;
     g.a = aoR; g.lda = m; g.a_trans = 0; g.a_rows = idx; g.a_nrows = n;
     g.b = aoR; g.ldb = m; g.b_trans = 1;
-    g.c = B.z; g.ldc = n;
+    g.c = B.z; g.ldc = k; g.c_trans = 1;
     g.alpha = 1.0; g.beta = 0.0; g.epi_square = 1; g.batch = 1;
     THC_TRY(gemm_f64(g, s));
   }
   THC_HIP(hipEventRecord(ev[1], s));
   // 2. CC^T = ZC^T[:, idx]; Cholesky; explicit inverse
-  hipLaunchKernelGGL(gather_cols, dim3(grid_for((long)k * k)), dim3(256), 0, s, B.z, n, idx, k, B.c);
+  hipLaunchKernelGGL(gather_rows_t, dim3(grid_for((long)k * k)), dim3(256), 0, s, B.z, n, idx, k, B.c);
   hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * k)), dim3(256), 0, s, B.c, B.l, (long)k * k);
   THC_HIP(hipGetLastError());
   THC_TRY(cholesky_f64(B.l, k, k, B.info, B.chol_w, s));
@@ -184,24 +197,29 @@ int isdf_run(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int
     GemmArgs g = {};
     g.m = k; g.n = n; g.k = k;
     g.a = B.ci; g.lda = k; g.a_trans = 0;
-    g.b = B.z; g.ldb = n; g.b_trans = 0;
+    g.b = B.z; g.ldb = k; g.b_trans = 1;
     g.c = B.x; g.ldc = n;
     g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
     THC_TRY(gemm_f64(g, s));
   }
   THC_HIP(hipEventRecord(ev[3], s));
-  // 3b. FP64 iterative refinement: R = Z - C X; X += C^-1 R
+  // 3b. FP64 iterative refinement in the transposed layout:
+  //     Rt = Zt - X^T C^T;  X += C^-1 Rt^T
   for (int it = 0; it < refine; ++it) {
     hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * n)), dim3(256), 0, s, B.z, B.r, (long)k * n);
     GemmArgs g = {};
-    g.m = k; g.n = n; g.k = k;
-    g.a = B.c; g.lda = k; g.a_trans = 0;
-    g.b = B.x; g.ldb = n; g.b_trans = 0;
-    g.c = B.r; g.ldc = n;
+    g.m = n; g.n = k; g.k = k;
+    g.a = B.x; g.lda = n; g.a_trans = 1;
+    g.b = B.c; g.ldb = k; g.b_trans = 1;
+    g.c = B.r; g.ldc = k;
     g.alpha = -1.0; g.beta = 1.0; g.batch = 1;
     THC_TRY(gemm_f64(g, s));
-    GemmArgs h = g;
-    h.a = B.ci; h.b = B.r; h.c = B.x; h.alpha = 1.0; h.beta = 1.0;
+    GemmArgs h = {};
+    h.m = k; h.n = n; h.k = k;
+    h.a = B.ci; h.lda = k; h.a_trans = 0;
+    h.b = B.r; h.ldb = k; h.b_trans = 1;
+    h.c = B.x; h.ldc = n;
+    h.alpha = 1.0; h.beta = 1.0; h.batch = 1;
     THC_TRY(gemm_f64(h, s));
   }
   THC_HIP(hipEventRecord(ev[4], s));
@@ -213,8 +231,12 @@ int isdf_run(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int
     THC_TRY(fft_coulomb(B.x32, k, fft, coulG, B.b, 1, B.fft_w, s));
   }
   THC_HIP(hipEventRecord(ev[5], s));
-  // 5. V = B B^T, FP64 sum of the split-K partials
-  THC_TRY(syrk_splitk(B.b, k, g2, g2, prec == THC_PREC_FP64 ? 0 : 1, v_out, k, B.syrk_w, s));
+  // 5. V = B B^T, FP64 sum of the split-K partials (exact int8 digit products on the mixed path)
+  if (prec == THC_PREC_MIXED) {
+    THC_TRY(oz_syrk(B.b, k, g2, g2, 1, v_out, k, B.syrk_w, s));
+  } else {
+    THC_TRY(syrk_splitk(B.b, k, g2, g2, prec == THC_PREC_FP64 ? 0 : 1, v_out, k, B.syrk_w, s));
+  }
   THC_HIP(hipEventRecord(ev[6], s));
   // 6. Luv = chol(V)
   if (want_luv && luv_out) {

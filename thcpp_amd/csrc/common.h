@@ -81,6 +81,7 @@ int ensure_pinned(thc_ctx* ctx, size_t bytes);
 //   a_trans = 0: A[m * lda + k]  (row-major m x k)   1: A[k * lda + m]
 //   b_trans = 0: B[k * ldb + n]  (row-major k x n)   1: B[n * ldb + k]
 //   a_rows: optional row gather for a_trans = 0 (row m of op(A) = row a_rows[m] of A)
+//   c_trans: write the result transposed (element (m, n) at c[n * ldc + m])
 //   epi_square: C = (alpha * acc)^2 (beta ignored)
 //   lower: only tiles on/below the diagonal are computed (m, n square)
 //   batch: blockIdx.z, strides in elements
@@ -90,6 +91,7 @@ struct GemmArgs {
   long a_nrows;  // rows of A addressable through a_rows (gathered rows outside [0, a_nrows) read as 0)
   const void* b; long ldb; int b_trans;
   void* c; long ldc;
+  int c_trans;   // store C^T: element (m, n) at c[n * ldc + m]
   double alpha, beta;
   int epi_square;
   int lower;
@@ -114,6 +116,12 @@ size_t spd_inverse_work_bytes(int k);
 int spd_inverse_from_chol(const double* l, int k, long lda, double* ainv, void* work, hipStream_t s);
 int zero_upper(double* a, int k, long lda, hipStream_t s);
 int symmetrize_from_lower(double* a, int k, long lda, hipStream_t s);
+
+// ---- int8-emulated Gram (ozaki.hip) ----------------------------------------
+// V = B B^T (B: k x g2, f64 if prec 0, f32 if prec 1, row stride ld) from
+// exact int8 digit products, written full symmetric into v.
+size_t oz_syrk_work_bytes(int k, long g2);
+int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s);
 
 // ---- FFT (fft.hip) ----------------------------------------------------------
 size_t fft_work_bytes(int k, const int n[3], int prec);

@@ -1,13 +1,20 @@
 // LDS-tiled MFMA GEMM for gfx950 (FP64: v_mfma_f64_16x16x4f64, FP32:
 // v_mfma_f32_16x16x4f32), 256 threads = 4 wave64 in a 2 x 2 layout.
 //
-// Operand fragments (checked on the device by tests/test_gpu_gemm.py):
+// Operand fragments (checked on the device by the GPU tests):
 //   A 16x4 block: lane l holds A[l % 16][l / 16]
 //   B 4x16 block: lane l holds B[l / 16][l % 16]
 //   D 16x16, 4 values per lane, column l % 16, row
 //       f64: (l / 16) + 4 v        f32: 4 (l / 16) + v
-// Tiles are staged k-major in LDS (As[k][m], Bs[k][n]) with one element of
-// padding per row so the transposing stores do not all hit one bank.
+//
+// Operand layouts are template parameters.  An operand that is contiguous
+// along k (A row-major, or B given as n x k) is staged in LDS k-contiguous
+// (row stride BK + 1), one that is contiguous along m / n is staged k-major
+// (row stride BM + 1 / BN + 1); both make the 16-lane fragment reads hit 16
+// distinct bank pairs.  Each thread fetches a run of 8 (or 4) contiguous
+// elements per tile.  Interior tiles take an unpredicated path (vector loads
+// when the operands are 16-byte aligned); only edge tiles test bounds.
+// Rows gathered through an index list (Phi_mu) are resolved once per tile.
 #include "common.h"
 
 namespace thc {
@@ -36,27 +43,23 @@ enum { EPI_STORE = 0, EPI_SQUARE = 1, EPI_PARTIAL = 2 };
 
 struct KArgs {
   long m, n, k;
-  const void* a; long lda; int a_trans; const int* a_rows; long a_nrows;
-  const void* b; long ldb; int b_trans;
-  void* c; long ldc;
+  const void* a; long lda; const int* a_rows; long a_nrows;
+  const void* b; long ldb;
+  void* c; long ldc; int c_trans;
   double alpha, beta;
   int lower;
   long stride_a, stride_b, stride_c;
   long kchunk;  // split-K chunk (EPI_PARTIAL)
 };
 
-// Decode the tile handled by this workgroup.  Tiles are walked in groups of 8
-// tile-rows so neighbouring workgroups share A rows, and workgroup ids are
-// regrouped so each of the 8 XCDs (which receive ids round-robin) gets a
-// contiguous run of tiles and therefore reuses operands in its own L2.
 __device__ inline bool tile_coords(const KArgs& p, int BM, int BN, int& tm, int& tn) {
   const int nbm = (int)((p.m + BM - 1) / BM);
   const int nbn = (int)((p.n + BN - 1) / BN);
   int id = blockIdx.x;
   const int nb = gridDim.x;
+  // ids reach the 8 XCDs round-robin: give each XCD a contiguous run of tiles
   if ((nb & 7) == 0) id = (id & 7) * (nb >> 3) + (id >> 3);
   if (p.lower) {
-    // triangular enumeration tn <= tm
     int t = (int)((sqrt(8.0 * id + 1.0) - 1.0) * 0.5);
     while ((long)(t + 1) * (t + 2) / 2 <= id) ++t;
     while ((long)t * (t + 1) / 2 > id) --t;
@@ -64,24 +67,83 @@ __device__ inline bool tile_coords(const KArgs& p, int BM, int BN, int& tm, int&
     tn = id - t * (t + 1) / 2;
     return tm < nbm;
   }
-  const int per_group = 8 * nbn;
+  // walk groups of up to 16 tile rows column by column: the B column block a
+  // group needs is read once and the A panel stays in cache
+  const int G = 16;
+  const int per_group = G * nbn;
   const int g = id / per_group;
-  const int first = g * 8;
-  const int gsize = min(nbm - first, 8);
+  const int first = g * G;
+  const int gsize = min(nbm - first, G);
   const int r = id % per_group;
   tm = first + r % gsize;
   tn = r / gsize;
   return tm < nbm && tn < nbn;
 }
 
-template <class T, int BM, int BN, int BK, int TM, int TN, int EPI>
+// Per-thread share of one operand tile: RUN contiguous elements of one line.
+//   KCONTIG: the operand is contiguous along k (lines are m or n indices)
+//   otherwise contiguous along m / n (lines are k indices)
+template <class T, int BMN, int BK, bool KCONTIG>
+struct TileMap {
+  static constexpr int ELEMS = BMN * BK / 256;
+  static constexpr int RUN = ELEMS;  // one run per thread
+  __device__ static inline void coords(int tid, int& line, int& off) {
+    if (KCONTIG) {  // BMN lines of BK
+      constexpr int TPL = BK / RUN;
+      line = tid / TPL;
+      off = (tid % TPL) * RUN;
+    } else {        // BK lines of BMN
+      constexpr int TPL = BMN / RUN;
+      line = tid / TPL;
+      off = (tid % TPL) * RUN;
+    }
+  }
+};
+
+template <class T, int N>
+__device__ inline void load_run(const T* __restrict__ src, T* dst, bool vec) {
+  if (vec) {
+    if (sizeof(T) == 8) {
+      typedef double v2 __attribute__((ext_vector_type(2)));
+      const v2* s = reinterpret_cast<const v2*>(src);
+#pragma unroll
+      for (int e = 0; e < N / 2; ++e) {
+        v2 x = s[e];
+        dst[2 * e] = x[0];
+        dst[2 * e + 1] = x[1];
+      }
+    } else {
+      typedef float v4 __attribute__((ext_vector_type(4)));
+      const v4* s = reinterpret_cast<const v4*>(src);
+#pragma unroll
+      for (int e = 0; e < N / 4; ++e) {
+        v4 x = s[e];
+        dst[4 * e] = x[0];
+        dst[4 * e + 1] = x[1];
+        dst[4 * e + 2] = x[2];
+        dst[4 * e + 3] = x[3];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < N; ++e) dst[e] = src[e];
+  }
+}
+
+template <class T, int BM, int BN, int BK, int TM, int TN, int EPI, bool AK, bool BK_, bool GATHER, bool VEC>
 __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
+  // AK: A contiguous along k (A[m * lda + k]); else A[k * lda + m]
+  // BK_: B contiguous along k (B[n * ldb + k]); else B[k * ldb + n]
   constexpr int WTM = TM * 16, WTN = TN * 16;
   static_assert(2 * WTM == BM && 2 * WTN == BN, "2x2 waves");
-  constexpr int LDA_S = BM + 1, LDB_S = BN + 1;
-  __shared__ T As[2][BK * LDA_S];
-  __shared__ T Bs[2][BK * LDB_S];
+  constexpr int A_LINES = AK ? BM : BK, A_LEN = AK ? BK + 1 : BM + 1;
+  constexpr int B_LINES = BK_ ? BN : BK, B_LEN = BK_ ? BK + 1 : BN + 1;
+  __shared__ T As[2][A_LINES * A_LEN];
+  __shared__ T Bs[2][B_LINES * B_LEN];
   typedef typename Mfma<T>::acc_t acc_t;
+  typedef TileMap<T, BM, BK, AK> MA;
+  typedef TileMap<T, BN, BK, BK_> MB;
+  constexpr int RA = MA::RUN, RB = MB::RUN;
 
   int tm, tn;
   if (!tile_coords(p, BM, BN, tm, tn)) return;
@@ -102,65 +164,82 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  constexpr int A_ELEMS = BM * BK / 256;
-  constexpr int B_ELEMS = BN * BK / 256;
-  T ra[A_ELEMS], rb[B_ELEMS];
+  int a_line, a_off, b_line, b_off;
+  MA::coords(tid, a_line, a_off);
+  MB::coords(tid, b_line, b_off);
 
-  // Global -> registers for the k-tile starting at k0.
-  auto load_tile = [&](long k0) {
-#pragma unroll
-    for (int e = 0; e < A_ELEMS; ++e) {
-      const int lin = e * 256 + tid;
-      int mm, kk;
-      if (p.a_trans == 0) { kk = lin % BK; mm = lin / BK; }
-      else { mm = lin % BM; kk = lin / BM; }
-      const long gm = m0 + mm, gk = k0 + kk;
-      T v = T(0);
-      if (gm < p.m && gk < kend) {
-        if (p.a_trans == 0) {
-          if (p.a_rows) {
-            const long row = p.a_rows[gm];
-            if (row >= 0 && row < p.a_nrows) v = A[row * p.lda + gk];
-          } else {
-            v = A[gm * p.lda + gk];
-          }
-        } else {
-          v = A[gk * p.lda + gm];
-        }
-      }
-      ra[e] = v;
+  // A: fixed line pointer (row m for AK, resolved through the gather list)
+  const T* a_ptr = nullptr;
+  bool a_row_ok = true;
+  if (AK) {
+    const long gm = m0 + a_line;
+    a_row_ok = gm < p.m;
+    long row = gm;
+    if (GATHER) {
+      row = a_row_ok ? (long)p.a_rows[gm] : 0;
+      a_row_ok = a_row_ok && row >= 0 && row < p.a_nrows;
     }
-#pragma unroll
-    for (int e = 0; e < B_ELEMS; ++e) {
-      const int lin = e * 256 + tid;
-      int nn, kk;
-      if (p.b_trans == 0) { nn = lin % BN; kk = lin / BN; }
-      else { kk = lin % BK; nn = lin / BK; }
-      const long gn = n0 + nn, gk = k0 + kk;
-      T v = T(0);
-      if (gn < p.n && gk < kend) {
-        v = (p.b_trans == 0) ? B[gk * p.ldb + gn] : B[gn * p.ldb + gk];
+    a_ptr = A + (a_row_ok ? row : 0) * p.lda + a_off;
+  } else {
+    a_ptr = A + (long)a_line * p.lda + m0 + a_off;
+  }
+  const T* b_ptr;
+  bool b_row_ok = true;
+  if (BK_) {
+    const long gn = n0 + b_line;
+    b_row_ok = gn < p.n;
+    b_ptr = B + (b_row_ok ? gn : 0) * p.ldb + b_off;
+  } else {
+    b_ptr = B + (long)b_line * p.ldb + n0 + b_off;
+  }
+  // operand rows outside the matrix read a clamped valid row and are zeroed
+  const bool a_mn_full = AK || (m0 + BM <= p.m);
+  const bool b_mn_full = BK_ || (n0 + BN <= p.n);
+
+  T ra[RA], rb[RB];
+
+  auto load_tile = [&](long k0) {
+    const bool full_k = k0 + BK <= kend;
+    if (full_k && a_mn_full) {
+      if (AK) {
+        load_run<T, RA>(a_ptr + k0, ra, VEC);  // invalid rows are zeroed in store_tile
+      } else {
+        load_run<T, RA>(a_ptr + k0 * p.lda, ra, VEC);
       }
-      rb[e] = v;
+    } else {
+#pragma unroll
+      for (int e = 0; e < RA; ++e) {
+        bool ok;
+        long off;
+        if (AK) { ok = a_row_ok && (k0 + a_off + e < kend); off = k0 + e; }
+        else { ok = (k0 + a_line < kend) && (m0 + a_off + e < p.m); off = k0 * p.lda + e; }
+        ra[e] = ok ? a_ptr[off] : T(0);
+      }
+    }
+    if (full_k && b_mn_full) {
+      if (BK_) {
+        load_run<T, RB>(b_ptr + k0, rb, VEC);
+      } else {
+        load_run<T, RB>(b_ptr + k0 * p.ldb, rb, VEC);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < RB; ++e) {
+        bool ok;
+        long off;
+        if (BK_) { ok = b_row_ok && (k0 + b_off + e < kend); off = k0 + e; }
+        else { ok = (k0 + b_line < kend) && (n0 + b_off + e < p.n); off = k0 * p.ldb + e; }
+        rb[e] = ok ? b_ptr[off] : T(0);
+      }
     }
   };
   auto store_tile = [&](int buf) {
+    T* as = As[buf] + a_line * A_LEN + a_off;
+    T* bs = Bs[buf] + b_line * B_LEN + b_off;
 #pragma unroll
-    for (int e = 0; e < A_ELEMS; ++e) {
-      const int lin = e * 256 + tid;
-      int mm, kk;
-      if (p.a_trans == 0) { kk = lin % BK; mm = lin / BK; }
-      else { mm = lin % BM; kk = lin / BM; }
-      As[buf][kk * LDA_S + mm] = ra[e];
-    }
+    for (int e = 0; e < RA; ++e) as[e] = a_row_ok ? ra[e] : T(0);
 #pragma unroll
-    for (int e = 0; e < B_ELEMS; ++e) {
-      const int lin = e * 256 + tid;
-      int nn, kk;
-      if (p.b_trans == 0) { nn = lin % BN; kk = lin / BN; }
-      else { kk = lin % BK; nn = lin / BK; }
-      Bs[buf][kk * LDB_S + nn] = rb[e];
-    }
+    for (int e = 0; e < RB; ++e) bs[e] = b_row_ok ? rb[e] : T(0);
   };
 
   acc_t acc[TM][TN];
@@ -171,6 +250,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[i][j][v] = T(0);
 
+  const int l16 = lane & 15, l4 = lane >> 4;
   const long nk = (kend - kbeg + BK - 1) / BK;
   if (nk > 0) {
     load_tile(kbeg);
@@ -179,14 +259,22 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
     for (long t = 0; t < nk; ++t) {
       const int buf = (int)(t & 1);
       if (t + 1 < nk) load_tile(kbeg + (t + 1) * BK);
+      const T* as = As[buf];
+      const T* bs = Bs[buf];
 #pragma unroll
       for (int k4 = 0; k4 < BK / 4; ++k4) {
-        const int kr = k4 * 4 + (lane >> 4);
+        const int kr = k4 * 4 + l4;
         T af[TM], bf[TN];
 #pragma unroll
-        for (int i = 0; i < TM; ++i) af[i] = As[buf][kr * LDA_S + wm * WTM + i * 16 + (lane & 15)];
+        for (int i = 0; i < TM; ++i) {
+          const int mi = wm * WTM + i * 16 + l16;
+          af[i] = AK ? as[mi * A_LEN + kr] : as[kr * A_LEN + mi];
+        }
 #pragma unroll
-        for (int j = 0; j < TN; ++j) bf[j] = Bs[buf][kr * LDB_S + wn * WTN + j * 16 + (lane & 15)];
+        for (int j = 0; j < TN; ++j) {
+          const int nj = wn * WTN + j * 16 + l16;
+          bf[j] = BK_ ? bs[nj * B_LEN + kr] : bs[kr * B_LEN + nj];
+        }
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -197,9 +285,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
     }
   }
 
-  // Epilogue.
   if (EPI == EPI_PARTIAL) {
-    // partial tile, tile-local row-major BM x BN, indexed [split][tile]
     const long tile_lin = (long)tm * (tm + 1) / 2 + tn;
     const long ntiles = gridDim.x;
     T* W = C + ((long)blockIdx.y * ntiles + tile_lin) * (BM * BN);
@@ -210,7 +296,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int r = wm * WTM + i * 16 + Mfma<T>::row(lane, v);
-          const int cc = wn * WTN + j * 16 + (lane & 15);
+          const int cc = wn * WTN + j * 16 + l16;
           W[r * BN + cc] = acc[i][j][v];
         }
     return;
@@ -224,9 +310,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const long r = m0 + wm * WTM + i * 16 + Mfma<T>::row(lane, v);
-        const long cc = n0 + wn * WTN + j * 16 + (lane & 15);
+        const long cc = n0 + wn * WTN + j * 16 + l16;
         if (r < p.m && cc < p.n && (!p.lower || cc <= r)) {
-          T* dst = Cb + r * p.ldc + cc;
+          T* dst = p.c_trans ? Cb + cc * p.ldc + r : Cb + r * p.ldc + cc;
           if (EPI == EPI_SQUARE) {
             const T x = alpha * acc[i][j][v];
             *dst = x * x;
@@ -239,8 +325,6 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
       }
 }
 
-// Sum split-K partial tiles in split order (FP64) into the lower triangle of V
-// and mirror it to the upper triangle.
 template <class T, int BM, int BN>
 __global__ __launch_bounds__(256) void syrk_reduce_kernel(const T* work, int nsplit, long ntiles, long m,
                                                           double* v, long ldv) {
@@ -260,38 +344,67 @@ __global__ __launch_bounds__(256) void syrk_reduce_kernel(const T* work, int nsp
   }
 }
 
-template <class T, int BM, int BN, int BK, int TM, int TN>
-int launch(const GemmArgs& g, hipStream_t s) {
-  KArgs p;
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+KArgs make_args(const GemmArgs& g) {
+  KArgs p = {};
   p.m = g.m; p.n = g.n; p.k = g.k;
-  p.a = g.a; p.lda = g.lda; p.a_trans = g.a_trans; p.a_rows = g.a_rows; p.a_nrows = g.a_nrows;
-  p.b = g.b; p.ldb = g.ldb; p.b_trans = g.b_trans;
-  p.c = g.c; p.ldc = g.ldc;
+  p.a = g.a; p.lda = g.lda; p.a_rows = g.a_rows; p.a_nrows = g.a_nrows;
+  p.b = g.b; p.ldb = g.ldb;
+  p.c = g.c; p.ldc = g.ldc; p.c_trans = g.c_trans;
   p.alpha = g.alpha; p.beta = g.beta;
   p.lower = g.lower;
   p.stride_a = g.stride_a; p.stride_b = g.stride_b; p.stride_c = g.stride_c;
-  p.kchunk = 0;
-  const long nbm = (g.m + BM - 1) / BM, nbn = (g.n + BN - 1) / BN;
-  long nblk;
-  if (g.lower) nblk = nbm * (nbm + 1) / 2;
-  else nblk = nbm * nbn;
-  if (nblk <= 0 || g.batch <= 0) return THC_OK;
-  if (nblk > 0x7fffffffL) return THC_ERR_ARG;
-  dim3 grid((unsigned)nblk, 1, (unsigned)g.batch);
-  if (g.epi_square)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, TM, TN, EPI_SQUARE>), grid, dim3(256), 0, s, p);
+  return p;
+}
+
+template <class T, int BM, int BN, int BK, int TM, int TN, int EPI, bool AK, bool BKc, bool GA>
+void launch_vec(const KArgs& p, bool vec, dim3 grid, hipStream_t s) {
+  if (vec)
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, TM, TN, EPI, AK, BKc, GA, true>), grid, dim3(256), 0, s, p);
   else
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, TM, TN, EPI_STORE>), grid, dim3(256), 0, s, p);
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, TM, TN, EPI, AK, BKc, GA, false>), grid, dim3(256), 0, s, p);
+}
+
+// Every operand layout the path uses: (A k-contiguous?, B k-contiguous?, gather?)
+template <class T, int BM, int BN, int BK, int TM, int TN, int EPI>
+int launch_layout(const GemmArgs& g, const KArgs& p, dim3 grid, hipStream_t s) {
+  const int vw = 16 / (int)sizeof(T);  // elements per 16 bytes
+  const bool vec = aligned16(g.a) && aligned16(g.b) && g.lda % vw == 0 && g.ldb % vw == 0 &&
+                   g.stride_a % vw == 0 && g.stride_b % vw == 0;
+  const bool ak = g.a_trans == 0, bk = g.b_trans == 1, ga = g.a_rows != nullptr;
+  if (ga && !ak) return THC_ERR_ARG;
+  if (ak && bk) {
+    if (ga) launch_vec<T, BM, BN, BK, TM, TN, EPI, true, true, true>(p, vec, grid, s);
+    else launch_vec<T, BM, BN, BK, TM, TN, EPI, true, true, false>(p, vec, grid, s);
+  } else if (ak && !bk) {
+    launch_vec<T, BM, BN, BK, TM, TN, EPI, true, false, false>(p, vec, grid, s);
+  } else if (!ak && !bk) {
+    launch_vec<T, BM, BN, BK, TM, TN, EPI, false, false, false>(p, vec, grid, s);
+  } else {
+    launch_vec<T, BM, BN, BK, TM, TN, EPI, false, true, false>(p, vec, grid, s);
+  }
   THC_HIP(hipGetLastError());
   return THC_OK;
+}
+
+template <class T, int BM, int BN, int BK, int TM, int TN>
+int launch(const GemmArgs& g, hipStream_t s) {
+  const KArgs p = make_args(g);
+  const long nbm = (g.m + BM - 1) / BM, nbn = (g.n + BN - 1) / BN;
+  const long nblk = g.lower ? nbm * (nbm + 1) / 2 : nbm * nbn;
+  if (nblk <= 0 || g.batch <= 0) return THC_OK;
+  if (nblk > 0x7fffffffL) return THC_ERR_ARG;
+  if (g.a_rows && g.lower) return THC_ERR_ARG;
+  dim3 grid((unsigned)nblk, 1, (unsigned)g.batch);
+  if (g.epi_square) return launch_layout<T, BM, BN, BK, TM, TN, EPI_SQUARE>(g, p, grid, s);
+  return launch_layout<T, BM, BN, BK, TM, TN, EPI_STORE>(g, p, grid, s);
 }
 
 template <class T>
 int gemm_dispatch(const GemmArgs& g, hipStream_t s) {
   if (g.m <= 0 || g.n <= 0) return THC_OK;
   if (g.lower && g.m != g.n) return THC_ERR_ARG;
-  // Small problems (factorisation panels, batched triangular blocks) use
-  // 64 x 64 tiles so more workgroups exist.
   const long tiles128 = ((g.m + 127) / 128) * ((g.n + 127) / 128) * g.batch;
   if (tiles128 < 512) return launch<T, 64, 64, 16, 2, 2>(g, s);
   return launch<T, 128, 128, 16, 4, 4>(g, s);
@@ -307,7 +420,6 @@ int gemm_f32(const GemmArgs& g, hipStream_t s) { return gemm_dispatch<float>(g, 
 static void syrk_plan(long m, long k, long& ntiles, int& nsplit, long& kchunk) {
   const long nb = (m + SY_BM - 1) / SY_BM;
   ntiles = nb * (nb + 1) / 2;
-  // aim for >= 2048 workgroups; chunks are whole BK multiples
   long want = (2048 + ntiles - 1) / ntiles;
   long maxsplit = (k + 4 * SY_BK - 1) / (4 * SY_BK);
   if (want > maxsplit) want = maxsplit;
@@ -332,20 +444,22 @@ int syrk_splitk(const void* a, long m, long k, long lda, int prec, double* v, lo
   syrk_plan(m, k, ntiles, nsplit, kchunk);
   KArgs p = {};
   p.m = m; p.n = m; p.k = k;
-  p.a = a; p.lda = lda; p.a_trans = 0; p.a_rows = nullptr;
-  p.b = a; p.ldb = lda; p.b_trans = 1;
+  p.a = a; p.lda = lda; p.a_rows = nullptr;
+  p.b = a; p.ldb = lda;
   p.c = work; p.ldc = 0;
   p.alpha = 1.0; p.beta = 0.0;
   p.lower = 1;
   p.kchunk = kchunk;
   dim3 grid((unsigned)ntiles, (unsigned)nsplit, 1);
+  const int vw = prec == 0 ? 2 : 4;
+  const bool vec = aligned16(a) && lda % vw == 0;
   if (prec == 0) {
-    hipLaunchKernelGGL((gemm_kernel<double, SY_BM, SY_BN, SY_BK, 4, 4, EPI_PARTIAL>), grid, dim3(256), 0, s, p);
+    launch_vec<double, SY_BM, SY_BN, SY_BK, 4, 4, EPI_PARTIAL, true, true, false>(p, vec, grid, s);
     THC_HIP(hipGetLastError());
     hipLaunchKernelGGL((syrk_reduce_kernel<double, SY_BM, SY_BN>), dim3((unsigned)ntiles), dim3(256), 0, s,
                        (const double*)work, nsplit, ntiles, m, v, ldv);
   } else {
-    hipLaunchKernelGGL((gemm_kernel<float, SY_BM, SY_BN, SY_BK, 4, 4, EPI_PARTIAL>), grid, dim3(256), 0, s, p);
+    launch_vec<float, SY_BM, SY_BN, SY_BK, 4, 4, EPI_PARTIAL, true, true, false>(p, vec, grid, s);
     THC_HIP(hipGetLastError());
     hipLaunchKernelGGL((syrk_reduce_kernel<float, SY_BM, SY_BN>), dim3((unsigned)ntiles), dim3(256), 0, s,
                        (const float*)work, nsplit, ntiles, m, v, ldv);

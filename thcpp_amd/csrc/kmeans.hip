@@ -200,20 +200,32 @@ __global__ __launch_bounds__(256) void cs_hist(const int* keys, long n, int nbin
 }
 
 // Column scan over tiles, in place: tile_count[t][b] -> exclusive offset of
-// tile t within bin b; total[b] = count of bin b.
+// tile t within bin b; total[b] = count of bin b.  A workgroup takes 64 bins;
+// its 4 waves scan a quarter of the tiles each, then add the earlier quarters.
 __global__ __launch_bounds__(256) void cs_colscan(int* tile_count, int ntiles, int nbins, int* total,
                                                   const State* st) {
   if (st && st->done) return;
-  const int b = blockIdx.x * 256 + threadIdx.x;
-  if (b >= nbins) return;
+  __shared__ int part[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int b = blockIdx.x * 64 + lane;
+  const int per = (ntiles + 3) / 4;
+  const int t0 = wave * per, t1 = min(ntiles, t0 + per);
   int run = 0;
-  for (int t = 0; t < ntiles; ++t) {
-    const long o = (long)t * nbins + b;
-    const int c = tile_count[o];
-    tile_count[o] = run;
-    run += c;
+  if (b < nbins)
+    for (int t = t0; t < t1; ++t) run += tile_count[(long)t * nbins + b];
+  part[wave][lane] = run;
+  __syncthreads();
+  int off = 0;
+  for (int w = 0; w < wave; ++w) off += part[w][lane];
+  if (b < nbins) {
+    for (int t = t0; t < t1; ++t) {
+      const long o = (long)t * nbins + b;
+      const int c = tile_count[o];
+      tile_count[o] = off;
+      off += c;
+    }
+    if (wave == 3) total[b] = off;
   }
-  total[b] = run;
 }
 
 // Exclusive scan of total[0..nbins) into start (one workgroup of 1024).
@@ -365,26 +377,46 @@ __global__ __launch_bounds__(256) void assign(const double* sx, const double* sy
   }
 }
 
+// One wave per cluster.  Lanes gather the members' rho and rho * x (each
+// product rounded on its own, as the reference's multiply-then-add does) into
+// LDS in ascending index order, then lanes 0..3 each run one of the four
+// sequential sums (w, x, y, z) over them.
+constexpr int CS_CHUNK = 256;
+
 __global__ __launch_bounds__(256) void centroid_sums(const int* order, const int* start, const int* total,
                                                      const double* rho, const double* grid, int k,
                                                      double* cen_out, State* st) {
   if (st && st->done) return;
-  const int c = blockIdx.x * 256 + threadIdx.x;
+  __shared__ double buf[4][4][CS_CHUNK];  // [wave][w, x, y, z][member]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.x * 4 + wave;
   if (c >= k) return;
   const int s = start[c], cnt = total[c];
-  double w = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
-  for (int t = 0; t < cnt; ++t) {
-    const long i = order[s + t];
-    const double r = rho[i];
-    w += r;
-    ax += r * grid[i * 3 + 0];
-    ay += r * grid[i * 3 + 1];
-    az += r * grid[i * 3 + 2];
+  double acc = 0.0;  // lane q < 4 owns accumulator q
+  for (int base = 0; base < cnt; base += CS_CHUNK) {
+    const int len = min(CS_CHUNK, cnt - base);
+    for (int t = lane; t < len; t += 64) {
+      const long i = order[s + base + t];
+      const double r = rho[i];
+      buf[wave][0][t] = r;
+      buf[wave][1][t] = r * grid[i * 3 + 0];
+      buf[wave][2][t] = r * grid[i * 3 + 1];
+      buf[wave][3][t] = r * grid[i * 3 + 2];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (lane < 4) {
+      const double* v = buf[wave][lane];
+      for (int t = 0; t < len; ++t) acc += v[t];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
-  if (w < 1e-8) atomicOr(&st->error, 1);
-  cen_out[c * 3 + 0] = ax / w;
-  cen_out[c * 3 + 1] = ay / w;
-  cen_out[c * 3 + 2] = az / w;
+  const double w = __shfl(acc, 0);
+  if (lane == 0 && w < 1e-8) atomicOr(&st->error, 1);
+  if (lane >= 1 && lane < 4) cen_out[c * 3 + lane - 1] = acc / w;
 }
 
 // normed_difference(new, cur) with the reference's sequential sum, % changed,
@@ -645,7 +677,7 @@ static bool carve(Arena& a, const Layout& L, Bufs& b, bool full) {
 static void enqueue_sort(const Layout& L, const Bufs& b, const int* keys, int nbins, int nbits, int* out,
                          const State* st, hipStream_t s) {
   hipLaunchKernelGGL(cs_hist, dim3(L.ncs), dim3(256), 0, s, keys, L.n, nbins, b.tc, st);
-  hipLaunchKernelGGL(cs_colscan, dim3(ceil_div(nbins, 256)), dim3(256), 0, s, b.tc, L.ncs, nbins, b.total, st);
+  hipLaunchKernelGGL(cs_colscan, dim3(ceil_div(nbins, 64)), dim3(256), 0, s, b.tc, L.ncs, nbins, b.total, st);
   hipLaunchKernelGGL(cs_totscan, dim3(1), dim3(1024), 0, s, b.total, nbins, b.start, st);
   hipLaunchKernelGGL(cs_scatter, dim3(L.ncs), dim3(256), 0, s, keys, L.n, nbins, nbits, b.tc, b.start, out, st);
 }
@@ -673,7 +705,7 @@ static void enqueue_kmeans(const Layout& L, const Bufs& b, const double* grid, c
     hipLaunchKernelGGL(assign, dim3(L.ntiles), dim3(256), 0, s, b.sx, b.sy, b.sz, b.perm, L.n, b.boxes,
                        b.cen[cur], L.k, mnew, mold, b.st);
     enqueue_sort(L, b, mnew, L.k, L.nbits_k, b.order, b.st, s);
-    hipLaunchKernelGGL(centroid_sums, dim3(ceil_div(L.k, 256)), dim3(256), 0, s, b.order, b.start, b.total, rho,
+    hipLaunchKernelGGL(centroid_sums, dim3(ceil_div(L.k, 4)), dim3(256), 0, s, b.order, b.start, b.total, rho,
                        grid, L.k, b.cen[nxt], b.st);
     hipLaunchKernelGGL(diff_control, dim3(1), dim3(1024), dyn, s, b.cen[nxt], b.cen[cur], L.k, L.n, threshold,
                        it, L.max_it, nxt, b.st, b.diffs, b.pcts);
@@ -808,7 +840,7 @@ int kmeans_update(thc_ctx* ctx, const double* rho, const double* grid, long n, c
   hipLaunchKernelGGL(init_state, dim3(1), dim3(64), 0, s, b.st, (int)MODE_LATEST);
   hipLaunchKernelGGL(check_keys, dim3(256), dim3(256), 0, s, grid_map, n, k, b.st);
   enqueue_sort(L, b, grid_map, k, L.nbits_k, order, nullptr, s);
-  hipLaunchKernelGGL(centroid_sums, dim3(ceil_div(k, 256)), dim3(256), 0, s, order, b.start, b.total, rho, grid, k,
+  hipLaunchKernelGGL(centroid_sums, dim3(ceil_div(k, 4)), dim3(256), 0, s, order, b.start, b.total, rho, grid, k,
                      centroids, b.st);
   THC_HIP(hipGetLastError());
   State hst;

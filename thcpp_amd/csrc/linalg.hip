@@ -20,55 +20,74 @@ namespace {
 
 constexpr int NB = 64;
 
-// Factor the jb x jb diagonal block at (j0, j0) in place; write its lower
+// In-LDS factor and inverse of one NB x NB lower block, one wave.
+// Factor (Cholesky-Crout): lane r owns row r and computes
+// L[r][k] = (A[r][k] - sum_{q<k} L[r][q] L[k][q]) / L[k][k]; row k is read as
+// broadcasts, the lane's own row with the (NB + 1) stride that keeps the 64
+// lanes on distinct banks.  Inverse: lane c owns column c of L^-1,
+// x[i] = (delta_ic - sum_{q<i} L[i][q] x[q]) / L[i][i].  Dot products use
+// four partial sums so loads pipeline and the dependent chain is short.
+// Rows and columns >= jb are an identity extension.  Returns the 1-based first
+// failing pivot (0 if none).
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ inline double dot_lds(const double* a, int astride, const double* b, int bstride, int n) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int q = 0;
+  for (; q + 4 <= n; q += 4) {
+    s0 += a[q * astride] * b[q * bstride];
+    s1 += a[(q + 1) * astride] * b[(q + 1) * bstride];
+    s2 += a[(q + 2) * astride] * b[(q + 2) * bstride];
+    s3 += a[(q + 3) * astride] * b[(q + 3) * bstride];
+  }
+  for (; q < n; ++q) s0 += a[q * astride] * b[q * bstride];
+  return (s0 + s1) + (s2 + s3);
+}
+
+__device__ int block_chol_inv(double (*Ls)[NB + 1], double (*Ds)[NB + 1], int jb) {
+  const int r = threadIdx.x;
+  int bad = 0;
+  for (int k = 0; k < NB; ++k) {
+    const double s = Ls[r][k] - dot_lds(&Ls[r][0], 1, &Ls[k][0], 1, k);
+    const double d = __shfl(s, k);
+    if (!(d > 0.0) && bad == 0 && k < jb) bad = k + 1;
+    const double piv = sqrt(d);
+    if (r >= k) Ls[r][k] = (r == k) ? piv : s / piv;
+    wave_sync_lds();
+  }
+  // column r of L^-1, kept in Ds[.][r]
+  for (int i = 0; i < NB; ++i) {
+    double s = (i == r) ? 1.0 : 0.0;
+    if (i > r) s = -dot_lds(&Ls[i][r], 1, &Ds[r][r], NB + 1, i - r);
+    Ds[i][r] = (i < r) ? 0.0 : s / Ls[i][i];
+  }
+  wave_sync_lds();
+  return bad;
+}
+
+// Factor the jb x jb diagonal block at (j0, j0) in place and write its lower
 // inverse to dinv (NB x NB, row-major, zero upper).  info: first failing
 // leading minor (1-based, global), kept if already set.
-__global__ __launch_bounds__(256) void potrf_diag(double* a, long lda, int j0, int jb, int* info, double* dinv) {
+__global__ __launch_bounds__(64) void potrf_diag(double* a, long lda, int j0, int jb, int* info, double* dinv) {
   __shared__ double L[NB][NB + 1];
   __shared__ double D[NB][NB + 1];
-  const int tid = threadIdx.x;
-  for (int e = tid; e < jb * jb; e += 256) {
-    const int r = e / jb, c = e % jb;
-    L[r][c] = (c <= r) ? a[(long)(j0 + r) * lda + j0 + c] : 0.0;
+  const int r = threadIdx.x;
+  for (int c = 0; c < NB; ++c) {
+    double v = 0.0;
+    if (r < jb && c < jb) v = c <= r ? a[(long)(j0 + r) * lda + j0 + c] : 0.0;
+    else if (r == c) v = 1.0;
+    L[r][c] = v;
   }
   __syncthreads();
-  for (int k = 0; k < jb; ++k) {
-    if (tid == 0) {
-      const double d = L[k][k];
-      if (!(d > 0.0) && *info == 0) *info = j0 + k + 1;
-      L[k][k] = sqrt(d);
-    }
-    __syncthreads();
-    const double piv = L[k][k];
-    for (int r = k + 1 + tid; r < jb; r += 256) L[r][k] /= piv;
-    __syncthreads();
-    const int m = jb - k - 1;
-    for (int e = tid; e < m * m; e += 256) {
-      const int r = k + 1 + e / m, c = k + 1 + e % m;
-      if (c <= r) L[r][c] -= L[r][k] * L[c][k];
-    }
-    __syncthreads();
-  }
-  // lower-triangular inverse by forward substitution, one column per thread
-  if (tid < jb) {
-    const int c = tid;
-    for (int r = 0; r < c; ++r) D[r][c] = 0.0;
-    D[c][c] = 1.0 / L[c][c];
-    for (int r = c + 1; r < jb; ++r) {
-      double s = 0.0;
-      for (int q = c; q < r; ++q) s += L[r][q] * D[q][c];
-      D[r][c] = -s / L[r][r];
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < jb * jb; e += 256) {
-    const int r = e / jb, c = e % jb;
-    if (c <= r) a[(long)(j0 + r) * lda + j0 + c] = L[r][c];
-  }
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int r = e / NB, c = e % NB;
-    dinv[e] = (r < jb && c < jb) ? D[r][c] : 0.0;
-  }
+  const int bad = block_chol_inv(L, D, jb);
+  if (r == 0 && bad && *info == 0) *info = j0 + bad;
+  if (r < jb)
+    for (int c = 0; c <= r; ++c) a[(long)(j0 + r) * lda + j0 + c] = L[r][c];
+  for (int c = 0; c < NB; ++c) dinv[r * NB + c] = (r < jb && c < jb) ? D[r][c] : 0.0;
 }
 
 __global__ __launch_bounds__(256) void copy_block(const double* src, long lds, double* dst, long ldd, long rows,
@@ -110,30 +129,19 @@ __global__ __launch_bounds__(256) void pad_lower(const double* l, int k, long ld
 }
 
 // Invert each 64 x 64 lower diagonal block of lp into linv (same positions).
-__global__ __launch_bounds__(256) void invert_diag_blocks(const double* lp, int kp, double* linv) {
+__global__ __launch_bounds__(64) void invert_diag_blocks(const double* lp, int kp, double* linv) {
   __shared__ double L[NB][NB + 1];
   __shared__ double D[NB][NB + 1];
-  const int j0 = blockIdx.x * NB, tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int r = e / NB, c = e % NB;
-    L[r][c] = lp[(long)(j0 + r) * kp + j0 + c];
+  const int j0 = blockIdx.x * NB, r = threadIdx.x;
+  for (int c = 0; c < NB; ++c) L[r][c] = lp[(long)(j0 + r) * kp + j0 + c];
+  __syncthreads();
+  for (int i = 0; i < NB; ++i) {
+    double s = (i == r) ? 1.0 : 0.0;
+    if (i > r) s = -dot_lds(&L[i][r], 1, &D[r][r], NB + 1, i - r);
+    D[i][r] = (i < r) ? 0.0 : s / L[i][i];
   }
   __syncthreads();
-  if (tid < NB) {
-    const int c = tid;
-    for (int r = 0; r < c; ++r) D[r][c] = 0.0;
-    D[c][c] = 1.0 / L[c][c];
-    for (int r = c + 1; r < NB; ++r) {
-      double s = 0.0;
-      for (int q = c; q < r; ++q) s += L[r][q] * D[q][c];
-      D[r][c] = -s / L[r][r];
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int r = e / NB, c = e % NB;
-    linv[(long)(j0 + r) * kp + j0 + c] = D[r][c];
-  }
+  for (int c = 0; c < NB; ++c) linv[(long)(j0 + r) * kp + j0 + c] = D[r][c];
 }
 
 __global__ __launch_bounds__(256) void fill_zero(double* a, long count) {
@@ -153,7 +161,7 @@ int cholesky_f64(double* a, int k, long lda, int* d_info, void* work, hipStream_
   double* panel = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + align_up(sizeof(double) * NB * NB, 256));
   for (int j0 = 0; j0 < k; j0 += NB) {
     const int jb = std::min(NB, k - j0);
-    hipLaunchKernelGGL(potrf_diag, dim3(1), dim3(256), 0, s, a, lda, j0, jb, d_info, dinv);
+    hipLaunchKernelGGL(potrf_diag, dim3(1), dim3(64), 0, s, a, lda, j0, jb, d_info, dinv);
     THC_HIP(hipGetLastError());
     const long rows = k - j0 - jb;
     if (rows <= 0) continue;
@@ -201,7 +209,7 @@ int spd_inverse_from_chol(const double* l, int k, long lda, double* ainv, void* 
   const long kk = (long)kp * kp;
   hipLaunchKernelGGL(pad_lower, dim3(grid_for(kk)), dim3(256), 0, s, l, k, lda, lp, kp);
   hipLaunchKernelGGL(fill_zero, dim3(grid_for(kk)), dim3(256), 0, s, linv, kk);
-  hipLaunchKernelGGL(invert_diag_blocks, dim3(kp / NB), dim3(256), 0, s, lp, kp, linv);
+  hipLaunchKernelGGL(invert_diag_blocks, dim3(kp / NB), dim3(64), 0, s, lp, kp, linv);
   THC_HIP(hipGetLastError());
   for (int sz = NB; sz < kp; sz *= 2) {
     const int pairs = kp / (2 * sz);

@@ -1,0 +1,219 @@
+// V = B B^T for the mixed-precision path, evaluated exactly in int8 MFMA
+// arithmetic on splits of B (an Ozaki-style emulation of the FP64 Gram).
+//
+// Split: every row of B is cut into chunks of OZ_CH columns; each (row, chunk)
+// gets a power-of-two exponent e with max |b| < 2^e, and b 2^-e in (-1, 1) is
+// written as OZ_S signed 7-bit digits, b = 2^e sum_p d_p 2^-7(p+1) + r,
+// |r| < 2^(e - 7 OZ_S).  Every step (scale by 128, truncate, subtract) is
+// exact in double.
+// Product: for one chunk, sum_g b_rg b_cg = 2^(e_r + e_c) sum_{p+q<OZ_S}
+// 2^-7(p+q+2) sum_g d_p,rg d_q,cg.  The inner sums are int8 x int8 products
+// accumulated in int32 by v_mfma_i32_16x16x64_i8, grouped by p + q (products
+// with the same p + q share a scale), exact as long as
+// OZ_S * OZ_CH * 127^2 < 2^31.  Digit products with p + q >= OZ_S are
+// dropped; they are below 2^-7(OZ_S+1) of |b_r| |b_c| per term.
+// Each workgroup takes one 128 x 128 lower tile and one chunk, converts its
+// groups to double, scales by 2^(e_r + e_c) and writes a partial tile; the
+// partials are summed in chunk order in FP64.
+#include "common.h"
+
+namespace thc {
+namespace {
+
+typedef int i4 __attribute__((ext_vector_type(4)));
+
+constexpr int OZ_S = 5;       // digits per value
+constexpr int OZ_CH = 4096;   // columns per chunk (one exponent per row and chunk)
+constexpr int OZ_T = 128;     // tile edge
+constexpr int OZ_KS = 64;     // k per MFMA step
+constexpr int OZ_LD = OZ_KS + 16;  // LDS row stride in bytes (16 bytes of padding)
+static_assert((long)OZ_S * OZ_CH * 127 * 127 < 2147483647L, "int32 group sums must not overflow");
+
+__device__ inline double block_max(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  v = red[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); ++i) v = fmax(v, red[i]);
+  return v;
+}
+
+// digits S[p][row][g] (g < g2p, zero padded) and exponents E[row][chunk]
+template <class T>
+__global__ __launch_bounds__(256) void oz_split(const T* __restrict__ b, long ld, long g2, long g2p, int k,
+                                                int nchunk, signed char* __restrict__ S, int* __restrict__ E) {
+  __shared__ double red[4];
+  const int c = blockIdx.x, row = blockIdx.y;
+  const T* src = b + (long)row * ld;
+  const long g0 = (long)c * OZ_CH;
+  constexpr int PER = OZ_CH / 256;
+  double v[PER];
+  double mx = 0.0;
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const long g = g0 + e * 256 + threadIdx.x;
+    v[e] = g < g2 ? (double)src[g] : 0.0;
+    mx = fmax(mx, fabs(v[e]));
+  }
+  mx = block_max(mx, red);
+  int ex = 0;
+  if (mx > 0.0) frexp(mx, &ex);  // mx < 2^ex
+  if (threadIdx.x == 0) E[(long)row * nchunk + c] = ex;
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const long g = g0 + e * 256 + threadIdx.x;
+    double x = ldexp(v[e], -ex);
+#pragma unroll
+    for (int p = 0; p < OZ_S; ++p) {
+      x *= 128.0;
+      const double q = trunc(x);
+      x -= q;
+      S[((long)p * k + row) * g2p + g] = (signed char)(int)q;
+    }
+  }
+}
+
+// One lower 128 x 128 tile (tm >= tn) and one chunk per workgroup; 8 waves
+// in a 2 (rows) x 4 (columns) grid, each wave 64 x 32 = 4 x 2 MFMA blocks.
+__global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S, long g2p, int k,
+                                               const int* __restrict__ E, int nchunk, double* __restrict__ work,
+                                               long ntiles) {
+  __shared__ __attribute__((aligned(16))) signed char As[OZ_S][OZ_T][OZ_LD];
+  __shared__ __attribute__((aligned(16))) signed char Bs[OZ_S][OZ_T][OZ_LD];
+  const long tile_lin = blockIdx.x;
+  int tm = (int)((sqrt(8.0 * tile_lin + 1.0) - 1.0) * 0.5);
+  while ((long)(tm + 1) * (tm + 2) / 2 <= tile_lin) ++tm;
+  while ((long)tm * (tm + 1) / 2 > tile_lin) --tm;
+  const int tn = (int)(tile_lin - (long)tm * (tm + 1) / 2);
+  const int c = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 waves
+  const int l16 = lane & 15, l4 = lane >> 4;
+
+  i4 acc[OZ_S][4][2];
+#pragma unroll
+  for (int d = 0; d < OZ_S; ++d)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[d][i][j] = i4{0, 0, 0, 0};
+
+  const long kbase = (long)c * OZ_CH;
+  for (int ks = 0; ks < OZ_CH / OZ_KS; ++ks) {
+    const long k0 = kbase + (long)ks * OZ_KS;
+    // slabs: OZ_S digit planes x 128 rows x 64 bytes for each operand, 16 B per load
+    for (int e = tid; e < OZ_S * OZ_T * 4; e += 512) {
+      const int p = e / (OZ_T * 4), r = (e / 4) % OZ_T, seg = e % 4;
+      const int ra = tm * OZ_T + r, rb = tn * OZ_T + r;
+      i4 va = i4{0, 0, 0, 0}, vb = i4{0, 0, 0, 0};
+      if (ra < k) va = *reinterpret_cast<const i4*>(S + ((long)p * k + ra) * g2p + k0 + seg * 16);
+      if (rb < k) vb = *reinterpret_cast<const i4*>(S + ((long)p * k + rb) * g2p + k0 + seg * 16);
+      *reinterpret_cast<i4*>(&As[p][r][seg * 16]) = va;
+      *reinterpret_cast<i4*>(&Bs[p][r][seg * 16]) = vb;
+    }
+    __syncthreads();
+    i4 bq[OZ_S][2];
+#pragma unroll
+    for (int q = 0; q < OZ_S; ++q)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        bq[q][j] = *reinterpret_cast<const i4*>(&Bs[q][wn * 32 + j * 16 + l16][l4 * 16]);
+#pragma unroll
+    for (int p = 0; p < OZ_S; ++p) {
+      i4 ap[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ap[i] = *reinterpret_cast<const i4*>(&As[p][wm * 64 + i * 16 + l16][l4 * 16]);
+#pragma unroll
+      for (int q = 0; q < OZ_S - p; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            acc[p + q][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ap[i], bq[q][j], acc[p + q][i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // scale and write the partial tile: D[4 (l/16) + v][l % 16]
+  double* W = work + ((long)c * ntiles + tile_lin) * (OZ_T * OZ_T);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rl = wm * 64 + i * 16 + 4 * l4 + v;
+        const int cl = wn * 32 + j * 16 + l16;
+        const int r = tm * OZ_T + rl, col = tn * OZ_T + cl;
+        double val = 0.0;
+        if (r < k && col < k) {
+          double sum = 0.0;
+#pragma unroll
+          for (int d = OZ_S - 1; d >= 0; --d) sum += ldexp((double)acc[d][i][j][v], -7 * (d + 2));
+          val = ldexp(sum, E[(long)r * nchunk + c] + E[(long)col * nchunk + c]);
+        }
+        W[rl * OZ_T + cl] = val;
+      }
+}
+
+__global__ __launch_bounds__(256) void oz_reduce(const double* work, int nchunk, long ntiles, int k, double* v,
+                                                 long ldv) {
+  const long tile_lin = blockIdx.x;
+  int tm = (int)((sqrt(8.0 * tile_lin + 1.0) - 1.0) * 0.5);
+  while ((long)(tm + 1) * (tm + 2) / 2 <= tile_lin) ++tm;
+  while ((long)tm * (tm + 1) / 2 > tile_lin) --tm;
+  const int tn = (int)(tile_lin - (long)tm * (tm + 1) / 2);
+  for (int e = threadIdx.x; e < OZ_T * OZ_T; e += 256) {
+    const int r = e / OZ_T, c = e % OZ_T;
+    const long gr = (long)tm * OZ_T + r, gc = (long)tn * OZ_T + c;
+    if (gr >= k || gc >= k || gc > gr) continue;
+    double s = 0.0;
+    for (int ch = 0; ch < nchunk; ++ch) s += work[((long)ch * ntiles + tile_lin) * (OZ_T * OZ_T) + e];
+    v[gr * ldv + gc] = s;
+    v[gc * ldv + gr] = s;
+  }
+}
+
+void oz_plan(int k, long g2, int& nchunk, long& g2p, long& ntiles) {
+  nchunk = (int)((g2 + OZ_CH - 1) / OZ_CH);
+  g2p = (long)nchunk * OZ_CH;
+  const long nb = (k + OZ_T - 1) / OZ_T;
+  ntiles = nb * (nb + 1) / 2;
+}
+
+}  // namespace
+
+size_t oz_syrk_work_bytes(int k, long g2) {
+  int nchunk;
+  long g2p, ntiles;
+  oz_plan(k, g2, nchunk, g2p, ntiles);
+  return align_up((size_t)OZ_S * k * g2p, 256) + align_up(sizeof(int) * (size_t)k * nchunk, 256) +
+         align_up(sizeof(double) * (size_t)nchunk * ntiles * OZ_T * OZ_T, 256) + 256;
+}
+
+int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s) {
+  if (k <= 0) return THC_OK;
+  int nchunk;
+  long g2p, ntiles;
+  oz_plan(k, g2, nchunk, g2p, ntiles);
+  char* w = reinterpret_cast<char*>(work);
+  signed char* S = reinterpret_cast<signed char*>(w);
+  w += align_up((size_t)OZ_S * k * g2p, 256);
+  int* E = reinterpret_cast<int*>(w);
+  w += align_up(sizeof(int) * (size_t)k * nchunk, 256);
+  double* part = reinterpret_cast<double*>(w);
+  if (prec == 0)
+    hipLaunchKernelGGL(oz_split<double>, dim3(nchunk, k), dim3(256), 0, s, (const double*)b, ld, g2, g2p, k, nchunk,
+                       S, E);
+  else
+    hipLaunchKernelGGL(oz_split<float>, dim3(nchunk, k), dim3(256), 0, s, (const float*)b, ld, g2, g2p, k, nchunk,
+                       S, E);
+  THC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)ntiles, nchunk), dim3(512), 0, s, S, g2p, k, E, nchunk, part, ntiles);
+  THC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(oz_reduce, dim3((unsigned)ntiles), dim3(256), 0, s, part, nchunk, ntiles, k, v, ldv);
+  THC_HIP(hipGetLastError());
+  return THC_OK;
+}
+
+}  // namespace thc
