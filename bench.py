@@ -10,6 +10,7 @@ is timed beside it on the same inputs, and the relative V error between the
 two is reported.
 
     python bench.py [--steps K] [--warmup W] [--config NAME] [--no-cpu]
+    python bench.py --copy-cal     # 1 GiB device copy x3, for HBM counter calibration
 """
 from __future__ import annotations
 
@@ -35,8 +36,24 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--config", default="diamond64_n64_k2048")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-batch", action="store_true", help="skip the 32-cell batched leg")
     p.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU sample work")
+    p.add_argument("--copy-cal", action="store_true",
+                   help="only run a 1 GiB float64 device copy three times (the known-size kernel that "
+                        "FETCH_SIZE / WRITE_SIZE counters are scaled against) and exit")
     return p.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 27  # 2^27 doubles = 1 GiB
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+    return 0
 
 
 def cpu_baseline(s, budget):
@@ -89,8 +106,39 @@ def cpu_baseline(s, budget):
                        % (kmu, s.K, t_kmeans, t_z, t_qr, t_apply, t_fft, t_muv))}
 
 
+def batched_leg(ctx, steps, warmup):
+    """The uneven 32-cell batch (side 6-14 bohr, 8-64 atoms, grids 24^3-48^3,
+    N_mu = 8-10 M) through thc_factorize_batched, mixed precision, on 4
+    streams and, for comparison, on 1."""
+    import torch
+    from thcpp_amd import _lib as L
+    from thcpp_amd import synthetic
+    batch = synthetic.make_batch(count=32, seed=0, max_it=20)
+    systems = [{"grid": ctx.f64(s.grid), "rho": ctx.f64(s.rho), "aoR": ctx.f64(s.aoR), "coulG": ctx.f64(s.coulG),
+                "fft_grid": s.n, "thc_cfac": s.thc_cfac, "max_it": s.max_it, "seed": s.seed} for s in batch]
+    out = {"workload": "batch of 32 uneven diamond cells, 20 k-means iterations each",
+           "systems": len(batch), "sum_N": int(sum(s.N for s in batch)),
+           "sum_N_mu": int(sum(s.K for s in batch))}
+    for streams in (4, 1):
+        for _ in range(warmup):
+            ctx.factorize_batched(systems, precision=L.PREC_MIXED, max_streams=streams)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st, _res = ctx.factorize_batched(systems, precision=L.PREC_MIXED, max_streams=streams)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        if st != L.THC_OK:
+            raise RuntimeError("batched factorisation status %d" % st)
+        out["streams_%d" % streams] = {"ms_per_batch": round(dt * 1e3, 3),
+                                       "systems_per_s": round(len(batch) / dt, 2)}
+    return out
+
+
 def main():
     a = parse()
+    if a.copy_cal:
+        return copy_calibration()
     if a.gpus != 1:
         print("bench.py measures one GPU only (the ISDF path has no multi-GPU sharding); run with --gpus 1",
               file=sys.stderr)
@@ -154,6 +202,8 @@ def main():
                      "achieved": round(achieved, 2), "peak": PEAK["fp64_tflops"], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["fp64_tflops"], 4), "traffic": None},
     }
+    if not a.no_batch:
+        line["batched"] = batched_leg(ctx, max(1, min(a.steps, 3)), 1)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(s, a.cpu_budget)
     print(json.dumps(line))

@@ -183,6 +183,44 @@ int thc_factorize(thc_ctx *ctx, const double *grid, const double *rho, const dou
                   int *interp_idx, double *v, double *luv,
                   thc_kmeans_stats *kstats, thc_isdf_stats *istats);
 
+/* ---------------------------------------------------------------- batches */
+
+/* One system of a batch: inputs as for thc_factorize (device pointers), with
+ * k = thc_cfac * m interpolation points.  The out fields are written by
+ * thc_factorize_batched; kstats.diffs / kstats.pcts may point to host arrays
+ * of max_it entries (or be NULL) before the call. */
+typedef struct thc_system {
+  const double *grid;      /* n x 3 */
+  const double *rho;       /* n */
+  const double *aoR;       /* n x m */
+  const double *coulG;     /* n */
+  long n;
+  int m;
+  int thc_cfac;
+  int max_it;
+  double threshold;
+  unsigned int seed;
+  int fft_grid[3];
+  int *interp_idx;         /* out, k ints */
+  double *v;               /* out, k x k */
+  double *luv;             /* out, k x k (nullable) */
+  int status;              /* out: thc_factorize's status for this system */
+  thc_kmeans_stats kstats; /* out */
+  thc_isdf_stats istats;   /* out */
+} thc_system;
+
+/* Factorise many independent systems of uneven sizes (the reference runs one
+ * system per thcpp.x invocation, src/main.cpp:18-77; this is that loop).
+ * Systems are spread, largest first, over up to max_streams HIP streams of
+ * the context (<= 0: 4), each with its own workspace and driven by its own
+ * host thread, so that small systems overlap on the GPU.  Each system's
+ * results equal thc_factorize's on the same inputs.  Returns THC_OK when
+ * every system returned THC_OK, else the first error status by system order
+ * (or, without errors, the first warning); per-system statuses are in
+ * systems[i].status. */
+int thc_factorize_batched(thc_ctx *ctx, int nsys, thc_system *systems, const thc_isdf_opts *opts,
+                          int max_streams);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,73 @@
+"""Batched factorisation of uneven cells (thc_factorize_batched): every
+system's result equals the single-system factorisation on the same inputs,
+its k-means points equal the oracle's (src/kmeans.cpp:187-264, one system at a
+time as src/main.cpp:50-60 runs it), and a failing system leaves the others
+computed."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import kmeans_oracle as KO
+from thcpp_amd import _lib as L
+from thcpp_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def batch():
+    # grids 24^3 .. 45^3 (odd and 3/5-smooth edges), M 16..80, N_mu 128..720
+    return synthetic.make_batch(count=6, seed=3, max_it=6)
+
+
+def as_dict(s):
+    return {"grid": s.grid, "rho": s.rho, "aoR": s.aoR, "coulG": s.coulG, "fft_grid": s.n,
+            "thc_cfac": s.thc_cfac, "max_it": s.max_it, "seed": s.seed}
+
+
+@pytest.mark.parametrize("prec", [L.PREC_FP64, L.PREC_MIXED])
+@pytest.mark.parametrize("streams", [1, 4])
+def test_batched_equals_single(ctx, batch, prec, streams):
+    print("batched vs single: precision %d, %d streams, %d systems" % (prec, streams, len(batch)), flush=True)
+    st, outs = ctx.factorize_batched([as_dict(s) for s in batch], precision=prec, max_streams=streams)
+    assert st == L.THC_OK
+    for s, o in zip(batch, outs):
+        assert o["status"] == L.THC_OK
+        assert o["iters"] == s.max_it
+        one = ctx.factorize(s.grid, s.rho, s.aoR, s.thc_cfac, s.max_it, s.n, s.coulG, precision=prec)
+        # same kernels on the same shapes, fixed reduction orders: bitwise equal
+        assert torch.equal(o["interp_idx"], one["interp_idx"])
+        assert torch.equal(o["V"], one["V"])
+        assert torch.equal(o["Luv"], one["Luv"])
+
+
+def test_batched_kmeans_matches_oracle(ctx, batch):
+    st, outs = ctx.factorize_batched([as_dict(s) for s in batch], precision=L.PREC_MIXED)
+    assert st == L.THC_OK
+    for s, o in zip(batch, outs):
+        ref = KO.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, s.seed)
+        assert ref["status"] == 0
+        assert np.array_equal(o["interp_idx"].cpu().numpy(), ref["interp_idx"]), s.n
+
+
+def test_batched_error_isolated(ctx, batch):
+    """A system with thc_cfac = 0 (no interpolation points) fails with
+    THC_ERR_ARG; the rest are still factorised and the call returns that
+    system's status."""
+    systems = [as_dict(s) for s in batch[:3]]
+    bad = dict(systems[1])
+    bad["thc_cfac"] = 0
+    systems[1] = bad
+    st, outs = ctx.factorize_batched(systems, precision=L.PREC_FP64)
+    assert st == L.THC_ERR_ARG
+    assert outs[1]["status"] == L.THC_ERR_ARG
+    for i in (0, 2):
+        s = batch[i]
+        assert outs[i]["status"] == L.THC_OK
+        one = ctx.factorize(s.grid, s.rho, s.aoR, s.thc_cfac, s.max_it, s.n, s.coulG)
+        assert torch.equal(outs[i]["V"], one["V"])
+
+
+def test_batched_empty(ctx):
+    st, outs = ctx.factorize_batched([])
+    assert st == L.THC_OK and outs == []

@@ -45,6 +45,13 @@ class IsdfStats(ctypes.Structure):
     _fields_ = [("cct_chol_info", _i), ("luv_chol_info", _i), ("stage_ms", _d * 8)]
 
 
+class System(ctypes.Structure):
+    _fields_ = [("grid", _vp), ("rho", _vp), ("aoR", _vp), ("coulG", _vp), ("n", _l), ("m", _i),
+                ("thc_cfac", _i), ("max_it", _i), ("threshold", _d), ("seed", _u), ("fft_grid", _i * 3),
+                ("interp_idx", _vp), ("v", _vp), ("luv", _vp), ("status", _i), ("kstats", KmeansStats),
+                ("istats", IsdfStats)]
+
+
 # name -> (restype, argtypes); every entry point of include/thcpp_amd.h
 SIGNATURES = {
     "thc_ctx_create": (_i, [_i, _vp, ctypes.POINTER(_vp)]),
@@ -69,6 +76,7 @@ SIGNATURES = {
     "thc_factorize": (_i, [_vp, _vp, _vp, _vp, _l, _i, _i, _i, _d, _u, ctypes.POINTER(_i), _vp,
                            ctypes.POINTER(IsdfOpts), _vp, _vp, _vp, ctypes.POINTER(KmeansStats),
                            ctypes.POINTER(IsdfStats)]),
+    "thc_factorize_batched": (_i, [_vp, _i, ctypes.POINTER(System), ctypes.POINTER(IsdfOpts), _i]),
 }
 
 _lib = None

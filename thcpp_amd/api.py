@@ -176,3 +176,36 @@ class Context:
                                    ctypes.byref(ist))
         return {"status": st, "interp_idx": idx, "V": v, "Luv": luv, "kmeans_ms": ks.elapsed_ms,
                 "iters": ks.iterations, "stage_ms": list(ist.stage_ms)}
+
+    def factorize_batched(self, systems, precision=L.PREC_FP64, refine=0, want_luv=True, max_streams=4):
+        """systems: sequence of dicts with grid, rho, aoR, coulG, fft_grid, thc_cfac, max_it and optional
+        threshold (default -1) and seed (default 0).  Returns (status, list of per-system result dicts)."""
+        keep, outs = [], []
+        arr = (L.System * len(systems))()
+        for i, sd in enumerate(systems):
+            g, r, a = self.f64(sd["grid"]), self.f64(sd["rho"]).reshape(-1), self.f64(sd["aoR"])
+            cg = self.f64(sd["coulG"]).reshape(-1)
+            n, m = a.shape
+            k = int(sd["thc_cfac"]) * m
+            idx = torch.empty(k, dtype=torch.int32, device=self.device)
+            v = torch.empty((k, k), dtype=torch.float64, device=self.device)
+            luv = torch.empty((k, k), dtype=torch.float64, device=self.device) if want_luv else None
+            keep += [g, r, a, cg]
+            q = arr[i]
+            q.grid, q.rho, q.aoR, q.coulG = g.data_ptr(), r.data_ptr(), a.data_ptr(), cg.data_ptr()
+            q.n, q.m, q.thc_cfac, q.max_it = n, m, int(sd["thc_cfac"]), int(sd["max_it"])
+            q.threshold, q.seed = float(sd.get("threshold", -1.0)), int(sd.get("seed", 0))
+            for j in range(3):
+                q.fft_grid[j] = int(sd["fft_grid"][j])
+            q.interp_idx, q.v = idx.data_ptr(), v.data_ptr()
+            q.luv = luv.data_ptr() if luv is not None else None
+            outs.append({"interp_idx": idx, "V": v, "Luv": luv})
+        opts = L.IsdfOpts(precision, refine, 1 if want_luv else 0)
+        st = L.lib().thc_factorize_batched(self.h, len(systems), arr, ctypes.byref(opts), max_streams)
+        for i, o in enumerate(outs):
+            o["status"] = arr[i].status
+            o["iters"] = arr[i].kstats.iterations
+            o["kmeans_ms"] = arr[i].kstats.elapsed_ms
+            o["stage_ms"] = list(arr[i].istats.stage_ms)
+        del keep
+        return st, outs

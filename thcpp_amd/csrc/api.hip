@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "common.h"
 #include "kmeans.h"
@@ -77,9 +78,6 @@ __global__ __launch_bounds__(256) void gather_rows_t(const double* zt, long n, c
   }
 }
 
-__global__ __launch_bounds__(256) void to_f32(const double* x, float* y, long count) {
-  for (long e = blockIdx.x * 256L + threadIdx.x; e < count; e += 256L * gridDim.x) y[e] = (float)x[e];
-}
 
 __global__ __launch_bounds__(256) void copy_f64(const double* x, double* y, long count) {
   for (long e = blockIdx.x * 256L + threadIdx.x; e < count; e += 256L * gridDim.x) y[e] = x[e];
@@ -95,7 +93,6 @@ inline int grid_for(long count) { return (int)std::min<long>(8192, std::max<long
 struct IsdfBufs {
   double *z, *x, *c, *l, *ci, *r, *v;
   void* b;      // weighted half spectrum (f64 or f32)
-  float* x32;   // FP32 copy of X for the FP32 FFT
   void *chol_w, *inv_w, *fft_w, *syrk_w;
   int* info;
 };
@@ -109,7 +106,6 @@ size_t isdf_bytes(long n, int k, const int fft[3], int prec, int refine, bool ow
   b += 4 * carve_bytes<double>(kk);                          // c, l, ci, v
   if (refine > 0) b += carve_bytes<double>((size_t)k * n);   // r
   b += prec == 0 ? carve_bytes<double>((size_t)k * g2) : carve_bytes<float>((size_t)k * g2);
-  if (prec != 0) b += carve_bytes<float>((size_t)k * n);
   const int fprec = prec == 0 ? 0 : 1;  // FFT / spectrum precision
   b += chol_work_bytes(k) + 256 + spd_inverse_work_bytes(k) + 256 + fft_work_bytes(k, fft, fprec) + 256 +
        (prec == THC_PREC_MIXED ? oz_syrk_work_bytes(k, g2) : syrk_work_bytes(k, g2, fprec)) + 256;
@@ -130,14 +126,13 @@ bool isdf_carve(Arena& a, long n, int k, const int fft[3], int prec, int refine,
   B.r = refine > 0 ? a.take<double>((size_t)k * n) : nullptr;
   if (prec == 0) B.b = a.take<double>((size_t)k * g2);
   else B.b = a.take<float>((size_t)k * g2);
-  B.x32 = prec != 0 ? a.take<float>((size_t)k * n) : nullptr;
   B.chol_w = a.take<char>(chol_work_bytes(k));
   B.inv_w = a.take<char>(spd_inverse_work_bytes(k));
   const int fprec = prec == 0 ? 0 : 1;
   B.fft_w = a.take<char>(fft_work_bytes(k, fft, fprec));
   B.syrk_w = a.take<char>(prec == THC_PREC_MIXED ? oz_syrk_work_bytes(k, g2) : syrk_work_bytes(k, g2, fprec));
   B.info = a.take<int>(16);
-  return B.z && B.x && B.c && B.l && B.ci && B.v && (refine <= 0 || B.r) && B.b && (prec == 0 || B.x32) &&
+  return B.z && B.x && B.c && B.l && B.ci && B.v && (refine <= 0 || B.r) && B.b &&
          B.chol_w && B.inv_w && B.fft_w && B.syrk_w && B.info;
 }
 
@@ -173,14 +168,15 @@ int isdf_run(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int
   hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, B.info, 16);
   hipLaunchKernelGGL(check_idx, dim3(grid_for(k)), dim3(256), 0, s, idx, k, n, B.info + 2);
   THC_HIP(hipEventRecord(ev[0], s));
-  // 1. ZC^T = (Phi_mu Phi^T)^2: A rows gathered by idx, epilogue squares; stored
-  //    transposed (N x K, Zt[r][mu]) so the solve reads both operands along k
+  // 1. Zt = (Phi Phi_mu^T)^2, N x K row-major (Zt[r][mu] = ZC^T[mu][r]): B rows
+  //    gathered by idx, epilogue squares; this layout lets the solve read both
+  //    operands along k and keeps the 4 GB store row-contiguous
   {
     GemmArgs g = {};
-    g.m = k; g.n = n; g.k = m;
-    g.a = aoR; g.lda = m; g.a_trans = 0; g.a_rows = idx; g.a_nrows = n;
-    g.b = aoR; g.ldb = m; g.b_trans = 1;
-    g.c = B.z; g.ldc = k; g.c_trans = 1;
+    g.m = n; g.n = k; g.k = m;
+    g.a = aoR; g.lda = m; g.a_trans = 0;
+    g.b = aoR; g.ldb = m; g.b_trans = 1; g.b_rows = idx; g.b_nrows = n;
+    g.c = B.z; g.ldc = k; g.c_trans = 0;
     g.alpha = 1.0; g.beta = 0.0; g.epi_square = 1; g.batch = 1;
     THC_TRY(gemm_f64(g, s));
   }
@@ -227,8 +223,7 @@ int isdf_run(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int
   if (prec == THC_PREC_FP64) {
     THC_TRY(fft_coulomb(B.x, k, fft, coulG, B.b, 0, B.fft_w, s));
   } else {
-    hipLaunchKernelGGL(to_f32, dim3(grid_for((long)k * n)), dim3(256), 0, s, B.x, B.x32, (long)k * n);
-    THC_TRY(fft_coulomb(B.x32, k, fft, coulG, B.b, 1, B.fft_w, s));
+    THC_TRY(fft_coulomb(B.x, k, fft, coulG, B.b, 1, B.fft_w, s));
   }
   THC_HIP(hipEventRecord(ev[5], s));
   // 5. V = B B^T, FP64 sum of the split-K partials (exact int8 digit products on the mixed path)
@@ -340,6 +335,7 @@ int thc_ctx_destroy(thc_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  for (thc_ctx* l : c->lanes) thc_ctx_destroy(l);
   delete c;
   return THC_OK;
 }
@@ -510,6 +506,71 @@ int thc_factorize(thc_ctx* ctx, const double* grid, const double* rho, const dou
   const int st2 = isdf_run(ctx, aoR, n, m, interp_idx, (int)k, fft_grid, coulG, opts, v, luv, nullptr, istats);
   if (st2 != THC_OK) return st2;
   return st;
+}
+
+int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isdf_opts* opts, int max_streams) {
+  if (!ctx || nsys < 0 || (nsys > 0 && !sys)) return THC_ERR_ARG;
+  if (nsys == 0) return THC_OK;
+  const int nl = std::min(std::min(max_streams > 0 ? max_streams : 4, nsys), 16);
+  THC_HIP(hipSetDevice(ctx->device));
+  // lane 0 is the context itself; the others are created once and kept
+  while ((int)ctx->lanes.size() < nl - 1) {
+    thc_ctx* c = nullptr;
+    THC_TRY(thc_ctx_create(ctx->device, nullptr, &c));
+    c->use_graphs = ctx->use_graphs;
+    ctx->lanes.push_back(c);
+  }
+  // longest first onto the least loaded lane (cost ~ solve + k-means work)
+  std::vector<int> order(nsys);
+  std::vector<double> cost(nsys);
+  for (int i = 0; i < nsys; ++i) {
+    order[i] = i;
+    const double k = (double)sys[i].thc_cfac * sys[i].m;
+    cost[i] = (double)sys[i].n * k * (k + sys[i].m + 4.0 * std::max(sys[i].max_it, 0));
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  std::vector<std::vector<int>> plan(nl);
+  std::vector<double> load(nl, 0.0);
+  for (int i : order) {
+    const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    plan[j].push_back(i);
+    load[j] += cost[i];
+  }
+  std::vector<std::string> errs(nsys);
+  auto work = [&](int lane) {
+    thc_ctx* c = lane == 0 ? ctx : ctx->lanes[lane - 1];
+    const bool dev_ok = hipSetDevice(c->device) == hipSuccess;
+    for (int i : plan[lane]) {
+      thc_system& q = sys[i];
+      if (!dev_ok) {
+        q.status = THC_ERR_HIP;
+        errs[i] = "hipSetDevice failed in a batch worker";
+        continue;
+      }
+      q.status = thc_factorize(c, q.grid, q.rho, q.aoR, q.n, q.m, q.thc_cfac, q.max_it, q.threshold, q.seed,
+                               q.fft_grid, q.coulG, opts, q.interp_idx, q.v, q.luv, &q.kstats, &q.istats);
+      if (q.status != THC_OK && q.status < 100) errs[i] = g_last_error;
+    }
+  };
+  std::vector<std::thread> threads;
+  for (int lane = 1; lane < nl; ++lane) {
+    try {
+      threads.emplace_back(work, lane);
+    } catch (...) {
+      work(lane);  // no thread available: run this lane's share here
+    }
+  }
+  work(0);
+  for (auto& t : threads) t.join();
+  int first_warn = THC_OK;
+  for (int i = 0; i < nsys; ++i) {
+    if (sys[i].status != THC_OK && sys[i].status < 100) {
+      g_last_error = errs[i];
+      return sys[i].status;
+    }
+    if (sys[i].status != THC_OK && first_warn == THC_OK) first_warn = sys[i].status;
+  }
+  return first_warn;
 }
 
 }  // extern "C"

@@ -63,6 +63,7 @@ struct thc_ctx {
   hipEvent_t ev[10] = {};
   void* pinned = nullptr;  // small pinned host buffer for status read-back
   size_t pinned_bytes = 0;
+  std::vector<thc_ctx*> lanes;  // extra stream contexts for batched calls (owned)
 };
 
 namespace thc {
@@ -81,6 +82,7 @@ int ensure_pinned(thc_ctx* ctx, size_t bytes);
 //   a_trans = 0: A[m * lda + k]  (row-major m x k)   1: A[k * lda + m]
 //   b_trans = 0: B[k * ldb + n]  (row-major k x n)   1: B[n * ldb + k]
 //   a_rows: optional row gather for a_trans = 0 (row m of op(A) = row a_rows[m] of A)
+//   b_rows: optional row gather for b_trans = 1 (column n of op(B) = row b_rows[n] of B)
 //   c_trans: write the result transposed (element (m, n) at c[n * ldc + m])
 //   epi_square: C = (alpha * acc)^2 (beta ignored)
 //   lower: only tiles on/below the diagonal are computed (m, n square)
@@ -89,7 +91,8 @@ struct GemmArgs {
   long m, n, k;
   const void* a; long lda; int a_trans; const int* a_rows;
   long a_nrows;  // rows of A addressable through a_rows (gathered rows outside [0, a_nrows) read as 0)
-  const void* b; long ldb; int b_trans;
+  const void* b; long ldb; int b_trans; const int* b_rows;
+  long b_nrows;  // rows of B addressable through b_rows
   void* c; long ldc;
   int c_trans;   // store C^T: element (m, n) at c[n * ldc + m]
   double alpha, beta;
@@ -125,7 +128,8 @@ int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ld
 
 // ---- FFT (fft.hip) ----------------------------------------------------------
 size_t fft_work_bytes(int k, const int n[3], int prec);
-// x: k x n0 n1 n2 (double if prec 0, float if prec 1); out: k x g2 weighted half spectrum
+// x: k x n0 n1 n2 double; arithmetic and out (k x g2 weighted half spectrum) in
+// double (prec 0) or float (prec 1)
 int fft_coulomb(const void* x, int k, const int n[3], const double* coulG, void* out, int prec,
                 void* work, hipStream_t s);
 inline long half_spectrum_len(const int n[3]) { return 2L * n[0] * n[1] * (n[2] / 2 + 1); }

@@ -110,9 +110,10 @@ __device__ C2<T>* fft_lines(C2<T>* a, C2<T>* b, const Plan1D& pl, int nlines, in
   return a;
 }
 
-// x: (k, n0, n1, n2) real; t: (k, n0, n1, n2h) complex.
+// x: (k, n0, n1, n2) real FP64 (rounded to T as it is read); t: (k, n0, n1,
+// n2h) complex.
 template <class T>
-__global__ __launch_bounds__(256) void pass_a(const T* __restrict__ x, C2<T>* __restrict__ t, FftParams p) {
+__global__ __launch_bounds__(256) void pass_a(const double* __restrict__ x, C2<T>* __restrict__ t, FftParams p) {
   extern __shared__ unsigned char smem_raw[];
   const int n1 = p.n1, n2 = p.n2, n2h = p.n2h;
   const int n1p = (n1 + 1) / 2;
@@ -120,13 +121,13 @@ __global__ __launch_bounds__(256) void pass_a(const T* __restrict__ x, C2<T>* __
   C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
   C2<T>* bufb = bufa + cap;
   const long mu = blockIdx.y, ix = blockIdx.x;
-  const T* src = x + (mu * p.n0 + ix) * (long)n1 * n2;
+  const double* src = x + (mu * p.n0 + ix) * (long)n1 * n2;
   // pack rows (2r, 2r+1) as re/im
   for (int e = threadIdx.x; e < n1p * n2; e += blockDim.x) {
     const int r = e / n2, z = e % n2;
     const int y0 = 2 * r, y1 = 2 * r + 1;
-    const T re = src[(long)y0 * n2 + z];
-    const T im = (y1 < n1) ? src[(long)y1 * n2 + z] : T(0);
+    const T re = (T)src[(long)y0 * n2 + z];
+    const T im = (y1 < n1) ? (T)src[(long)y1 * n2 + z] : T(0);
     bufa[e] = C2<T>{re, im};
   }
   __syncthreads();
@@ -236,7 +237,7 @@ int run(const void* x, int k, const int n[3], const double* coulG, void* out, vo
   p.twz = dz; p.twy = dy; p.twx = dx;
   THC_HIP(hipFuncSetAttribute((const void*)pass_a<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la));
   THC_HIP(hipFuncSetAttribute((const void*)pass_b<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
-  hipLaunchKernelGGL(pass_a<T>, dim3(n[0], k), dim3(256), la, s, reinterpret_cast<const T*>(x), tmid, p);
+  hipLaunchKernelGGL(pass_a<T>, dim3(n[0], k), dim3(256), la, s, reinterpret_cast<const double*>(x), tmid, p);
   THC_HIP(hipGetLastError());
   hipLaunchKernelGGL(pass_b<T>, dim3(n[1], k), dim3(256), lb, s, tmid, coulG, reinterpret_cast<T*>(out), p);
   THC_HIP(hipGetLastError());

@@ -1,5 +1,9 @@
 // LDS-tiled MFMA GEMM for gfx950 (FP64: v_mfma_f64_16x16x4f64, FP32:
-// v_mfma_f32_16x16x4f32), 256 threads = 4 wave64 in a 2 x 2 layout.
+// v_mfma_f32_16x16x4f32).  A workgroup is a WM x WN grid of wave64s; the
+// large-matrix configuration is a 128 x 128 x 16 tile on 8 waves (4 x 2, each
+// wave 32 x 64), which keeps a wave under 128 VGPRs so that four waves share
+// every SIMD (a 2 x 2 grid of 64 x 64 wave tiles needs 340 VGPRs and runs one
+// wave per SIMD at ~60% of the rate).
 //
 // Operand fragments (checked on the device by the GPU tests):
 //   A 16x4 block: lane l holds A[l % 16][l / 16]
@@ -14,7 +18,8 @@
 // distinct bank pairs.  Each thread fetches a run of 8 (or 4) contiguous
 // elements per tile.  Interior tiles take an unpredicated path (vector loads
 // when the operands are 16-byte aligned); only edge tiles test bounds.
-// Rows gathered through an index list (Phi_mu) are resolved once per tile.
+// Rows gathered through an index list (Phi_mu, on either operand) are
+// resolved once per tile.
 #include "common.h"
 
 namespace thc {
@@ -44,7 +49,7 @@ enum { EPI_STORE = 0, EPI_SQUARE = 1, EPI_PARTIAL = 2 };
 struct KArgs {
   long m, n, k;
   const void* a; long lda; const int* a_rows; long a_nrows;
-  const void* b; long ldb;
+  const void* b; long ldb; const int* b_rows; long b_nrows;
   void* c; long ldc; int c_trans;
   double alpha, beta;
   int lower;
@@ -83,9 +88,9 @@ __device__ inline bool tile_coords(const KArgs& p, int BM, int BN, int& tm, int&
 // Per-thread share of one operand tile: RUN contiguous elements of one line.
 //   KCONTIG: the operand is contiguous along k (lines are m or n indices)
 //   otherwise contiguous along m / n (lines are k indices)
-template <class T, int BMN, int BK, bool KCONTIG>
+template <class T, int BMN, int BK, bool KCONTIG, int NT = 256>
 struct TileMap {
-  static constexpr int ELEMS = BMN * BK / 256;
+  static constexpr int ELEMS = BMN * BK / NT;
   static constexpr int RUN = ELEMS;  // one run per thread
   __device__ static inline void coords(int tid, int& line, int& off) {
     if (KCONTIG) {  // BMN lines of BK
@@ -130,20 +135,23 @@ __device__ inline void load_run(const T* __restrict__ src, T* dst, bool vec) {
   }
 }
 
-template <class T, int BM, int BN, int BK, int TM, int TN, int EPI, bool AK, bool BK_, bool GATHER, bool VEC>
-__global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
+template <class T, int BM, int BN, int BK, int TM, int TN, int WM, int WN, int EPI, bool AK, bool BK_, bool GA,
+          bool GB, bool VEC>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs p) {
   // AK: A contiguous along k (A[m * lda + k]); else A[k * lda + m]
   // BK_: B contiguous along k (B[n * ldb + k]); else B[k * ldb + n]
-  constexpr int WTM = TM * 16, WTN = TN * 16;
-  static_assert(2 * WTM == BM && 2 * WTN == BN, "2x2 waves");
+  constexpr int WTM = TM * 16, WTN = TN * 16, NT = 64 * WM * WN;
+  static_assert(WM * WTM == BM && WN * WTN == BN, "wave grid must cover the tile");
   constexpr int A_LINES = AK ? BM : BK, A_LEN = AK ? BK + 1 : BM + 1;
   constexpr int B_LINES = BK_ ? BN : BK, B_LEN = BK_ ? BK + 1 : BN + 1;
   __shared__ T As[2][A_LINES * A_LEN];
   __shared__ T Bs[2][B_LINES * B_LEN];
   typedef typename Mfma<T>::acc_t acc_t;
-  typedef TileMap<T, BM, BK, AK> MA;
-  typedef TileMap<T, BN, BK, BK_> MB;
+  typedef TileMap<T, BM, BK, AK, NT> MA;
+  typedef TileMap<T, BN, BK, BK_, NT> MB;
   constexpr int RA = MA::RUN, RB = MB::RUN;
+  static_assert(MA::ELEMS * NT == BM * BK && MB::ELEMS * NT == BN * BK, "tile must split evenly over threads");
+  static_assert(!VEC || (RA * sizeof(T) % 16 == 0 && RB * sizeof(T) % 16 == 0), "vector runs are 16-byte multiples");
 
   int tm, tn;
   if (!tile_coords(p, BM, BN, tm, tn)) return;
@@ -162,7 +170,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WN, wn = wave % WN;
 
   int a_line, a_off, b_line, b_off;
   MA::coords(tid, a_line, a_off);
@@ -175,7 +183,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
     const long gm = m0 + a_line;
     a_row_ok = gm < p.m;
     long row = gm;
-    if (GATHER) {
+    if (GA) {
       row = a_row_ok ? (long)p.a_rows[gm] : 0;
       a_row_ok = a_row_ok && row >= 0 && row < p.a_nrows;
     }
@@ -188,7 +196,12 @@ __global__ __launch_bounds__(256) void gemm_kernel(KArgs p) {
   if (BK_) {
     const long gn = n0 + b_line;
     b_row_ok = gn < p.n;
-    b_ptr = B + (b_row_ok ? gn : 0) * p.ldb + b_off;
+    long row = gn;
+    if (GB) {
+      row = b_row_ok ? (long)p.b_rows[gn] : 0;
+      b_row_ok = b_row_ok && row >= 0 && row < p.b_nrows;
+    }
+    b_ptr = B + (b_row_ok ? row : 0) * p.ldb + b_off;
   } else {
     b_ptr = B + (long)b_line * p.ldb + n0 + b_off;
   }
@@ -350,7 +363,7 @@ KArgs make_args(const GemmArgs& g) {
   KArgs p = {};
   p.m = g.m; p.n = g.n; p.k = g.k;
   p.a = g.a; p.lda = g.lda; p.a_rows = g.a_rows; p.a_nrows = g.a_nrows;
-  p.b = g.b; p.ldb = g.ldb;
+  p.b = g.b; p.ldb = g.ldb; p.b_rows = g.b_rows; p.b_nrows = g.b_nrows;
   p.c = g.c; p.ldc = g.ldc; p.c_trans = g.c_trans;
   p.alpha = g.alpha; p.beta = g.beta;
   p.lower = g.lower;
@@ -358,47 +371,62 @@ KArgs make_args(const GemmArgs& g) {
   return p;
 }
 
-template <class T, int BM, int BN, int BK, int TM, int TN, int EPI, bool AK, bool BKc, bool GA>
+// Tile configuration: BM x BN x BK on a WM x WN wave grid, TM x TN MFMA
+// blocks per wave.
+template <int BM_, int BN_, int BK_, int TM_, int TN_, int WM_, int WN_>
+struct Cfg {
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, TM = TM_, TN = TN_, WM = WM_, WN = WN_;
+  static constexpr int THREADS = 64 * WM * WN;
+};
+typedef Cfg<128, 128, 16, 2, 4, 4, 2> CfgLarge;  // 8 waves of 32 x 64
+typedef Cfg<64, 64, 16, 2, 2, 2, 2> CfgSmall;    // 4 waves of 32 x 32
+
+template <class T, class C, int EPI, bool AK, bool BKc, bool GA, bool GB>
 void launch_vec(const KArgs& p, bool vec, dim3 grid, hipStream_t s) {
   if (vec)
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, TM, TN, EPI, AK, BKc, GA, true>), grid, dim3(256), 0, s, p);
+    hipLaunchKernelGGL((gemm_kernel<T, C::BM, C::BN, C::BK, C::TM, C::TN, C::WM, C::WN, EPI, AK, BKc, GA, GB, true>),
+                       grid, dim3(C::THREADS), 0, s, p);
   else
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, TM, TN, EPI, AK, BKc, GA, false>), grid, dim3(256), 0, s, p);
+    hipLaunchKernelGGL((gemm_kernel<T, C::BM, C::BN, C::BK, C::TM, C::TN, C::WM, C::WN, EPI, AK, BKc, GA, GB, false>),
+                       grid, dim3(C::THREADS), 0, s, p);
 }
 
-// Every operand layout the path uses: (A k-contiguous?, B k-contiguous?, gather?)
-template <class T, int BM, int BN, int BK, int TM, int TN, int EPI>
+// Every operand layout the path uses: (A k-contiguous?, B k-contiguous?,
+// gather on A?, gather on B?); gathers need the gathered operand k-contiguous
+// and only one operand gathered at a time.
+template <class T, class C, int EPI>
 int launch_layout(const GemmArgs& g, const KArgs& p, dim3 grid, hipStream_t s) {
   const int vw = 16 / (int)sizeof(T);  // elements per 16 bytes
   const bool vec = aligned16(g.a) && aligned16(g.b) && g.lda % vw == 0 && g.ldb % vw == 0 &&
                    g.stride_a % vw == 0 && g.stride_b % vw == 0;
-  const bool ak = g.a_trans == 0, bk = g.b_trans == 1, ga = g.a_rows != nullptr;
-  if (ga && !ak) return THC_ERR_ARG;
+  const bool ak = g.a_trans == 0, bk = g.b_trans == 1, ga = g.a_rows != nullptr, gb = g.b_rows != nullptr;
+  if ((ga && !ak) || (gb && !bk) || (ga && gb)) return THC_ERR_ARG;
   if (ak && bk) {
-    if (ga) launch_vec<T, BM, BN, BK, TM, TN, EPI, true, true, true>(p, vec, grid, s);
-    else launch_vec<T, BM, BN, BK, TM, TN, EPI, true, true, false>(p, vec, grid, s);
+    if (ga) launch_vec<T, C, EPI, true, true, true, false>(p, vec, grid, s);
+    else if (gb) launch_vec<T, C, EPI, true, true, false, true>(p, vec, grid, s);
+    else launch_vec<T, C, EPI, true, true, false, false>(p, vec, grid, s);
   } else if (ak && !bk) {
-    launch_vec<T, BM, BN, BK, TM, TN, EPI, true, false, false>(p, vec, grid, s);
+    launch_vec<T, C, EPI, true, false, false, false>(p, vec, grid, s);
   } else if (!ak && !bk) {
-    launch_vec<T, BM, BN, BK, TM, TN, EPI, false, false, false>(p, vec, grid, s);
+    launch_vec<T, C, EPI, false, false, false, false>(p, vec, grid, s);
   } else {
-    launch_vec<T, BM, BN, BK, TM, TN, EPI, false, true, false>(p, vec, grid, s);
+    launch_vec<T, C, EPI, false, true, false, false>(p, vec, grid, s);
   }
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
 
-template <class T, int BM, int BN, int BK, int TM, int TN>
+template <class T, class C>
 int launch(const GemmArgs& g, hipStream_t s) {
   const KArgs p = make_args(g);
-  const long nbm = (g.m + BM - 1) / BM, nbn = (g.n + BN - 1) / BN;
+  const long nbm = (g.m + C::BM - 1) / C::BM, nbn = (g.n + C::BN - 1) / C::BN;
   const long nblk = g.lower ? nbm * (nbm + 1) / 2 : nbm * nbn;
   if (nblk <= 0 || g.batch <= 0) return THC_OK;
   if (nblk > 0x7fffffffL) return THC_ERR_ARG;
-  if (g.a_rows && g.lower) return THC_ERR_ARG;
+  if ((g.a_rows || g.b_rows) && g.lower) return THC_ERR_ARG;
   dim3 grid((unsigned)nblk, 1, (unsigned)g.batch);
-  if (g.epi_square) return launch_layout<T, BM, BN, BK, TM, TN, EPI_SQUARE>(g, p, grid, s);
-  return launch_layout<T, BM, BN, BK, TM, TN, EPI_STORE>(g, p, grid, s);
+  if (g.epi_square) return launch_layout<T, C, EPI_SQUARE>(g, p, grid, s);
+  return launch_layout<T, C, EPI_STORE>(g, p, grid, s);
 }
 
 template <class T>
@@ -406,11 +434,11 @@ int gemm_dispatch(const GemmArgs& g, hipStream_t s) {
   if (g.m <= 0 || g.n <= 0) return THC_OK;
   if (g.lower && g.m != g.n) return THC_ERR_ARG;
   const long tiles128 = ((g.m + 127) / 128) * ((g.n + 127) / 128) * g.batch;
-  if (tiles128 < 512) return launch<T, 64, 64, 16, 2, 2>(g, s);
-  return launch<T, 128, 128, 16, 4, 4>(g, s);
+  if (tiles128 < 512) return launch<T, CfgSmall>(g, s);
+  return launch<T, CfgLarge>(g, s);
 }
 
-constexpr int SY_BM = 128, SY_BN = 128, SY_BK = 16;
+constexpr int SY_BM = CfgLarge::BM, SY_BN = CfgLarge::BN, SY_BK = CfgLarge::BK;
 
 }  // namespace
 
@@ -454,12 +482,12 @@ int syrk_splitk(const void* a, long m, long k, long lda, int prec, double* v, lo
   const int vw = prec == 0 ? 2 : 4;
   const bool vec = aligned16(a) && lda % vw == 0;
   if (prec == 0) {
-    launch_vec<double, SY_BM, SY_BN, SY_BK, 4, 4, EPI_PARTIAL, true, true, false>(p, vec, grid, s);
+    launch_vec<double, CfgLarge, EPI_PARTIAL, true, true, false, false>(p, vec, grid, s);
     THC_HIP(hipGetLastError());
     hipLaunchKernelGGL((syrk_reduce_kernel<double, SY_BM, SY_BN>), dim3((unsigned)ntiles), dim3(256), 0, s,
                        (const double*)work, nsplit, ntiles, m, v, ldv);
   } else {
-    launch_vec<float, SY_BM, SY_BN, SY_BK, 4, 4, EPI_PARTIAL, true, true, false>(p, vec, grid, s);
+    launch_vec<float, CfgLarge, EPI_PARTIAL, true, true, false, false>(p, vec, grid, s);
     THC_HIP(hipGetLastError());
     hipLaunchKernelGGL((syrk_reduce_kernel<float, SY_BM, SY_BN>), dim3((unsigned)ntiles), dim3(256), 0, s,
                        (const float*)work, nsplit, ntiles, m, v, ldv);

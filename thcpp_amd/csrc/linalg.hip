@@ -3,10 +3,11 @@
 // src/matrix_operations.h:259-267) and the SPD inverse used to apply
 // CC^T^-1 to ZC^T as one MFMA GEMM.
 //
-// Cholesky: right-looking, 64-column panels.  Per panel: one workgroup
-// factors the 64 x 64 diagonal block in LDS and inverts it; the panel below is
-// multiplied by that inverse (GEMM); the trailing lower triangle takes the
-// rank-64 update (GEMM, lower tiles only).
+// Cholesky: right-looking, 64-column panels.  Per panel: one wave factors the
+// 64 x 64 diagonal block (row per lane, in registers); the panel below is
+// solved against it row by row (L21 = A21 L11^-T, one row per thread); the
+// trailing lower triangle takes the rank-64 update (MFMA GEMM, lower tiles
+// only).
 // Inverse: the lower factor is padded with an identity to 64 * 2^p, its
 // 64 x 64 diagonal blocks are inverted in parallel, then recursive doubling
 // forms Linv21 = -Linv22 (L21 Linv11) for every pair of blocks of one size in
@@ -20,83 +21,149 @@ namespace {
 
 constexpr int NB = 64;
 
-// In-LDS factor and inverse of one NB x NB lower block, one wave.
-// Factor (Cholesky-Crout): lane r owns row r and computes
-// L[r][k] = (A[r][k] - sum_{q<k} L[r][q] L[k][q]) / L[k][k]; row k is read as
-// broadcasts, the lane's own row with the (NB + 1) stride that keeps the 64
-// lanes on distinct banks.  Inverse: lane c owns column c of L^-1,
-// x[i] = (delta_ic - sum_{q<i} L[i][q] x[q]) / L[i][i].  Dot products use
-// four partial sums so loads pipeline and the dependent chain is short.
-// Rows and columns >= jb are an identity extension.  Returns the 1-based first
-// failing pivot (0 if none).
 __device__ inline void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-__device__ inline double dot_lds(const double* a, int astride, const double* b, int bstride, int n) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int q = 0;
-  for (; q + 4 <= n; q += 4) {
-    s0 += a[q * astride] * b[q * bstride];
-    s1 += a[(q + 1) * astride] * b[(q + 1) * bstride];
-    s2 += a[(q + 2) * astride] * b[(q + 2) * bstride];
-    s3 += a[(q + 3) * astride] * b[(q + 3) * bstride];
+// Lower NB x NB block at (i0, j0) of a, jb x jb valid, as rows of an LDS
+// array; outside jb it is the identity.  Load r reads one matrix row (the 64
+// lanes read 64 consecutive doubles).  All 64 loads are issued before the
+// first LDS store (masked lanes read a[0], which exists), so the block costs
+// one memory round trip.  TRANS stores the block transposed.
+template <bool TRANS>
+__device__ inline void load_lower_block(const double* a, long lda, int i0, int j0, int jb, double (*Ls)[NB + 1]) {
+  const int t = threadIdx.x;
+  double v[NB];
+#pragma unroll
+  for (int rr = 0; rr < NB; ++rr) {
+    const bool ok = rr < jb && t <= rr;
+    v[rr] = a[ok ? (long)(i0 + rr) * lda + j0 + t : 0];
   }
-  for (; q < n; ++q) s0 += a[q * astride] * b[q * bstride];
-  return (s0 + s1) + (s2 + s3);
+#pragma unroll
+  for (int rr = 0; rr < NB; ++rr) {
+    const bool ok = rr < jb && t <= rr;
+    const double x = ok ? v[rr] : ((rr >= jb && rr == t) ? 1.0 : 0.0);
+    if (TRANS) Ls[t][rr] = x;
+    else Ls[rr][t] = x;
+  }
 }
 
-__device__ int block_chol_inv(double (*Ls)[NB + 1], double (*Ds)[NB + 1], int jb) {
+// Rows r0 .. r0 + NB - 1 (below 'rows': zero) of the rows x jb panel p (row
+// stride lda) into Xs, columns >= jb zero; batched loads as above.
+__device__ inline void load_panel(const double* p, long lda, long r0, long rows, int jb, double (*Xs)[NB + 1]) {
+  const int t = threadIdx.x;
+  double v[NB];
+#pragma unroll
+  for (int rr = 0; rr < NB; ++rr) {
+    const bool ok = r0 + rr < rows && t < jb;
+    v[rr] = p[ok ? (r0 + rr) * lda + t : 0];
+  }
+#pragma unroll
+  for (int rr = 0; rr < NB; ++rr) Xs[rr][t] = (r0 + rr < rows && t < jb) ? v[rr] : 0.0;
+}
+
+// Forward substitution for one row held in registers: x L^T = b, with L given
+// transposed (Lt[c][c2] = L[c2][c], a broadcast read for the whole wave).
+// Right-looking: once x[c] is final every later entry drops its x[c] term, so
+// the updates of one step are independent of each other.
+__device__ inline void solve_row(double (&x)[NB], const double (*Lt)[NB + 1]) {
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    x[c] = x[c] / Lt[c][c];
+#pragma unroll
+    for (int c2 = c + 1; c2 < NB; ++c2) x[c2] = fma(-x[c], Lt[c][c2], x[c2]);
+  }
+}
+
+__device__ inline double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Factor the jb x jb diagonal block at (j0, j0) in place (lower part only).
+// Right-looking, lane r owning row r in registers: at step k the pivot
+// L[k][k] = sqrt(A[k][k]) is read from lane k, column k is scaled by its
+// reciprocal, and every lane drops L[r][k] L[c][k] from its entries c > k,
+// with L[c][k] read from lane c.  Rows/columns >= jb are an identity
+// extension.  Also writes Lt (NB x NB, Lt[c][r] = L[r][c], zero above the
+// diagonal of L) and rinv[c] = 1 / L[c][c] for the panel solve.  info: first
+// failing leading minor (1-based, global), kept if already set.
+__global__ __launch_bounds__(64) void potrf_diag(double* a, long lda, int j0, int jb, int* info, double* lt,
+                                                 double* rinv) {
+  __shared__ double Ls[NB][NB + 1];
   const int r = threadIdx.x;
+  load_lower_block<false>(a, lda, j0, j0, jb, Ls);
+  wave_sync_lds();
+  double row[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) row[c] = Ls[r][c];
   int bad = 0;
+  double my_rp = 1.0;
+#pragma unroll
   for (int k = 0; k < NB; ++k) {
-    const double s = Ls[r][k] - dot_lds(&Ls[r][0], 1, &Ls[k][0], 1, k);
-    const double d = __shfl(s, k);
+    const double d = readlane_d(row[k], k);
     if (!(d > 0.0) && bad == 0 && k < jb) bad = k + 1;
     const double piv = sqrt(d);
-    if (r >= k) Ls[r][k] = (r == k) ? piv : s / piv;
-    wave_sync_lds();
+    const double rp = 1.0 / piv;
+    if (r == k) my_rp = rp;
+    row[k] = (r == k) ? piv : row[k] * rp;
+#pragma unroll
+    for (int c = k + 1; c < NB; ++c) row[c] = fma(-row[k], readlane_d(row[k], c), row[c]);
   }
-  // column r of L^-1, kept in Ds[.][r]
-  for (int i = 0; i < NB; ++i) {
-    double s = (i == r) ? 1.0 : 0.0;
-    if (i > r) s = -dot_lds(&Ls[i][r], 1, &Ds[r][r], NB + 1, i - r);
-    Ds[i][r] = (i < r) ? 0.0 : s / Ls[i][i];
+  if (r == 0 && bad && *info == 0) *info = j0 + bad;
+  rinv[r] = my_rp;
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    lt[c * NB + r] = (c <= r) ? row[c] : 0.0;
+    Ls[r][c] = row[c];
   }
   wave_sync_lds();
-  return bad;
+  for (int rr = 0; rr < jb; ++rr)
+    if (r <= rr) a[(long)(j0 + rr) * lda + j0 + r] = Ls[rr][r];
 }
 
-// Factor the jb x jb diagonal block at (j0, j0) in place and write its lower
-// inverse to dinv (NB x NB, row-major, zero upper).  info: first failing
-// leading minor (1-based, global), kept if already set.
-__global__ __launch_bounds__(64) void potrf_diag(double* a, long lda, int j0, int jb, int* info, double* dinv) {
-  __shared__ double L[NB][NB + 1];
-  __shared__ double D[NB][NB + 1];
-  const int r = threadIdx.x;
-  for (int c = 0; c < NB; ++c) {
-    double v = 0.0;
-    if (r < jb && c < jb) v = c <= r ? a[(long)(j0 + r) * lda + j0 + c] : 0.0;
-    else if (r == c) v = 1.0;
-    L[r][c] = v;
+// L21 = A21 L11^-T in place for the 'rows' rows below the diagonal block at
+// (j0, j0): one row per thread (x L11^T = a, forward substitution,
+// right-looking, each x[c] scaled by the stored reciprocal of L[c][c]), 64 rows
+// per workgroup.  L11^T and the reciprocals come from potrf_diag's output.
+__global__ __launch_bounds__(64) void trsm_panel(double* a, long lda, int j0, int jb, long rows,
+                                                 const double* __restrict__ lt, const double* __restrict__ rinv) {
+  __shared__ double Lt[NB][NB + 1];
+  __shared__ double Xs[NB][NB + 1];
+  __shared__ double Ri[NB];
+  const int t = threadIdx.x;
+  const long r0 = (long)blockIdx.x * NB;
+  double* p = a + (long)(j0 + jb) * lda + j0;
+  {
+    double v[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) v[c] = lt[c * NB + t];
+    const double ri = rinv[t];
+    load_panel(p, lda, r0, rows, jb, Xs);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) Lt[c][t] = v[c];
+    Ri[t] = ri;
   }
-  __syncthreads();
-  const int bad = block_chol_inv(L, D, jb);
-  if (r == 0 && bad && *info == 0) *info = j0 + bad;
-  if (r < jb)
-    for (int c = 0; c <= r; ++c) a[(long)(j0 + r) * lda + j0 + c] = L[r][c];
-  for (int c = 0; c < NB; ++c) dinv[r * NB + c] = (r < jb && c < jb) ? D[r][c] : 0.0;
-}
-
-__global__ __launch_bounds__(256) void copy_block(const double* src, long lds, double* dst, long ldd, long rows,
-                                                  int cols) {
-  const long total = rows * cols;
-  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += 256L * gridDim.x) {
-    const long r = e / cols;
-    const int c = (int)(e % cols);
-    dst[r * ldd + c] = src[r * lds + c];
+  wave_sync_lds();
+  double x[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) x[c] = Xs[t][c];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    x[c] *= Ri[c];
+#pragma unroll
+    for (int c2 = c + 1; c2 < NB; ++c2) x[c2] = fma(-x[c], Lt[c][c2], x[c2]);
+  }
+#pragma unroll
+  for (int c = 0; c < NB; ++c) Xs[t][c] = x[c];
+  wave_sync_lds();
+  for (int rr = 0; rr < NB; ++rr) {
+    const long pr = r0 + rr;
+    if (pr < rows && t < jb) p[pr * lda + t] = Xs[rr][t];
   }
 }
 
@@ -128,20 +195,19 @@ __global__ __launch_bounds__(256) void pad_lower(const double* l, int k, long ld
   }
 }
 
-// Invert each 64 x 64 lower diagonal block of lp into linv (same positions).
+// Invert each 64 x 64 lower diagonal block of lp into linv (same positions):
+// thread t solves x L^T = e_t, i.e. x = column t of L^-1.
 __global__ __launch_bounds__(64) void invert_diag_blocks(const double* lp, int kp, double* linv) {
-  __shared__ double L[NB][NB + 1];
-  __shared__ double D[NB][NB + 1];
-  const int j0 = blockIdx.x * NB, r = threadIdx.x;
-  for (int c = 0; c < NB; ++c) L[r][c] = lp[(long)(j0 + r) * kp + j0 + c];
-  __syncthreads();
-  for (int i = 0; i < NB; ++i) {
-    double s = (i == r) ? 1.0 : 0.0;
-    if (i > r) s = -dot_lds(&L[i][r], 1, &D[r][r], NB + 1, i - r);
-    D[i][r] = (i < r) ? 0.0 : s / L[i][i];
-  }
-  __syncthreads();
-  for (int c = 0; c < NB; ++c) linv[(long)(j0 + r) * kp + j0 + c] = D[r][c];
+  __shared__ double Lt[NB][NB + 1];
+  const int j0 = blockIdx.x * NB, t = threadIdx.x;
+  load_lower_block<true>(lp, kp, j0, j0, NB, Lt);
+  wave_sync_lds();
+  double x[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) x[c] = (c == t) ? 1.0 : 0.0;
+  solve_row(x, Lt);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) linv[(long)(j0 + i) * kp + j0 + t] = x[i];
 }
 
 __global__ __launch_bounds__(256) void fill_zero(double* a, long count) {
@@ -152,34 +218,24 @@ inline int grid_for(long count) { return (int)std::min<long>(4096, std::max<long
 
 }  // namespace
 
-size_t chol_work_bytes(int k) {
-  return align_up(sizeof(double) * NB * NB, 256) + align_up(sizeof(double) * (size_t)k * NB, 256) + 512;
-}
+size_t chol_work_bytes(int) { return align_up(sizeof(double) * NB * NB, 256) + align_up(sizeof(double) * NB, 256); }
 
 int cholesky_f64(double* a, int k, long lda, int* d_info, void* work, hipStream_t s) {
-  double* dinv = reinterpret_cast<double*>(work);
-  double* panel = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + align_up(sizeof(double) * NB * NB, 256));
+  double* lt = reinterpret_cast<double*>(work);
+  double* rinv = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + align_up(sizeof(double) * NB * NB, 256));
   for (int j0 = 0; j0 < k; j0 += NB) {
     const int jb = std::min(NB, k - j0);
-    hipLaunchKernelGGL(potrf_diag, dim3(1), dim3(64), 0, s, a, lda, j0, jb, d_info, dinv);
+    hipLaunchKernelGGL(potrf_diag, dim3(1), dim3(64), 0, s, a, lda, j0, jb, d_info, lt, rinv);
     THC_HIP(hipGetLastError());
     const long rows = k - j0 - jb;
     if (rows <= 0) continue;
-    // panel = A[j0+jb:, j0:j0+jb] * Dinv^T
-    GemmArgs g = {};
-    g.m = rows; g.n = jb; g.k = jb;
-    g.a = a + (long)(j0 + jb) * lda + j0; g.lda = lda; g.a_trans = 0;
-    g.b = dinv; g.ldb = NB; g.b_trans = 1;
-    g.c = panel; g.ldc = NB;
-    g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
-    THC_TRY(gemm_f64(g, s));
-    hipLaunchKernelGGL(copy_block, dim3(grid_for(rows * jb)), dim3(256), 0, s, panel, (long)NB,
-                       a + (long)(j0 + jb) * lda + j0, lda, rows, jb);
-    // trailing lower update A22 -= panel panel^T
+    hipLaunchKernelGGL(trsm_panel, dim3(ceil_div(rows, NB)), dim3(64), 0, s, a, lda, j0, jb, rows, lt, rinv);
+    THC_HIP(hipGetLastError());
+    // trailing lower update A22 -= L21 L21^T
     GemmArgs u = {};
     u.m = rows; u.n = rows; u.k = jb;
-    u.a = panel; u.lda = NB; u.a_trans = 0;
-    u.b = panel; u.ldb = NB; u.b_trans = 1;
+    u.a = a + (long)(j0 + jb) * lda + j0; u.lda = lda; u.a_trans = 0;
+    u.b = u.a; u.ldb = lda; u.b_trans = 1;
     u.c = a + (long)(j0 + jb) * lda + (j0 + jb); u.ldc = lda;
     u.alpha = -1.0; u.beta = 1.0; u.lower = 1; u.batch = 1;
     THC_TRY(gemm_f64(u, s));

@@ -76,17 +76,26 @@ __global__ __launch_bounds__(256) void oz_split(const T* __restrict__ b, long ld
 
 // One lower 128 x 128 tile (tm >= tn) and one chunk per workgroup; 8 waves
 // in a 2 (rows) x 4 (columns) grid, each wave 64 x 32 = 4 x 2 MFMA blocks.
+// Work items w = chunk * ntiles + tile are dealt to the 8 XCDs in contiguous
+// runs (workgroup i runs on XCD i % 8): the workgroups resident on one XCD then
+// hold tiles of the same chunk and walk its columns together, so the digit
+// rows they share are fetched into that XCD's L2 once rather than once per
+// tile.
 __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S, long g2p, int k,
                                                const int* __restrict__ E, int nchunk, double* __restrict__ work,
                                                long ntiles) {
   __shared__ __attribute__((aligned(16))) signed char As[OZ_S][OZ_T][OZ_LD];
   __shared__ __attribute__((aligned(16))) signed char Bs[OZ_S][OZ_T][OZ_LD];
-  const long tile_lin = blockIdx.x;
+  const long total = ntiles * nchunk;
+  const long per_xcd = (total + 7) / 8;
+  const long w = (long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (w >= total) return;
+  const int c = (int)(w / ntiles);
+  const long tile_lin = w % ntiles;
   int tm = (int)((sqrt(8.0 * tile_lin + 1.0) - 1.0) * 0.5);
   while ((long)(tm + 1) * (tm + 2) / 2 <= tile_lin) ++tm;
   while ((long)tm * (tm + 1) / 2 > tile_lin) --tm;
   const int tn = (int)(tile_lin - (long)tm * (tm + 1) / 2);
-  const int c = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 waves
   const int l16 = lane & 15, l4 = lane >> 4;
@@ -209,7 +218,9 @@ int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ld
     hipLaunchKernelGGL(oz_split<float>, dim3(nchunk, k), dim3(256), 0, s, (const float*)b, ld, g2, g2p, k, nchunk,
                        S, E);
   THC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)ntiles, nchunk), dim3(512), 0, s, S, g2p, k, E, nchunk, part, ntiles);
+  const long slots = (ntiles * nchunk + 7) / 8 * 8;  // every w < ntiles * nchunk gets one workgroup
+  if (slots > 0x7fffffffL) return THC_ERR_ARG;
+  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, S, g2p, k, E, nchunk, part, ntiles);
   THC_HIP(hipGetLastError());
   hipLaunchKernelGGL(oz_reduce, dim3((unsigned)ntiles), dim3(256), 0, s, part, nchunk, ntiles, k, v, ldv);
   THC_HIP(hipGetLastError());

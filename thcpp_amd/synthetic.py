@@ -151,17 +151,20 @@ _SMOOTH = (24, 25, 27, 30, 32, 36, 40, 45, 48)
 
 
 def make_batch(count: int = 32, seed: int = 0, max_it: int = 20):
-    """Uneven batch of small cells: side L ~ U[6, 14] bohr, 8-64 carbon atoms on
-    jittered diamond sites, grid edge the 2,3,5-smooth size in [24, 48] nearest
-    to 3.5 points per bohr, M = 2 per atom, N_mu = cfac M with cfac ~ {8, 9, 10}
-    capped so that N_mu <= M (M + 1) / 2 (above that CC^T is singular for real
+    """Uneven batch of small cells: side L ~ U[6, 14] bohr holding 8-64 carbon
+    atoms on jittered diamond sites (the count follows the cell volume at
+    diamond density, 8 atoms per (6.74 bohr)^3, rounded to a multiple of 8 and
+    clamped to [8, 64], so every cell has ~90 grid points per interpolation
+    point), grid edge the 2,3,5-smooth size in [24, 48] nearest to 3.5 points
+    per bohr, M = 2 per atom, N_mu = cfac M with cfac ~ {8, 9, 10} capped so
+    that N_mu <= M (M + 1) / 2 (above that CC^T is singular for real
     orbitals)."""
     rng = np.random.default_rng(seed)
     systems = []
     base = diamond_sites(2, 1.0) / 2.0  # 64 fractional sites in the unit cube
     for s in range(count):
         L = float(rng.uniform(6.0, 14.0))
-        nat = int(rng.integers(1, 9)) * 8
+        nat = int(min(8, max(1, round((L / DIAMOND_A) ** 3)))) * 8
         frac = base[rng.permutation(64)[:nat]]
         atoms = (frac + rng.normal(scale=0.02, size=frac.shape)) * L
         target = 3.5 * L
