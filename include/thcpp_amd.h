@@ -183,6 +183,27 @@ int thc_factorize(thc_ctx *ctx, const double *grid, const double *rho, const dou
                   int *interp_idx, double *v, double *luv,
                   thc_kmeans_stats *kstats, thc_isdf_stats *istats);
 
+/* ---------------------------------------------------------------- half-rotated THC */
+
+/* ZC^T of the half-rotated factorisation, k x n row-major:
+ * (Phi_occ,mu Phi_occ^T) o (Phi_mu Phi^T) elementwise.  Replaces
+ * IVecs::setup_CZt_half (src/interpolating_vectors.cpp:143-161), which forms
+ * Pocc o conj(P); the orbitals here are real.  aoR_half: n x m_occ row-major
+ * (the orbital file's aoR_half). */
+int thc_czt_half(thc_ctx *ctx, const double *aoR, const double *aoR_half, long n, int m, int m_occ,
+                 const int *idx, int k, double *czt);
+
+/* IVecs with rotate = true at one rank (src/interpolating_vectors.cpp:17-57
+ * with setup_CZt_half, then kernel 406-455): the half-rotated ZC^T and CC^T,
+ * the least-squares solve, the FFT and
+ * Muv = sum_G IVMG(G)^T coulG(G) IVG(G) (construct_muv, :271-273), which for
+ * real interpolating vectors (IVMG = conj IVG, :384-391) is V of thc_isdf;
+ * written full (k x k), with no Cholesky (HalfTransformedMuv, :317-328).
+ * opts->want_luv is ignored. */
+int thc_isdf_half(thc_ctx *ctx, const double *aoR, const double *aoR_half, long n, int m, int m_occ,
+                  const int *idx, int k, const int fft_grid[3], const double *coulG,
+                  const thc_isdf_opts *opts, double *muv, double *xi, thc_isdf_stats *stats);
+
 /* ---------------------------------------------------------------- batches */
 
 /* One system of a batch: inputs as for thc_factorize (device pointers), with

@@ -16,10 +16,16 @@ are real here, so every imaginary part is zero, but the steps are the same):
   fft_vectors        src/interpolating_vectors.cpp:331-404 X^H, unnormalised forward 3-D DFT per mu
   construct_muv      src/interpolating_vectors.cpp:247-281 A = IVG sqrt(coulG); Muv = A^H A (lower)
   cholesky_luv       src/interpolating_vectors.cpp:283-315 pzpotrf('L'), upper triangle zeroed
+  czt_half           src/interpolating_vectors.cpp:143-161 ZC^T = Pocc * conj(P) (half-rotated)
+  fft_vectors_backward src/interpolating_vectors.cpp:384-391 unnormalised FFTW_BACKWARD of X^H
+  construct_muv_half src/interpolating_vectors.cpp:258-273 Muv = IVMG^T IVG, both times sqrt(coulG)
+  isdf_half          src/main.cpp:63-72 + IVecs(rotate=true): HalfTransformedMuv, no Cholesky
 
 Parity pins (tests/test_oracle_isdf.py): numpy/scipy closed forms on small
-inputs, brute-force DFT, plane-wave Coulomb closed form, the G=0 shift
-invariance, and the reference-held etaPm fixture for the Hadamard-squared Gram.
+inputs, brute-force DFT (both signs), plane-wave Coulomb closed form, the G=0
+shift invariance, the reference-held etaPm fixture for the Hadamard-squared Gram
+and, for the half-rotated Gram, the Khatri-Rao identity
+(A A^T) o (B B^T) = (A (.) B)(A (.) B)^T on the same fixture.
 """
 from __future__ import annotations
 
@@ -118,4 +124,49 @@ def isdf(aoR, idx, fft_grid, coulg, return_all=False):
     out = {"V": v, "Luv": luv, "chol_info": info, "imag_max": float(np.abs(muv.imag).max())}
     if return_all:
         out.update({"Z": z, "C": c, "X": x, "IVG": ivg})
+    return out
+
+
+def czt_half(aoR: np.ndarray, aoR_half: np.ndarray, idx: np.ndarray) -> np.ndarray:
+    """Half-rotated ZC^T = Pocc * conj(P), elementwise (setup_CZt_half,
+    interpolating_vectors.cpp:143-161): P from the full orbital set (aoR),
+    Pocc from the occupied set (aoR_half, (N, M_occ))."""
+    p = pseudo_dm(aoR, idx)
+    pocc = pseudo_dm(aoR_half, idx)
+    z = pocc * p.conj()
+    return z.real if (np.isrealobj(aoR) and np.isrealobj(aoR_half)) else z
+
+
+def fft_vectors_backward(x: np.ndarray, fft_grid) -> np.ndarray:
+    """IVMG[G, mu] = unnormalised FFTW_BACKWARD (e^{+2 pi i k.r/n}) of conj(X[mu, :])
+    (interpolating_vectors.cpp:384-391, the half-rotated pass).  (N, K) complex."""
+    n0, n1, n2 = (int(v) for v in fft_grid)
+    k, n = x.shape
+    assert n0 * n1 * n2 == n
+    cols = np.conj(x).reshape(k, n0, n1, n2)
+    return (np.fft.ifftn(cols, axes=(1, 2, 3)) * n).reshape(k, n).T
+
+
+def construct_muv_half(ivg: np.ndarray, ivmg: np.ndarray, coulg: np.ndarray) -> np.ndarray:
+    """Muv = IVMG^T IVG, both scaled by sqrt(coulG) (interpolating_vectors.cpp:258-273,
+    product 'T','N'); the full K x K matrix."""
+    w = np.sqrt(np.asarray(coulg, dtype=np.float64).reshape(-1))[:, None]
+    return (ivmg * w).T @ (ivg * w)
+
+
+def isdf_half(aoR, aoR_half, idx, fft_grid, coulg, return_all=False):
+    """IVecs with rotate = true (interpolating_vectors.cpp:17-57, 143-161, 406-455):
+    the half-rotated ZC^T and CC^T, QR least squares, forward and backward FFTs,
+    Muv = IVMG^T IVG (HalfTransformedMuv, no Cholesky, :317-328)."""
+    idx = np.asarray(idx)
+    z = czt_half(aoR, aoR_half, idx)
+    c = cct(z, idx)
+    x = least_squares(c, z)
+    ivg = fft_vectors(x, fft_grid)
+    ivmg = fft_vectors_backward(x, fft_grid)
+    muv = construct_muv_half(ivg, ivmg, coulg)
+    real = np.isrealobj(aoR) and np.isrealobj(aoR_half)
+    out = {"Muv": muv.real if real else muv, "imag_max": float(np.abs(muv.imag).max())}
+    if return_all:
+        out.update({"Z": z, "C": c, "X": x, "IVG": ivg, "IVMG": ivmg})
     return out

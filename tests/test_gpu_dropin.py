@@ -22,7 +22,7 @@ vp = ctypes.c_void_p
 def host():
     h = ctypes.CDLL(os.path.join(PKG, "libthcpp_amd_host.so"))
     h.thc_host_write_orbital_file.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, vp, vp, vp, vp, vp, vp,
-                                              vp, ctypes.c_double]
+                                              vp, ctypes.c_double, vp, ctypes.c_int]
     h.thc_host_read_f64.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.c_long]
     h.thc_host_read_f64.restype = ctypes.c_long
     h.thc_host_read_int.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.c_long]
@@ -48,7 +48,7 @@ def test_dropin_driver(tmp_path):
     hc = np.arange(s.M * s.M, dtype=np.float64).reshape(s.M, s.M)
     ne = np.array([4, 4], dtype=np.int32)
     assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(s.aoR), P(fg), P(s.coulG),
-                                         P(hc), P(ne), -2.5) == 0
+                                         P(hc), P(ne), -2.5, None, 0) == 0
     inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac,
            "interpolating_points": {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}}}
     (tmp_path / "input.json").write_text(json.dumps(inp))
@@ -73,6 +73,55 @@ def test_dropin_driver(tmp_path):
     np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/occups", np.int32), np.zeros(8))
     np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/hcore").reshape(s.M, s.M, 2)[:, :, 0], hc)
     assert "Step" in r.stdout and "Dumping Luv" in r.stdout
+
+
+def test_dropin_driver_half_rotated(tmp_path):
+    """half_rotated = true (src/main.cpp:57-72): the full pass writes Luv, a second
+    k-means pass with thc_half_cfac picks new points, the half-rotated pass
+    writes the HalfTransformed* datasets (Pocc o P, Muv with no Cholesky) and
+    THC/dims = [M, K, K_half]."""
+    h = host()
+    s = synthetic.make_system(reps=1, n=16, M=8, thc_cfac=3, max_it=7)
+    occ = np.ascontiguousarray(s.aoR[:, :4])
+    half_cfac = 2
+    kh = half_cfac * s.M
+    orb = str(tmp_path / "orbitals.h5")
+    thc = str(tmp_path / "thc.h5")
+    P = lambda a: a.ctypes.data_as(vp)
+    fg = np.array(s.n, dtype=np.int32)
+    hc = np.eye(s.M)
+    ne = np.array([4, 4], dtype=np.int32)
+    assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(s.aoR), P(fg), P(s.coulG),
+                                         P(hc), P(ne), 0.0, P(occ), occ.shape[1]) == 0
+    inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac, "half_rotated": True,
+           "thc_half_cfac": half_cfac,
+           "interpolating_points": {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}}}
+    (tmp_path / "input.json").write_text(json.dumps(inp))
+    r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    # full pass
+    idx = read(h, thc, "/Hamiltonian/THC/InterpIndx", np.int32)
+    np.testing.assert_array_equal(idx, KO.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, 0)["interp_idx"])
+    luv = read(h, thc, "/Hamiltonian/THC/Luv").reshape(s.K, s.K, 2)[:, :, 0]
+    v_ref = IO.isdf(s.aoR, idx, s.n, s.coulG)["V"]
+    assert np.linalg.norm(luv @ luv.T - v_ref) / np.linalg.norm(v_ref) < 1e-9
+    # half-rotated pass
+    idx_h = read(h, thc, "/Hamiltonian/THC/HalfTransformedFullInterpIndx", np.int32)
+    np.testing.assert_array_equal(idx_h, KO.kmeans(s.grid, s.rho, kh, s.max_it, -1.0, 0)["interp_idx"])
+    np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/THC/HalfTransformedOccInterpIndx", np.int32), idx_h)
+    full = read(h, thc, "/Hamiltonian/THC/HalfTransformedFullOrbitals").reshape(s.M, kh, 2)
+    np.testing.assert_array_equal(full[:, :, 0], s.aoR[idx_h].T)
+    occo = read(h, thc, "/Hamiltonian/THC/HalfTransformedOccOrbitals").reshape(4, kh, 2)
+    np.testing.assert_array_equal(occo[:, :, 0], occ[idx_h].T)
+    muv = read(h, thc, "/Hamiltonian/THC/HalfTransformedMuv").reshape(kh, kh, 2)
+    assert not muv[:, :, 1].any()
+    m_ref = IO.isdf_half(s.aoR, occ, idx_h, s.n, s.coulG)["Muv"]
+    err = np.linalg.norm(muv[:, :, 0] - m_ref) / np.linalg.norm(m_ref)
+    assert err < 1e-9, err
+    np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/THC/dims", np.int32), [s.M, s.K, kh])
+    assert "Dumping half transformed Muv" in r.stdout
 
 
 def test_dropin_missing_key_fails(tmp_path):

@@ -124,3 +124,48 @@ def test_cholesky_luv():
     bad[6, 6] = -2
     _, info = IO.cholesky_luv(bad)
     assert info == 7
+
+
+# ---------------------------------------------------------------- half-rotated path
+
+def test_czt_half_khatri_rao_fixture():
+    """Half-rotated Gram on the reference-held eta: (Pocc o P) restricted to the
+    points is (A A^T) o (B B^T) = (A (.) B)(A (.) B)^T, with A the occupied
+    columns, B all columns and (.) the row-wise Kronecker product (Schur product
+    theorem); with every orbital occupied it is the squared Gram."""
+    eta = np.load(os.path.join(GOLDEN, "etaPm.npy"))
+    occ = eta[:, : eta.shape[1] // 2]
+    idx = np.arange(eta.shape[0])
+    z = IO.czt_half(eta, occ, idx)
+    kr = np.einsum("ma,mi->mai", occ, eta).reshape(eta.shape[0], -1)
+    assert rel(z, kr @ kr.T) < 1e-14
+    np.testing.assert_array_equal(IO.czt_half(eta, eta, idx), IO.czt(eta, idx))
+    # a transposed or swapped factor breaks the identity
+    kr_wrong = np.einsum("ma,mi->mai", occ, eta[::-1]).reshape(eta.shape[0], -1)
+    assert rel(z, kr_wrong @ kr_wrong.T) > 1e-3
+
+
+@pytest.mark.parametrize("n", [(4, 3, 5), (3, 3, 3)])
+def test_fft_vectors_backward_bruteforce(n):
+    rng = np.random.default_rng(7 + sum(n))
+    x = rng.standard_normal((3, int(np.prod(n))))
+    ivmg = IO.fft_vectors_backward(x, n)
+    # e^{+2 pi i k.r / n}: the conjugate of the forward kernel
+    assert rel(ivmg.T, np.conj(_dft_bruteforce(np.conj(x), n))) < 1e-13
+    # for real X the backward transform is the conjugate of the forward one
+    assert rel(ivmg, np.conj(IO.fft_vectors(x, n))) < 1e-13
+
+
+def test_isdf_half_reduces_to_full_and_interpolates():
+    s = synthetic.make_system(reps=1, n=12, M=8, thc_cfac=2, max_it=1)
+    idx = np.sort(np.random.default_rng(1).choice(s.N, s.K, replace=False))
+    full = IO.isdf(s.aoR, idx, s.n, s.coulG)
+    same = IO.isdf_half(s.aoR, s.aoR, idx, s.n, s.coulG)
+    assert rel(same["Muv"], full["V"]) < 1e-12
+    occ = s.aoR[:, : s.M // 2]
+    half = IO.isdf_half(s.aoR, occ, idx, s.n, s.coulG, return_all=True)
+    assert np.abs(half["X"][:, idx] - np.eye(s.K)).max() < 1e-8
+    assert half["imag_max"] < 1e-10 * np.abs(half["Muv"]).max()
+    m = half["Muv"]
+    assert rel(m, m.T) < 1e-13
+    assert np.linalg.eigvalsh(m).min() > -1e-10 * np.abs(m).max()

@@ -110,6 +110,13 @@ class Context:
         L.check(L.lib().thc_czt(self.h, _p(a), a.shape[0], a.shape[1], _p(i), i.shape[0], _p(out)), "thc_czt")
         return out
 
+    def czt_half(self, aoR, aoR_half, idx):
+        a, ah, i = self.f64(aoR), self.f64(aoR_half), self.i32(idx)
+        out = torch.empty((i.shape[0], a.shape[0]), dtype=torch.float64, device=self.device)
+        L.check(L.lib().thc_czt_half(self.h, _p(a), _p(ah), a.shape[0], a.shape[1], ah.shape[1], _p(i),
+                                     i.shape[0], _p(out)), "thc_czt_half")
+        return out
+
     def cct(self, czt, idx):
         z, i = self.f64(czt), self.i32(idx)
         out = torch.empty((i.shape[0], i.shape[0]), dtype=torch.float64, device=self.device)
@@ -159,6 +166,20 @@ class Context:
                               _p(xi), ctypes.byref(stats))
         return {"status": st, "V": v, "Luv": luv, "xi": xi, "cct_info": stats.cct_chol_info,
                 "luv_info": stats.luv_chol_info, "stage_ms": list(stats.stage_ms)}
+
+    def isdf_half(self, aoR, aoR_half, idx, fft_grid, coulG, precision=L.PREC_FP64, refine=0, want_xi=False):
+        a, ah, i, cg = self.f64(aoR), self.f64(aoR_half), self.i32(idx), self.f64(coulG).reshape(-1)
+        n, m = a.shape
+        k = i.shape[0]
+        muv = torch.empty((k, k), dtype=torch.float64, device=self.device)
+        xi = torch.empty((k, n), dtype=torch.float64, device=self.device) if want_xi else None
+        opts = L.IsdfOpts(precision, refine, 0)
+        stats = L.IsdfStats()
+        fg = (ctypes.c_int * 3)(*[int(q) for q in fft_grid])
+        st = L.lib().thc_isdf_half(self.h, _p(a), _p(ah), n, m, ah.shape[1], _p(i), k, fg, _p(cg),
+                                   ctypes.byref(opts), _p(muv), _p(xi), ctypes.byref(stats))
+        return {"status": st, "Muv": muv, "xi": xi, "cct_info": stats.cct_chol_info,
+                "stage_ms": list(stats.stage_ms)}
 
     def factorize(self, grid, rho, aoR, thc_cfac, max_it, fft_grid, coulG, threshold=-1.0, seed=0,
                   precision=L.PREC_FP64, refine=0, want_luv=True):

@@ -143,10 +143,12 @@ __global__ void zero_ints(int* p, int count) {
 }  // namespace
 
 // IVecs ctor + kernel at one rank.
-int isdf_run(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int k, const int fft[3],
-             const double* coulG, const thc_isdf_opts* opts, double* v_out, double* luv_out, double* xi_out,
-             thc_isdf_stats* stats) {
+// aoR_half (n x m_occ, nullable) selects the half-rotated ZC^T (Pocc o P).
+int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m, int m_occ, const int* idx,
+             int k, const int fft[3], const double* coulG, const thc_isdf_opts* opts, double* v_out,
+             double* luv_out, double* xi_out, thc_isdf_stats* stats) {
   if (!ctx || !aoR || !idx || !fft || !coulG || !v_out || n < 1 || m < 1 || k < 1 || k > n) return THC_ERR_ARG;
+  if (aoR_half && m_occ < 1) return THC_ERR_ARG;
   const int prec = opts ? opts->precision : THC_PREC_FP64;
   const int refine = opts ? std::max(0, opts->refine_steps) : 0;
   const int want_luv = opts ? opts->want_luv : 1;
@@ -171,14 +173,24 @@ int isdf_run(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int
   // 1. Zt = (Phi Phi_mu^T)^2, N x K row-major (Zt[r][mu] = ZC^T[mu][r]): B rows
   //    gathered by idx, epilogue squares; this layout lets the solve read both
   //    operands along k and keeps the 4 GB store row-contiguous
+  //    Half-rotated (setup_CZt_half): Zt = P o Pocc, P stored first, then the
+  //    occupied-orbital product multiplied into it in the epilogue.
   {
     GemmArgs g = {};
     g.m = n; g.n = k; g.k = m;
     g.a = aoR; g.lda = m; g.a_trans = 0;
     g.b = aoR; g.ldb = m; g.b_trans = 1; g.b_rows = idx; g.b_nrows = n;
     g.c = B.z; g.ldc = k; g.c_trans = 0;
-    g.alpha = 1.0; g.beta = 0.0; g.epi_square = 1; g.batch = 1;
+    g.alpha = 1.0; g.beta = 0.0; g.epi_square = aoR_half ? 0 : 1; g.batch = 1;
     THC_TRY(gemm_f64(g, s));
+    if (aoR_half) {
+      GemmArgs h = g;
+      h.k = m_occ;
+      h.a = aoR_half; h.lda = m_occ;
+      h.b = aoR_half; h.ldb = m_occ;
+      h.epi_hadamard = 1;
+      THC_TRY(gemm_f64(h, s));
+    }
   }
   THC_HIP(hipEventRecord(ev[1], s));
   // 2. CC^T = ZC^T[:, idx]; Cholesky; explicit inverse
@@ -393,6 +405,27 @@ int thc_czt(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int 
   return THC_OK;
 }
 
+int thc_czt_half(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m, int m_occ, const int* idx,
+                 int k, double* czt) {
+  if (!ctx || !aoR || !aoR_half || !idx || !czt || n < 1 || m < 1 || m_occ < 1 || k < 1) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  GemmArgs g = {};
+  g.m = k; g.n = n; g.k = m;
+  g.a = aoR; g.lda = m; g.a_trans = 0; g.a_rows = idx; g.a_nrows = n;
+  g.b = aoR; g.ldb = m; g.b_trans = 1;
+  g.c = czt; g.ldc = n;
+  g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
+  THC_TRY(gemm_f64(g, ctx->stream));
+  GemmArgs h = g;
+  h.k = m_occ;
+  h.a = aoR_half; h.lda = m_occ;
+  h.b = aoR_half; h.ldb = m_occ;
+  h.epi_hadamard = 1;
+  THC_TRY(gemm_f64(h, ctx->stream));
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  return THC_OK;
+}
+
 int thc_cct(thc_ctx* ctx, const double* czt, long n, const int* idx, int k, double* cct) {
   if (!ctx || !czt || !idx || !cct || n < 1 || k < 1) return THC_ERR_ARG;
   THC_HIP(hipSetDevice(ctx->device));
@@ -490,7 +523,14 @@ int thc_construct_muv(thc_ctx* ctx, const double* b, int k, long g2, double* v) 
 int thc_isdf(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int k, const int fft_grid[3],
              const double* coulG, const thc_isdf_opts* opts, double* v, double* luv, double* xi,
              thc_isdf_stats* stats) {
-  return isdf_run(ctx, aoR, n, m, idx, k, fft_grid, coulG, opts, v, luv, xi, stats);
+  return isdf_run(ctx, aoR, nullptr, n, m, 0, idx, k, fft_grid, coulG, opts, v, luv, xi, stats);
+}
+
+int thc_isdf_half(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m, int m_occ,
+                  const int* idx, int k, const int fft_grid[3], const double* coulG, const thc_isdf_opts* opts,
+                  double* muv, double* xi, thc_isdf_stats* stats) {
+  if (!aoR_half) return THC_ERR_ARG;
+  return isdf_run(ctx, aoR, aoR_half, n, m, m_occ, idx, k, fft_grid, coulG, opts, muv, nullptr, xi, stats);
 }
 
 int thc_factorize(thc_ctx* ctx, const double* grid, const double* rho, const double* aoR, long n, int m,
@@ -503,7 +543,8 @@ int thc_factorize(thc_ctx* ctx, const double* grid, const double* rho, const dou
   const int st = thc_kmeans(ctx, grid, rho, n, (int)k, max_it, threshold, seed, nullptr, interp_idx, nullptr,
                             nullptr, kstats);
   if (st != THC_OK && st < 100) return st;
-  const int st2 = isdf_run(ctx, aoR, n, m, interp_idx, (int)k, fft_grid, coulG, opts, v, luv, nullptr, istats);
+  const int st2 = isdf_run(ctx, aoR, nullptr, n, m, 0, interp_idx, (int)k, fft_grid, coulG, opts, v, luv, nullptr,
+                           istats);
   if (st2 != THC_OK) return st2;
   return st;
 }

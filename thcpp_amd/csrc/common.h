@@ -85,6 +85,8 @@ int ensure_pinned(thc_ctx* ctx, size_t bytes);
 //   b_rows: optional row gather for b_trans = 1 (column n of op(B) = row b_rows[n] of B)
 //   c_trans: write the result transposed (element (m, n) at c[n * ldc + m])
 //   epi_square: C = (alpha * acc)^2 (beta ignored)
+//   epi_hadamard: C = (alpha * acc) * C elementwise (beta ignored; gathered
+//     operand, both operands k-contiguous)
 //   lower: only tiles on/below the diagonal are computed (m, n square)
 //   batch: blockIdx.z, strides in elements
 struct GemmArgs {
@@ -97,6 +99,7 @@ struct GemmArgs {
   int c_trans;   // store C^T: element (m, n) at c[n * ldc + m]
   double alpha, beta;
   int epi_square;
+  int epi_hadamard;
   int lower;
   int batch;
   long stride_a, stride_b, stride_c;

@@ -8,9 +8,11 @@
 
 #include "h5io.h"
 
+// aoR_half (n x m_occ, nullable) adds the occupied orbitals the half-rotated path reads.
 extern "C" int thc_host_write_orbital_file(const char* path, long n, int m, const double* grid, const double* rho,
                                            const double* aoR, const int* fft_grid, const double* coulG,
-                                           const double* hcore, const int* nelec, double e0) {
+                                           const double* hcore, const int* nelec, double e0,
+                                           const double* aoR_half, int m_occ) {
   try {
     thc::h5::File f(path, thc::h5::File::Truncate);
     const uint64_t N = n, M = m;
@@ -19,6 +21,12 @@ extern "C" int thc_host_write_orbital_file(const char* path, long n, int m, cons
     std::vector<double> z(2 * N * M, 0.0);
     for (uint64_t i = 0; i < N * M; ++i) z[2 * i] = aoR[i];
     f.write_compound_complex("aoR", z.data(), {N, M});
+    if (aoR_half && m_occ > 0) {
+      const uint64_t MO = m_occ;
+      std::vector<double> zh(2 * N * MO, 0.0);
+      for (uint64_t i = 0; i < N * MO; ++i) zh[2 * i] = aoR_half[i];
+      f.write_compound_complex("aoR_half", zh.data(), {N, MO});
+    }
     f.write_int("fft_grid", fft_grid, {3, 1});
     f.write_f64("fft_coulomb", coulG, {N, 1});
     std::vector<double> h(2 * M * M, 0.0);

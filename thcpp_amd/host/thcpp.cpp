@@ -210,7 +210,8 @@ std::vector<int> IPoints::kernel(thc::Json& input, ContextHandler::BlacsHandler&
   if (solver == "qrcp") {
     throw std::runtime_error("the QRCP interpolating-point solver is not part of this build (k-means only)");
   }
-  if (half_rotate) throw std::runtime_error("half-rotated interpolating points are not part of this build");
+  // the k-means solver does not depend on half_rotate (src/interpolating_points.cpp:50-58)
+  (void)half_rotate;
   std::cout << " * Using K-Means solver." << std::endl;
   std::cout << std::endl;
   KMeans::KMeansSolver KM(input, BH);
@@ -230,7 +231,6 @@ IVecs::IVecs(thc::Json& input, ContextHandler::BlacsHandler& BH, std::vector<int
   std::cout << "##   Setting up interpolative vector solver.   ##" << std::endl;
   std::cout << "#################################################" << std::endl;
   std::cout << std::endl;
-  if (half_rotated) throw std::runtime_error("the half-rotated THC path is not part of this build");
   {
     thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
     const auto d = fin.dims("aoR");
@@ -239,11 +239,63 @@ IVecs::IVecs(thc::Json& input, ContextHandler::BlacsHandler& BH, std::vector<int
     fft_grid = fin.read_int("fft_grid");
   }
   if (append) thc::h5::File(output_file, thc::h5::File::Truncate);
-  prefix = "";
   nmu = (int)interp_indxs.size();
-  setup_CZt(interp_indxs, BH);
+  if (half_rotated) {
+    // to distinguish Luv between the half rotated and rotated case
+    prefix = "HalfTransformed";
+    half_rotate = true;
+    setup_CZt_half(interp_indxs, BH);
+  } else {
+    prefix = "";
+    half_rotate = false;
+    setup_CZt(interp_indxs, BH);
+  }
   setup_CCt(interp_indxs, BH);
   check_rank(BH);
+}
+
+// <prfx>Orbitals (M, K) and <prfx>InterpIndx (src/interpolating_vectors.cpp:72-90)
+void IVecs::write_orbitals_at_points(const std::vector<double>& ao, int m, const std::vector<int>& interp_indxs,
+                                     const std::string& prfx) {
+  const int k = (int)interp_indxs.size();
+  std::vector<double> orb(2 * (size_t)m * k, 0.0);
+  for (int i = 0; i < m; ++i)
+    for (int mu = 0; mu < k; ++mu) orb[2 * ((size_t)i * k + mu)] = ao[(size_t)interp_indxs[mu] * m + i];
+  thc::h5::File fo(output_file, thc::h5::File::ReadWrite);
+  fo.ensure_group("/Hamiltonian/THC");
+  std::cout << " * Writing aoR_mu to file" << std::endl;
+  fo.write_complex_as_f64("/Hamiltonian/THC/" + prfx + "Orbitals", orb.data(), {(uint64_t)m, (uint64_t)k}, true);
+  std::cout << " * Writing indices of interpolating points to file" << std::endl;
+  fo.write_int("/Hamiltonian/THC/" + prfx + "InterpIndx", interp_indxs.data(), {(uint64_t)k});
+}
+
+// Half-rotated ZC^T = Pocc o P (src/interpolating_vectors.cpp:143-161), with the
+// full-set and occupied-set orbitals at the points written under the
+// HalfTransformedFull / HalfTransformedOcc prefixes.
+void IVecs::setup_CZt_half(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH) {
+  std::vector<double> aoR, aoR_half;
+  int mocc = 0;
+  {
+    thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
+    aoR = real_part_checked(fin.read_complex("aoR"), "aoR");
+    const auto d = fin.dims("aoR_half");
+    if (d.size() < 2 || (long)d[0] != ngrid) throw std::runtime_error("aoR_half must be (N, M_occ)");
+    mocc = (int)d[1];
+    aoR_half = real_part_checked(fin.read_complex("aoR_half"), "aoR_half");
+  }
+  const int k = nmu, m = nbasis;
+  const long n = ngrid;
+  write_orbitals_at_points(aoR, m, interp_indxs, "HalfTransformedFull");
+  write_orbitals_at_points(aoR_half, mocc, interp_indxs, "HalfTransformedOcc");
+  std::cout << " * Constructing CZt matrix" << std::endl;
+  std::cout << " * Matrix Shape: (" << k << ", " << n << ")" << std::endl;
+  double* d_ao = to_dev(aoR);
+  DevFree fa(d_ao);
+  double* d_aoh = to_dev(aoR_half);
+  DevFree fh(d_aoh);
+  d_idx = to_dev(interp_indxs);
+  d_czt = dev_alloc<double>((size_t)k * n);
+  status_ok(thc_czt_half(BH.ctx, d_ao, d_aoh, n, m, mocc, d_idx, k, d_czt), "thc_czt_half");
 }
 
 IVecs::~IVecs() {
@@ -260,19 +312,7 @@ void IVecs::setup_CZt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandl
   }
   const int k = nmu, m = nbasis;
   const long n = ngrid;
-  // THC/Orbitals (M, K) and THC/InterpIndx (src/interpolating_vectors.cpp:72-90)
-  {
-    std::vector<double> orb(2 * (size_t)m * k, 0.0);
-    for (int i = 0; i < m; ++i)
-      for (int mu = 0; mu < k; ++mu) orb[2 * ((size_t)i * k + mu)] = aoR[(size_t)interp_indxs[mu] * m + i];
-    thc::h5::File fo(output_file, thc::h5::File::ReadWrite);
-    fo.ensure_group("/Hamiltonian/THC");
-    std::cout << " * Writing aoR_mu to file" << std::endl;
-    fo.write_complex_as_f64("/Hamiltonian/THC/" + prefix + "Orbitals", orb.data(), {(uint64_t)m, (uint64_t)k},
-                            true);
-    std::cout << " * Writing indices of interpolating points to file" << std::endl;
-    fo.write_int("/Hamiltonian/THC/" + prefix + "InterpIndx", interp_indxs.data(), {(uint64_t)k});
-  }
+  write_orbitals_at_points(aoR, m, interp_indxs, prefix);
   std::cout << " * Constructing CZt matrix" << std::endl;
   std::cout << " * Matrix Shape: (" << k << ", " << n << ")" << std::endl;
   double* d_ao = to_dev(aoR);
@@ -363,7 +403,20 @@ void IVecs::kernel(ContextHandler::BlacsHandler& BH) {
   std::cout << "  * Time to construct Muv: "
             << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds."
             << std::endl;
-  // dump_thc_data (src/interpolating_vectors.cpp:283-315)
+  // dump_thc_data (src/interpolating_vectors.cpp:283-328)
+  if (half_rotate) {
+    // Muv = IVMG^T IVG; for real interpolating vectors IVMG = conj(IVG), so it is
+    // the matrix just formed, written full with no Cholesky
+    std::vector<double> mh = to_host(d_v, (size_t)k * k);
+    std::cout << " * Dumping half transformed Muv data to: " << output_file << "." << std::endl;
+    std::cout << std::endl;
+    std::vector<double> muv(2 * (size_t)k * k, 0.0);
+    for (size_t i = 0; i < (size_t)k * k; ++i) muv[2 * i] = mh[i];
+    thc::h5::File fo(output_file, thc::h5::File::ReadWrite);
+    fo.ensure_group("/Hamiltonian/THC");
+    fo.write_complex_as_f64("/Hamiltonian/THC/" + prefix + "Muv", muv.data(), {(uint64_t)k, (uint64_t)k}, true);
+    return;
+  }
   std::cout << " * Performing Cholesky decomposition on Muv." << std::endl;
   t0 = std::chrono::steady_clock::now();
   int cinfo = 0;
@@ -429,8 +482,16 @@ int thcpp_main(const std::string& input_path) {
     IVSolver.kernel(BH);
     if (!half_rotate) IVSolver.dump_qmcpack_data(thc_cfac, thc_half_cfac, BH);
   }
-  if (find_interp_vec && half_rotate)
-    throw std::runtime_error("the half-rotated THC path is not part of this build");
+  if (find_interp_vec && half_rotate) {
+    // second pass over the half-rotated products (src/main.cpp:63-72)
+    if (thc_half_cfac != thc_cfac) {
+      InterpolatingPoints::IPoints IP(input, BH);
+      interp_indxs = IP.kernel(input, BH, thc_half_cfac, half_rotate);
+    }
+    InterpolatingVectors::IVecs IVSolver(input, BH, interp_indxs, half_rotate, false);
+    IVSolver.kernel(BH);
+    IVSolver.dump_qmcpack_data(thc_cfac, thc_half_cfac, BH);
+  }
   std::cout << " * Total simulation time : "
             << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds."
             << std::endl;

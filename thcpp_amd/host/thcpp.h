@@ -69,6 +69,9 @@ class IVecs {
 
  private:
   void setup_CZt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH);
+  void setup_CZt_half(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH);
+  void write_orbitals_at_points(const std::vector<double>& ao, int m, const std::vector<int>& interp_indxs,
+                                const std::string& prfx);
   void setup_CCt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH);
   void check_rank(ContextHandler::BlacsHandler& BH);
   std::string input_file, output_file;
@@ -77,6 +80,7 @@ class IVecs {
   int nmu = 0;
   std::vector<int> fft_grid;
   std::string prefix;
+  bool half_rotate = false;
   // device buffers
   double* d_czt = nullptr;
   double* d_cct = nullptr;
