@@ -210,19 +210,34 @@ __global__ __launch_bounds__(256) void cs_colscan(int* tile_count, int ntiles, i
   const int b = blockIdx.x * 64 + lane;
   const int per = (ntiles + 3) / 4;
   const int t0 = wave * per, t1 = min(ntiles, t0 + per);
+  // chunks of 16 tiles: every load of a chunk is issued before its first use
+  // and before the chunk's stores (on this target stores and loads share one
+  // completion counter, so a load after a store would wait for the store)
+  constexpr int CH = 16;
   int run = 0;
   if (b < nbins)
-    for (int t = t0; t < t1; ++t) run += tile_count[(long)t * nbins + b];
+    for (int t = t0; t < t1; t += CH) {
+      int c[CH];
+#pragma unroll
+      for (int q = 0; q < CH; ++q) c[q] = (t + q < t1) ? tile_count[(long)(t + q) * nbins + b] : 0;
+#pragma unroll
+      for (int q = 0; q < CH; ++q) run += c[q];
+    }
   part[wave][lane] = run;
   __syncthreads();
   int off = 0;
   for (int w = 0; w < wave; ++w) off += part[w][lane];
   if (b < nbins) {
-    for (int t = t0; t < t1; ++t) {
-      const long o = (long)t * nbins + b;
-      const int c = tile_count[o];
-      tile_count[o] = off;
-      off += c;
+    for (int t = t0; t < t1; t += CH) {
+      int c[CH];
+#pragma unroll
+      for (int q = 0; q < CH; ++q) c[q] = (t + q < t1) ? tile_count[(long)(t + q) * nbins + b] : 0;
+#pragma unroll
+      for (int q = 0; q < CH; ++q)
+        if (t + q < t1) {
+          tile_count[(long)(t + q) * nbins + b] = off;
+          off += c[q];
+        }
     }
     if (wave == 3) total[b] = off;
   }
@@ -315,8 +330,21 @@ __global__ __launch_bounds__(256) void assign(const double* sx, const double* sy
   double box[6];
   for (int d = 0; d < 6; ++d) box[d] = b[d];
 
+  // this thread's centroids j = q * 256 + tid (k <= MAXB), loaded once
+  constexpr int CPT = MAXB / 256;
+  double rx[CPT], ry[CPT], rz[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    const int j = q * 256 + tid;
+    const bool ok = j < k;
+    rx[q] = ok ? cen[j * 3] : 0.0;
+    ry[q] = ok ? cen[j * 3 + 1] : 0.0;
+    rz[q] = ok ? cen[j * 3 + 2] : 0.0;
+  }
   double u = DBL_MAX;
-  for (int j = tid; j < k; j += 256) u = fmin(u, ub2_box(cen[j * 3], cen[j * 3 + 1], cen[j * 3 + 2], box));
+#pragma unroll
+  for (int q = 0; q < CPT; ++q)
+    if (q * 256 + tid < k) u = fmin(u, ub2_box(rx[q], ry[q], rz[q], box));
   u = wave_min(u);
   if (lane == 0) redd[wave] = u;
   __syncthreads();
@@ -324,14 +352,12 @@ __global__ __launch_bounds__(256) void assign(const double* sx, const double* sy
   const double thr = u * PRUNE;
 
   int ncand = 0;
-  for (int j0 = 0; j0 < k; j0 += 256) {
-    const int j = j0 + tid;
-    double cx = 0, cy = 0, cz = 0;
-    bool flag = false;
-    if (j < k) {
-      cx = cen[j * 3]; cy = cen[j * 3 + 1]; cz = cen[j * 3 + 2];
-      flag = lb2_box(cx, cy, cz, box) <= thr;
-    }
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    if (q * 256 >= k) break;  // uniform over the workgroup
+    const int j = q * 256 + tid;
+    const double cx = rx[q], cy = ry[q], cz = rz[q];
+    const bool flag = j < k && lb2_box(cx, cy, cz, box) <= thr;
     const unsigned long long bal = __ballot(flag);
     __syncthreads();
     if (lane == 0) redi[wave] = __popcll(bal);
@@ -453,8 +479,32 @@ __global__ __launch_bounds__(1024) void diff_control(const double* nxt, const do
   }
   __syncthreads();
   if (tid == 0) {
+    // strictly left-to-right sum (the reference's order), with the LDS reads
+    // of the next 16 terms issued before the adds of the current 16
     double acc = 0.0;
-    for (int i = 0; i < len; ++i) acc += terms[i];
+    const int nb = len / 16;
+    double a[16], c[16];
+    if (nb > 0) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) a[q] = terms[q];
+    }
+    for (int bi = 0; bi < nb; bi += 2) {
+      const bool more = bi + 1 < nb;
+      if (more) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) c[q] = terms[(bi + 1) * 16 + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc += a[q];
+      if (!more) break;
+      if (bi + 2 < nb) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) a[q] = terms[(bi + 2) * 16 + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc += c[q];
+    }
+    for (int i = nb * 16; i < len; ++i) acc += terms[i];
     const double diff = scale * sqrt(acc);
     const double pct = 100 * ((double)st->changed) / (double)n;
     diffs[it] = diff;

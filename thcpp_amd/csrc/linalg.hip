@@ -87,14 +87,17 @@ __device__ inline double readlane_d(double v, int lane) {
 // Factor the jb x jb diagonal block at (j0, j0) in place (lower part only).
 // Right-looking, lane r owning row r in registers: at step k the pivot
 // L[k][k] = sqrt(A[k][k]) is read from lane k, column k is scaled by its
-// reciprocal, and every lane drops L[r][k] L[c][k] from its entries c > k,
-// with L[c][k] read from lane c.  Rows/columns >= jb are an identity
-// extension.  Also writes Lt (NB x NB, Lt[c][r] = L[r][c], zero above the
-// diagonal of L) and rinv[c] = 1 / L[c][c] for the panel solve.  info: first
-// failing leading minor (1-based, global), kept if already set.
+// reciprocal and published in LDS as row k of Lt, and every lane drops
+// L[r][k] L[c][k] from its entries c > k (broadcast reads of Lt[k][c]).
+// Rows/columns >= jb are an identity extension.  Also writes lt (NB x NB,
+// lt[c][r] = L[r][c], zero above the diagonal of L) and rinv[c] = 1 / L[c][c]
+// for the panel solve.  info: first failing leading minor (1-based, global),
+// kept if already set.  (A rolled form with the rows in LDS is smaller code
+// but 2.3x slower: its read-modify-write chains are LDS-latency bound.)
 __global__ __launch_bounds__(64) void potrf_diag(double* a, long lda, int j0, int jb, int* info, double* lt,
                                                  double* rinv) {
   __shared__ double Ls[NB][NB + 1];
+  __shared__ __attribute__((aligned(16))) double Lt[NB][NB];
   const int r = threadIdx.x;
   load_lower_block<false>(a, lda, j0, j0, jb, Ls);
   wave_sync_lds();
@@ -110,9 +113,12 @@ __global__ __launch_bounds__(64) void potrf_diag(double* a, long lda, int j0, in
     const double piv = sqrt(d);
     const double rp = 1.0 / piv;
     if (r == k) my_rp = rp;
-    row[k] = (r == k) ? piv : row[k] * rp;
+    const double v = (r == k) ? piv : row[k] * rp;
+    row[k] = v;
+    Lt[k][r] = v;
+    wave_sync_lds();
 #pragma unroll
-    for (int c = k + 1; c < NB; ++c) row[c] = fma(-row[k], readlane_d(row[k], c), row[c]);
+    for (int c = k + 1; c < NB; ++c) row[c] = fma(-v, Lt[k][c], row[c]);
   }
   if (r == 0 && bad && *info == 0) *info = j0 + bad;
   rinv[r] = my_rp;
@@ -152,11 +158,29 @@ __global__ __launch_bounds__(64) void trsm_panel(double* a, long lda, int j0, in
   double x[NB];
 #pragma unroll
   for (int c = 0; c < NB; ++c) x[c] = Xs[t][c];
+  // col holds step c's factors: col[c] = 1 / L[c][c], col[c2] = L[c2][c] (c2 > c).
+  // The next step's column is read before this step's FMAs, behind a compiler
+  // fence, so the LDS latency overlaps the arithmetic.
+  double col[NB];
+  col[0] = Ri[0];
+#pragma unroll
+  for (int c2 = 1; c2 < NB; ++c2) col[c2] = Lt[0][c2];
 #pragma unroll
   for (int c = 0; c < NB; ++c) {
-    x[c] *= Ri[c];
+    double nxt[NB];
+    if (c + 1 < NB) {
+      nxt[c + 1] = Ri[c + 1];
 #pragma unroll
-    for (int c2 = c + 1; c2 < NB; ++c2) x[c2] = fma(-x[c], Lt[c][c2], x[c2]);
+      for (int c2 = c + 2; c2 < NB; ++c2) nxt[c2] = Lt[c + 1][c2];
+    }
+    __asm__ volatile("" ::: "memory");
+    x[c] *= col[c];
+#pragma unroll
+    for (int c2 = c + 1; c2 < NB; ++c2) x[c2] = fma(-x[c], col[c2], x[c2]);
+    if (c + 1 < NB) {
+#pragma unroll
+      for (int c2 = c + 1; c2 < NB; ++c2) col[c2] = nxt[c2];
+    }
   }
 #pragma unroll
   for (int c = 0; c < NB; ++c) Xs[t][c] = x[c];
