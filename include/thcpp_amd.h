@@ -45,7 +45,7 @@ enum thc_precision {
   THC_PREC_FP32 = 1,  /* FP32 MFMA for ZC^T and the G contraction, FP32 FFT; FP64 Cholesky/solve/V sum */
   THC_PREC_BF16 = 2,  /* reserved */
   THC_PREC_MIXED = 3  /* FP64 ZC^T, Cholesky and solve; FP32 FFT; V from exact int8 MFMA products of
-                         5 signed 7-bit digits per value (per row and 4096-column chunk exponents),
+                         5 signed 7-bit digits per value (per row and 8192-column chunk exponents),
                          summed in FP64 */
 };
 

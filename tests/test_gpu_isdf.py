@@ -178,15 +178,16 @@ def test_isdf_mixed_int8_cfg0(ctx, cfg0, cfg0_kmeans):
     np.testing.assert_array_equal(v, v.T)
 
 
-def test_isdf_mixed_uneven_sizes(ctx):
-    """K not a multiple of the 128 tile, grid with odd edges, spectrum length not
-    a multiple of the 4096 chunk."""
+@pytest.mark.parametrize("n,m,k", [((15, 12, 10), 9, 37), ((40, 36, 30), 24, 150)])
+def test_isdf_mixed_uneven_sizes(ctx, n, m, k):
+    """K not a multiple of the 128 tile (and spanning two tiles), grids with odd
+    edges, spectrum length below one 8192-column chunk and over five chunks with
+    a partial last one (2 * 40 * 36 * 16 = 46080)."""
     from thcpp_amd import synthetic
     rng = np.random.default_rng(7)
-    n = (15, 12, 10)
     N = int(np.prod(n))
-    a = np.linalg.qr(rng.standard_normal((N, 9)))[0]
-    idx = np.sort(rng.choice(N, 37, replace=False)).astype(np.int32)
+    a = np.linalg.qr(rng.standard_normal((N, m)))[0]
+    idx = np.sort(rng.choice(N, k, replace=False)).astype(np.int32)
     cg = synthetic.coulomb_kernel(6.5, n)
     ref = ctx.isdf(a, idx, n, cg)
     out = ctx.isdf(a, idx, n, cg, precision=L.PREC_MIXED)

@@ -48,6 +48,16 @@ struct FftParams {
   const void* twz; const void* twy; const void* twx;
 };
 
+// Division by a run-time divisor d >= 1 of dividends e with e * d < 2^32:
+// q = mulhi(e, floor((2^32 - 1) / d) + 1) is exact there, one multiply in place
+// of a division sequence (the index arithmetic of every butterfly and load).
+// Here e < 2^14 and d <= 128.
+struct FDiv {
+  unsigned d, m;
+  __device__ explicit FDiv(int dd) : d((unsigned)dd), m(dd > 1 ? 0xFFFFFFFFu / (unsigned)dd + 1u : 0u) {}
+  __device__ int div(int e) const { return d > 1 ? (int)__umulhi((unsigned)e, m) : e; }
+};
+
 // One Stockham stage over 'nlines' lines of length n, read src write dst.
 template <class T, int R>
 __device__ inline void stage(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, int n, int ns, int nlines,
@@ -55,10 +65,11 @@ __device__ inline void stage(const C2<T>* __restrict__ src, C2<T>* __restrict__ 
   const int nb = n / R;
   const int total = nlines * nb;
   const int tstep = n / (ns * R);
+  const FDiv fnb(nb), fns(ns);
   for (int w = threadIdx.x; w < total; w += blockDim.x) {
-    const int line = w / nb, t = w % nb;
+    const int line = fnb.div(w), t = w - line * nb;
     const int base = line * ls;
-    const int j = t % ns;
+    const int tq = fns.div(t), j = t - tq * ns;
     C2<T> x[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) x[q] = src[base + (t + q * nb) * es];
@@ -87,7 +98,7 @@ __device__ inline void stage(const C2<T>* __restrict__ src, C2<T>* __restrict__ 
         y[p] = acc;
       }
     }
-    const int d0 = (t / ns) * ns * R + j;
+    const int d0 = tq * ns * R + j;
 #pragma unroll
     for (int q = 0; q < R; ++q) dst[base + (d0 + q * ns) * es] = y[q];
   }
@@ -120,32 +131,47 @@ __global__ __launch_bounds__(256) void pass_a(const double* __restrict__ x, C2<T
   const int cap = max(n1p * n2, n1 * n2h);
   C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
   C2<T>* bufb = bufa + cap;
+  C2<T>* twz = bufb + cap;  // twiddle tables staged in LDS (each stage reads them)
+  C2<T>* twy = twz + n2;
   const long mu = blockIdx.y, ix = blockIdx.x;
+  for (int j = threadIdx.x; j < n2; j += blockDim.x) twz[j] = reinterpret_cast<const C2<T>*>(p.twz)[j];
+  for (int j = threadIdx.x; j < n1; j += blockDim.x) twy[j] = reinterpret_cast<const C2<T>*>(p.twy)[j];
   const double* src = x + (mu * p.n0 + ix) * (long)n1 * n2;
-  // pack rows (2r, 2r+1) as re/im
-  for (int e = threadIdx.x; e < n1p * n2; e += blockDim.x) {
-    const int r = e / n2, z = e % n2;
-    const int y0 = 2 * r, y1 = 2 * r + 1;
-    const T re = (T)src[(long)y0 * n2 + z];
-    const T im = (y1 < n1) ? (T)src[(long)y1 * n2 + z] : T(0);
-    bufa[e] = C2<T>{re, im};
+  // pack rows (2r, 2r+1) as re/im; LB loads per thread are issued before the
+  // first LDS store so the plane is fetched in one or two memory round trips
+  constexpr int LB = 8;
+  const int tot = n1p * n2;
+  const FDiv fn2(n2), fn2h(n2h);
+  for (int e0 = threadIdx.x; e0 < tot; e0 += LB * blockDim.x) {
+    double re[LB], im[LB];
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+      const int e = e0 + b * (int)blockDim.x;
+      const int r = fn2.div(e), z = e - r * n2;
+      const bool ok = e < tot, ok1 = ok && 2 * r + 1 < n1;
+      re[b] = src[ok ? (long)(2 * r) * n2 + z : 0];
+      im[b] = ok1 ? src[(long)(2 * r + 1) * n2 + z] : 0.0;
+    }
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+      const int e = e0 + b * (int)blockDim.x;
+      if (e < tot) bufa[e] = C2<T>{(T)re[b], (T)im[b]};
+    }
   }
   __syncthreads();
-  const C2<T>* twz = reinterpret_cast<const C2<T>*>(p.twz);
   C2<T>* res = fft_lines<T>(bufa, bufb, p.pz, n1p, 1, n2, twz);
   C2<T>* h = (res == bufa) ? bufb : bufa;
   // unpack: A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / (2i)
   for (int e = threadIdx.x; e < n1p * n2h; e += blockDim.x) {
-    const int r = e / n2h, kz = e % n2h;
+    const int r = fn2h.div(e), kz = e - r * n2h;
     const C2<T> z1 = res[r * n2 + kz];
-    const C2<T> z2 = res[r * n2 + (n2 - kz) % n2];
+    const C2<T> z2 = res[r * n2 + (kz == 0 ? 0 : n2 - kz)];
     const C2<T> a = C2<T>{(z1.x + z2.x) * T(0.5), (z1.y - z2.y) * T(0.5)};
     const C2<T> bb = C2<T>{(z1.y + z2.y) * T(0.5), (z2.x - z1.x) * T(0.5)};
     h[(2 * r) * n2h + kz] = a;
     if (2 * r + 1 < n1) h[(2 * r + 1) * n2h + kz] = bb;
   }
   __syncthreads();
-  const C2<T>* twy = reinterpret_cast<const C2<T>*>(p.twy);
   C2<T>* other = (h == bufa) ? bufb : bufa;
   C2<T>* fin = fft_lines<T>(h, other, p.py, n2h, n2h, 1, twy);
   C2<T>* dst = t + (mu * p.n0 + ix) * (long)n1 * n2h;
@@ -160,19 +186,33 @@ __global__ __launch_bounds__(256) void pass_b(const C2<T>* __restrict__ t, const
   const int n0 = p.n0, n1 = p.n1, n2 = p.n2, n2h = p.n2h;
   C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
   C2<T>* bufb = bufa + n0 * n2h;
+  C2<T>* twx = bufb + n0 * n2h;  // twiddle table staged in LDS
+  for (int j = threadIdx.x; j < n0; j += blockDim.x) twx[j] = reinterpret_cast<const C2<T>*>(p.twx)[j];
   const long mu = blockIdx.y;
   const int iy = blockIdx.x;
   const C2<T>* src = t + mu * (long)n0 * n1 * n2h;
-  for (int e = threadIdx.x; e < n0 * n2h; e += blockDim.x) {
-    const int ix = e / n2h, kz = e % n2h;
-    bufa[e] = src[((long)ix * n1 + iy) * n2h + kz];
+  constexpr int LB = 8;  // loads in flight per thread, as in pass_a
+  const int tot = n0 * n2h;
+  const FDiv fn2h(n2h);
+  for (int e0 = threadIdx.x; e0 < tot; e0 += LB * blockDim.x) {
+    C2<T> v[LB];
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+      const int e = e0 + b * (int)blockDim.x;
+      const int ix = fn2h.div(e), kz = e - ix * n2h;
+      v[b] = src[e < tot ? ((long)ix * n1 + iy) * n2h + kz : 0];
+    }
+#pragma unroll
+    for (int b = 0; b < LB; ++b) {
+      const int e = e0 + b * (int)blockDim.x;
+      if (e < tot) bufa[e] = v[b];
+    }
   }
   __syncthreads();
-  const C2<T>* twx = reinterpret_cast<const C2<T>*>(p.twx);
   C2<T>* fin = fft_lines<T>(bufa, bufb, p.px, n2h, n2h, 1, twx);
   T* dst = out + mu * (long)n0 * n1 * n2h * 2;
   for (int e = threadIdx.x; e < n0 * n2h; e += blockDim.x) {
-    const int kx = e / n2h, kz = e % n2h;
+    const int kx = fn2h.div(e), kz = e - kx * n2h;
     const long g = ((long)kx * n1 + iy) * n2 + kz;
     const double w = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
     const T f = (T)sqrt(w * coulG[g]);
@@ -206,14 +246,15 @@ __global__ void twiddle_kernel(int n, C2<T>* tw) {
 
 constexpr int MAXN = 128;
 
+// two ping-pong buffers plus the twiddle tables of the pass
 template <class T>
 size_t lds_a(int n1, int n2) {
   const int n2h = n2 / 2 + 1, n1p = (n1 + 1) / 2;
-  return 2 * sizeof(C2<T>) * (size_t)std::max(n1p * n2, n1 * n2h);
+  return 2 * sizeof(C2<T>) * (size_t)std::max(n1p * n2, n1 * n2h) + sizeof(C2<T>) * (size_t)(n1 + n2);
 }
 template <class T>
 size_t lds_b(int n0, int n2) {
-  return 2 * sizeof(C2<T>) * (size_t)n0 * (n2 / 2 + 1);
+  return 2 * sizeof(C2<T>) * (size_t)n0 * (n2 / 2 + 1) + sizeof(C2<T>) * (size_t)n0;
 }
 
 template <class T>

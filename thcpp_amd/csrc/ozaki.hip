@@ -23,7 +23,7 @@ namespace {
 typedef int i4 __attribute__((ext_vector_type(4)));
 
 constexpr int OZ_S = 5;       // digits per value
-constexpr int OZ_CH = 4096;   // columns per chunk (one exponent per row and chunk)
+constexpr int OZ_CH = 8192;   // columns per chunk (one exponent per row and chunk)
 constexpr int OZ_T = 128;     // tile edge
 constexpr int OZ_KS = 64;     // k per MFMA step
 constexpr int OZ_LD = OZ_KS + 16;  // LDS row stride in bytes (16 bytes of padding)
