@@ -15,6 +15,13 @@
 // Each workgroup takes one 128 x 128 lower tile and one chunk, converts its
 // groups to double, scales by 2^(e_r + e_c) and writes a partial tile; the
 // partials are summed in chunk order in FP64.
+// Slabs (5 digit planes x 128 rows x 64 columns per operand) are double
+// buffered in LDS (2 x 80 KB, the whole LDS of a CU) and filled by direct
+// global-to-LDS loads (global_load_lds_dwordx4: lane L of a wave writes bytes
+// 16 L .. 16 L + 15 of the wave's 1 KB block), so the next step's slabs stream
+// in while the current step's MFMAs run.  Rows are 64 bytes with no padding;
+// 16-byte segment s of row r is stored at segment (s + r / 4) mod 4, which
+// keeps the 16-row fragment reads free of bank conflicts.
 #include "common.h"
 
 namespace thc {
@@ -26,7 +33,6 @@ constexpr int OZ_S = 5;       // digits per value
 constexpr int OZ_CH = 8192;   // columns per chunk (one exponent per row and chunk)
 constexpr int OZ_T = 128;     // tile edge
 constexpr int OZ_KS = 64;     // k per MFMA step
-constexpr int OZ_LD = OZ_KS + 16;  // LDS row stride in bytes (16 bytes of padding)
 static_assert((long)OZ_S * OZ_CH * 127 * 127 < 2147483647L, "int32 group sums must not overflow");
 
 __device__ inline double block_max(double v, double* red) {
@@ -81,11 +87,15 @@ __global__ __launch_bounds__(256) void oz_split(const T* __restrict__ b, long ld
 // hold tiles of the same chunk and walk its columns together, so the digit
 // rows they share are fetched into that XCD's L2 once rather than once per
 // tile.
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ inline int oz_seg(int r, int s) { return (s + (r >> 2)) & 3; }
+
 __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S, long g2p, int k,
                                                const int* __restrict__ E, int nchunk, double* __restrict__ work,
                                                long ntiles) {
-  __shared__ __attribute__((aligned(16))) signed char As[OZ_S][OZ_T][OZ_LD];
-  __shared__ __attribute__((aligned(16))) signed char Bs[OZ_S][OZ_T][OZ_LD];
+  __shared__ __attribute__((aligned(16))) signed char As[2][OZ_S][OZ_T][OZ_KS];
+  __shared__ __attribute__((aligned(16))) signed char Bs[2][OZ_S][OZ_T][OZ_KS];
   const long total = ntiles * nchunk;
   const long per_xcd = (total + 7) / 8;
   const long w = (long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
@@ -108,31 +118,53 @@ __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[d][i][j] = i4{0, 0, 0, 0};
 
+  // A slab is OZ_S * OZ_T rows of 64 bytes = 40 blocks of 16 rows (1 KB); wave
+  // wv loads blocks wv, wv + 8, ... of both operands.  Lane L fills row
+  // 16 b + L / 4 of the block, physical segment L % 4, from the logical segment
+  // that maps there.  Rows past k read row k - 1 (valid memory); they only feed
+  // output rows or columns >= k, which are never stored.
+  constexpr int NBLK = OZ_S * OZ_T / 16;
+  static_assert(NBLK % 8 == 0, "slab blocks must split evenly over the 8 waves");
   const long kbase = (long)c * OZ_CH;
-  for (int ks = 0; ks < OZ_CH / OZ_KS; ++ks) {
+  const int lrow = lane >> 2, pseg = lane & 3;
+  auto issue = [&](int ks, int buf) {
     const long k0 = kbase + (long)ks * OZ_KS;
-    // slabs: OZ_S digit planes x 128 rows x 64 bytes for each operand, 16 B per load
-    for (int e = tid; e < OZ_S * OZ_T * 4; e += 512) {
-      const int p = e / (OZ_T * 4), r = (e / 4) % OZ_T, seg = e % 4;
-      const int ra = tm * OZ_T + r, rb = tn * OZ_T + r;
-      i4 va = i4{0, 0, 0, 0}, vb = i4{0, 0, 0, 0};
-      if (ra < k) va = *reinterpret_cast<const i4*>(S + ((long)p * k + ra) * g2p + k0 + seg * 16);
-      if (rb < k) vb = *reinterpret_cast<const i4*>(S + ((long)p * k + rb) * g2p + k0 + seg * 16);
-      *reinterpret_cast<i4*>(&As[p][r][seg * 16]) = va;
-      *reinterpret_cast<i4*>(&Bs[p][r][seg * 16]) = vb;
+#pragma unroll
+    for (int q = 0; q < NBLK / 8; ++q) {
+      const int b = wave + 8 * q;
+      const int fr = 16 * b + lrow;             // flattened (plane, row)
+      const int p = fr / OZ_T, r = fr % OZ_T;
+      const int ls = (pseg - (r >> 2)) & 3;     // logical segment stored at pseg
+      const int ra = min(tm * OZ_T + r, k - 1), rb = min(tn * OZ_T + r, k - 1);
+      __builtin_amdgcn_global_load_lds(S + ((long)p * k + ra) * g2p + k0 + ls * 16,
+                                       (lds_void*)(&As[buf][0][0][0] + b * 1024), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(S + ((long)p * k + rb) * g2p + k0 + ls * 16,
+                                       (lds_void*)(&Bs[buf][0][0][0] + b * 1024), 16, 0, 0);
     }
-    __syncthreads();
+  };
+  constexpr int NKS = OZ_CH / OZ_KS;
+  issue(0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int ks = 0; ks < NKS; ++ks) {
+    const int cur = ks & 1;
+    if (ks + 1 < NKS) issue(ks + 1, cur ^ 1);
     i4 bq[OZ_S][2];
 #pragma unroll
     for (int q = 0; q < OZ_S; ++q)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-        bq[q][j] = *reinterpret_cast<const i4*>(&Bs[q][wn * 32 + j * 16 + l16][l4 * 16]);
+      for (int j = 0; j < 2; ++j) {
+        const int r = wn * 32 + j * 16 + l16;
+        bq[q][j] = *reinterpret_cast<const i4*>(&Bs[cur][q][r][oz_seg(r, l4) * 16]);
+      }
 #pragma unroll
     for (int p = 0; p < OZ_S; ++p) {
       i4 ap[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) ap[i] = *reinterpret_cast<const i4*>(&As[p][wm * 64 + i * 16 + l16][l4 * 16]);
+      for (int i = 0; i < 4; ++i) {
+        const int r = wm * 64 + i * 16 + l16;
+        ap[i] = *reinterpret_cast<const i4*>(&As[cur][p][r][oz_seg(r, l4) * 16]);
+      }
 #pragma unroll
       for (int q = 0; q < OZ_S - p; ++q)
 #pragma unroll
@@ -141,6 +173,9 @@ __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S
           for (int j = 0; j < 2; ++j)
             acc[p + q][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ap[i], bq[q][j], acc[p + q][i][j], 0, 0, 0);
     }
+    // the next slabs have landed (this wave's loads) and every wave is done
+    // reading the current ones before they are refilled
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   }
   // scale and write the partial tile: D[4 (l/16) + v][l % 16]
