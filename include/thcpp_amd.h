@@ -72,7 +72,9 @@ typedef struct thc_isdf_opts {
 typedef struct thc_isdf_stats {
   int cct_chol_info;       /* 0, or the 1-based order of the first non-positive leading minor of CC^T */
   int luv_chol_info;       /* same for V */
-  double stage_ms[8];      /* device time: 0 ZC^T, 1 CC^T+chol+inverse, 2 solve, 3 FFT, 4 V, 5 Luv, 6 total, 7 refinement */
+  double stage_ms[8];      /* device time: 0 ZC^T, 1 the part of CC^T + Cholesky + inverse not hidden behind ZC^T
+                              (they run on a second stream beside it), 2 solve, 3 FFT, 4 V, 5 Luv, 6 total,
+                              7 refinement */
 } thc_isdf_stats;
 
 /* ---------------------------------------------------------------- context */

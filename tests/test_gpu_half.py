@@ -65,6 +65,10 @@ def test_isdf_half_vs_oracle(ctx, half_case):
     print("half-rotated cfg0 K=%d rel Muv %.3e rel xi %.3e" % (len(idx), e, ex))
     # QR (oracle) vs Cholesky-based inverse (GPU) at cond 3.5e8
     assert e < 1e-7 and ex < 1e-6
+    # xi_mu(r_nu) = delta: CC^T (formed from Phi_mu beside the ZC^T GEMM) must be
+    # the columns of ZC^T at the points
+    xi = out["xi"].cpu().numpy()
+    assert np.abs(xi[:, idx] - np.eye(len(idx))).max() < 1e-6
     m = out["Muv"].cpu().numpy()
     np.testing.assert_array_equal(m, m.T)
 

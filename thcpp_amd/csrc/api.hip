@@ -68,16 +68,15 @@ __global__ __launch_bounds__(256) void gather_cols(const double* z, long ldz, co
   }
 }
 
-// C[mu][nu] = Zt[idx[nu]][mu] (ZC^T stored transposed, N x K)
-__global__ __launch_bounds__(256) void gather_rows_t(const double* zt, long n, const int* idx, int k, double* c) {
-  const long total = (long)k * k;
+// rows idx of a (n x m) into out (k x m); indices outside [0, n) give zero rows
+__global__ __launch_bounds__(256) void gather_rows(const double* a, long n, int m, const int* idx, int k,
+                                                   double* out) {
+  const long total = (long)k * m;
   for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += 256L * gridDim.x) {
-    const long q = e / k, r = e % k;  // read row idx[q] of Zt contiguously
-    const long row = idx[q];
-    c[r * k + q] = (row >= 0 && row < n) ? zt[row * k + r] : 0.0;
+    const long row = idx[e / m];
+    out[e] = (row >= 0 && row < n) ? a[row * m + e % m] : 0.0;
   }
 }
-
 
 __global__ __launch_bounds__(256) void copy_f64(const double* x, double* y, long count) {
   for (long e = blockIdx.x * 256L + threadIdx.x; e < count; e += 256L * gridDim.x) y[e] = x[e];
@@ -92,15 +91,16 @@ inline int grid_for(long count) { return (int)std::min<long>(8192, std::max<long
 
 struct IsdfBufs {
   double *z, *x, *c, *l, *ci, *r, *v;
+  double *phim, *phio;  // Phi rows at the points (k x m), occupied set (k x m_occ)
   void* b;      // weighted half spectrum (f64 or f32)
   void *chol_w, *inv_w, *fft_w, *syrk_w;
   int* info;
 };
 
-size_t isdf_bytes(long n, int k, const int fft[3], int prec, int refine, bool own_x) {
+size_t isdf_bytes(long n, int k, int m, int m_occ, const int fft[3], int prec, int refine, bool own_x) {
   const long g2 = half_spectrum_len(fft);
   const size_t kk = (size_t)k * k;
-  size_t b = 0;
+  size_t b = carve_bytes<double>((size_t)k * m) + carve_bytes<double>((size_t)k * std::max(m_occ, 1));
   b += carve_bytes<double>((size_t)k * n);                   // z
   if (own_x) b += carve_bytes<double>((size_t)k * n);        // x
   b += 4 * carve_bytes<double>(kk);                          // c, l, ci, v
@@ -113,10 +113,13 @@ size_t isdf_bytes(long n, int k, const int fft[3], int prec, int refine, bool ow
   return b;
 }
 
-bool isdf_carve(Arena& a, long n, int k, const int fft[3], int prec, int refine, double* xi, IsdfBufs& B) {
+bool isdf_carve(Arena& a, long n, int k, int m, int m_occ, const int fft[3], int prec, int refine, double* xi,
+                IsdfBufs& B) {
   const long g2 = half_spectrum_len(fft);
   const size_t kk = (size_t)k * k;
   B = IsdfBufs{};
+  B.phim = a.take<double>((size_t)k * m);
+  B.phio = a.take<double>((size_t)k * std::max(m_occ, 1));
   B.z = a.take<double>((size_t)k * n);
   B.x = xi ? xi : a.take<double>((size_t)k * n);
   B.c = a.take<double>(kk);
@@ -132,7 +135,7 @@ bool isdf_carve(Arena& a, long n, int k, const int fft[3], int prec, int refine,
   B.fft_w = a.take<char>(fft_work_bytes(k, fft, fprec));
   B.syrk_w = a.take<char>(prec == THC_PREC_MIXED ? oz_syrk_work_bytes(k, g2) : syrk_work_bytes(k, g2, fprec));
   B.info = a.take<int>(16);
-  return B.z && B.x && B.c && B.l && B.ci && B.v && (refine <= 0 || B.r) && B.b &&
+  return B.phim && B.phio && B.z && B.x && B.c && B.l && B.ci && B.v && (refine <= 0 || B.r) && B.b &&
          B.chol_w && B.inv_w && B.fft_w && B.syrk_w && B.info;
 }
 
@@ -160,16 +163,51 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   if (nfft != n) return THC_ERR_ARG;
   THC_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  THC_TRY(ctx->ws.reserve(isdf_bytes(n, k, fft, prec, refine, xi_out == nullptr)));
+  THC_TRY(ctx->ws.reserve(isdf_bytes(n, k, m, m_occ, fft, prec, refine, xi_out == nullptr)));
   ctx->ws.reset();
   IsdfBufs B;
-  if (!isdf_carve(ctx->ws, n, k, fft, prec, refine, xi_out, B)) return THC_ERR_NO_MEMORY;
+  if (!isdf_carve(ctx->ws, n, k, m, m_occ, fft, prec, refine, xi_out, B)) return THC_ERR_NO_MEMORY;
   const long g2 = half_spectrum_len(fft);
   hipEvent_t* ev = ctx->ev;
 
   hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, B.info, 16);
   hipLaunchKernelGGL(check_idx, dim3(grid_for(k)), dim3(256), 0, s, idx, k, n, B.info + 2);
   THC_HIP(hipEventRecord(ev[0], s));
+  // 2 (on the second stream, beside stage 1). CC^T = ZC^T[:, idx] formed
+  //    directly from Phi_mu: the GEMM below sees the same operand values in
+  //    the same roles and k order as stage 1 does at those entries, so CC^T
+  //    equals the gathered columns of ZC^T bit for bit.  Then the Cholesky
+  //    factor and the explicit inverse.
+  hipStream_t sa = ctx->aux;
+  THC_HIP(hipEventRecord(ctx->fork, s));
+  THC_HIP(hipStreamWaitEvent(sa, ctx->fork, 0));
+  hipLaunchKernelGGL(gather_rows, dim3(grid_for((long)k * m)), dim3(256), 0, sa, aoR, n, m, idx, k, B.phim);
+  if (aoR_half)
+    hipLaunchKernelGGL(gather_rows, dim3(grid_for((long)k * m_occ)), dim3(256), 0, sa, aoR_half, n, m_occ, idx, k,
+                       B.phio);
+  THC_HIP(hipGetLastError());
+  {
+    GemmArgs g = {};
+    g.m = k; g.n = k; g.k = m;
+    g.a = B.phim; g.lda = m; g.a_trans = 0;
+    g.b = aoR; g.ldb = m; g.b_trans = 1; g.b_rows = idx; g.b_nrows = n;
+    g.c = B.c; g.ldc = k; g.c_trans = 0;
+    g.alpha = 1.0; g.beta = 0.0; g.epi_square = aoR_half ? 0 : 1; g.batch = 1;
+    THC_TRY(gemm_f64(g, sa));
+    if (aoR_half) {
+      GemmArgs h = g;
+      h.k = m_occ;
+      h.a = B.phio; h.lda = m_occ;
+      h.b = aoR_half; h.ldb = m_occ;
+      h.epi_hadamard = 1;
+      THC_TRY(gemm_f64(h, sa));
+    }
+  }
+  hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * k)), dim3(256), 0, sa, B.c, B.l, (long)k * k);
+  THC_HIP(hipGetLastError());
+  THC_TRY(cholesky_f64(B.l, k, k, B.info, B.chol_w, sa));
+  THC_TRY(spd_inverse_from_chol(B.l, k, k, B.ci, B.inv_w, sa));
+  THC_HIP(hipEventRecord(ctx->join, sa));
   // 1. Zt = (Phi Phi_mu^T)^2, N x K row-major (Zt[r][mu] = ZC^T[mu][r]): B rows
   //    gathered by idx, epilogue squares; this layout lets the solve read both
   //    operands along k and keeps the 4 GB store row-contiguous
@@ -193,12 +231,8 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
     }
   }
   THC_HIP(hipEventRecord(ev[1], s));
-  // 2. CC^T = ZC^T[:, idx]; Cholesky; explicit inverse
-  hipLaunchKernelGGL(gather_rows_t, dim3(grid_for((long)k * k)), dim3(256), 0, s, B.z, n, idx, k, B.c);
-  hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * k)), dim3(256), 0, s, B.c, B.l, (long)k * k);
-  THC_HIP(hipGetLastError());
-  THC_TRY(cholesky_f64(B.l, k, k, B.info, B.chol_w, s));
-  THC_TRY(spd_inverse_from_chol(B.l, k, k, B.ci, B.inv_w, s));
+  // join: the solve needs (CC^T)^-1
+  THC_HIP(hipStreamWaitEvent(s, ctx->join, 0));
   THC_HIP(hipEventRecord(ev[2], s));
   // 3. X = CC^T^-1 ZC^T
   {
@@ -326,8 +360,17 @@ int thc_ctx_create(int device, void* stream, thc_ctx** out) {
     hipError_t r = hipEventCreate(&e);
     if (r != hipSuccess) {
       set_last_error(hipGetErrorString(r), __FILE__, __LINE__);
+      thc_ctx_destroy(c);
       return THC_ERR_HIP;
     }
+  }
+  hipError_t r = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+  if (r == hipSuccess) r = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+  if (r == hipSuccess) r = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+  if (r != hipSuccess) {
+    set_last_error(hipGetErrorString(r), __FILE__, __LINE__);
+    thc_ctx_destroy(c);
+    return THC_ERR_HIP;
   }
   *out = c;
   return THC_OK;
@@ -345,6 +388,12 @@ int thc_ctx_destroy(thc_ctx* c) {
   c->kmws.release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->aux) {
+    (void)hipStreamSynchronize(c->aux);
+    (void)hipStreamDestroy(c->aux);
+  }
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   for (thc_ctx* l : c->lanes) thc_ctx_destroy(l);

@@ -61,6 +61,8 @@ struct thc_ctx {
   thc::Arena kmws;     // k-means state (kept across calls for graph replay)
   KmGraphCache* kmgraph = nullptr;
   hipEvent_t ev[10] = {};
+  hipStream_t aux = nullptr;      // second stream: CC^T factorisation beside the ZC^T GEMM
+  hipEvent_t fork = nullptr, join = nullptr;
   void* pinned = nullptr;  // small pinned host buffer for status read-back
   size_t pinned_bytes = 0;
   std::vector<thc_ctx*> lanes;  // extra stream contexts for batched calls (owned)
