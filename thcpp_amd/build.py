@@ -79,7 +79,9 @@ def build(force: bool = False, jobs: int = 8) -> str:
     relinked = False
     if cmds or not os.path.exists(LIB):
         tmp = LIB + ".tmp"
-        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+        # -z defs: an unresolved symbol (e.g. a kernel stub that was never
+        # emitted) fails here rather than when the library is first loaded
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp] + objs)
         os.replace(tmp, LIB)
         relinked = True
     _build_host(force or relinked, jobs)
