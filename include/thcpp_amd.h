@@ -116,6 +116,18 @@ int thc_update_centroids(thc_ctx *ctx, const double *rho, const double *grid, lo
 int thc_map_to_grid(thc_ctx *ctx, const double *grid, long n, const double *centroids,
                     int k, int *interp_idx);
 
+/* Interpolation points by QR with column pivoting of the orbital-product
+ * matrix ZT[(i k), a] = phi_i(r_a) phi_k(r_a) (i over the occupied set aoR_half
+ * when given): the first k pivot columns in ascending grid order, as
+ * QRCPSolver::kernel keeps them (src/qrcp.cpp:100-173, pzgeqpf at
+ * src/matrix_operations.h:756-788).  ZT is not formed: the pivots come from
+ * pivoted Cholesky of its Gram matrix (P o Pocc, P = Phi Phi^T), which selects
+ * the same columns in exact arithmetic.  interp_idx: k ints out (device);
+ * rdiag (host, nullable): the k pivot magnitudes |R_jj| in pivot order.
+ * Returns THC_ERR_NOT_POSDEF when k exceeds the numerical rank. */
+int thc_qrcp_points(thc_ctx *ctx, const double *aoR, const double *aoR_half, long n, int m, int m_occ, int k,
+                    int *interp_idx, double *rdiag);
+
 /* Full weighted k-means (KMeansSolver::kernel, src/kmeans.cpp:187-264) at one
  * rank: initial centroids from host_init_idx (host, k ints) or, if NULL, from
  * thc_kmeans_initial_indices(n, k, seed); then max_it Lloyd iterations that stop

@@ -124,6 +124,38 @@ def test_dropin_driver_half_rotated(tmp_path):
     assert "Dumping half transformed Muv" in r.stdout
 
 
+def test_dropin_driver_qrcp(tmp_path):
+    """interpolating_points.qrcp (src/interpolating_points.cpp:40-49,
+    src/qrcp.cpp:100-173): the points are the first thc_cfac * M QRCP pivots,
+    ascending, and the rest of the file follows from them."""
+    from oracle import qrcp_oracle as QO
+    h = host()
+    s = synthetic.make_system(reps=1, n=16, M=8, thc_cfac=3, max_it=1)
+    orb = str(tmp_path / "orbitals.h5")
+    thc = str(tmp_path / "thc.h5")
+    P = lambda a: a.ctypes.data_as(vp)
+    fg = np.array(s.n, dtype=np.int32)
+    hc = np.eye(s.M)
+    ne = np.array([4, 4], dtype=np.int32)
+    assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(s.aoR), P(fg), P(s.coulG),
+                                         P(hc), P(ne), 0.0, None, 0) == 0
+    inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac, "interpolating_points": {"qrcp": {}}}
+    (tmp_path / "input.json").write_text(json.dumps(inp))
+    r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    idx = read(h, thc, "/Hamiltonian/THC/InterpIndx", np.int32)
+    ref_idx, ref_rd, _ = QO.qrcp_points(s.aoR, s.K)
+    np.testing.assert_array_equal(idx, ref_idx)
+    luv = read(h, thc, "/Hamiltonian/THC/Luv").reshape(s.K, s.K, 2)[:, :, 0]
+    v_ref = IO.isdf(s.aoR, idx, s.n, s.coulG)["V"]
+    assert np.linalg.norm(luv @ luv.T - v_ref) / np.linalg.norm(v_ref) < 1e-9
+    diag = [l for l in r.stdout.splitlines() if l.startswith("full_ix_diag")]
+    assert len(diag) == s.K
+    np.testing.assert_allclose([float(l.split()[2]) for l in diag], ref_rd, rtol=1e-7)
+
+
 def test_dropin_missing_key_fails(tmp_path):
     (tmp_path / "input.json").write_text(json.dumps({"orbital_file": "x.h5", "output_file": "y.h5"}))
     r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,

@@ -103,6 +103,18 @@ class Context:
                 "duplicates": stats.duplicates, "interp_zeroed": bool(stats.interp_zeroed),
                 "elapsed_ms": stats.elapsed_ms}
 
+    def qrcp_points(self, aoR, k, aoR_half=None):
+        """First k QRCP pivots of the orbital-product matrix, ascending, and the
+        pivot magnitudes |R_jj| in pivot order (src/qrcp.cpp:100-173)."""
+        a = self.f64(aoR)
+        ah = self.f64(aoR_half) if aoR_half is not None else None
+        n, m = a.shape
+        idx = torch.empty(k, dtype=torch.int32, device=self.device)
+        rd = np.zeros(k)
+        st = L.lib().thc_qrcp_points(self.h, _p(a), _p(ah), n, m, ah.shape[1] if ah is not None else 0, k, _p(idx),
+                                     rd.ctypes.data_as(ctypes.c_void_p))
+        return st, idx, rd
+
     # ------------------------------------------------------------ ISDF stages
     def czt(self, aoR, idx):
         a, i = self.f64(aoR), self.i32(idx)

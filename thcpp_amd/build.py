@@ -29,7 +29,7 @@ HOST_HEADERS = ["h5io.h", "json.h", "thcpp.h"]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
               "-I" + os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["kmeans.hip", "gemm.hip", "linalg.hip", "fft.hip", "ozaki.hip", "api.hip"]
+HIP_SOURCES = ["kmeans.hip", "gemm.hip", "linalg.hip", "fft.hip", "ozaki.hip", "qrcp.hip", "api.hip"]
 CXX_SOURCES = ["seed.cpp"]
 HEADERS = ["common.h", "kmeans.h"]
 

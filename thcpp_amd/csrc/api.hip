@@ -454,6 +454,16 @@ int thc_czt(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int 
   return THC_OK;
 }
 
+int thc_qrcp_points(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m, int m_occ, int k,
+                    int* interp_idx, double* rdiag) {
+  if (!ctx || !aoR || !interp_idx || n < 1 || m < 1 || k < 1 || k > n) return THC_ERR_ARG;
+  if (aoR_half && m_occ < 1) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  THC_TRY(ctx->ws.reserve(qrcp_work_bytes(n, k)));
+  ctx->ws.reset();
+  return qrcp_points(aoR, aoR_half, n, m, aoR_half ? m_occ : 0, k, interp_idx, rdiag, ctx->ws, ctx->stream);
+}
+
 int thc_czt_half(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m, int m_occ, const int* idx,
                  int k, double* czt) {
   if (!ctx || !aoR || !aoR_half || !idx || !czt || n < 1 || m < 1 || m_occ < 1 || k < 1) return THC_ERR_ARG;

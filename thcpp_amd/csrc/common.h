@@ -131,6 +131,14 @@ int symmetrize_from_lower(double* a, int k, long lda, hipStream_t s);
 size_t oz_syrk_work_bytes(int k, long g2);
 int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s);
 
+// ---- QR with column pivoting (qrcp.hip) ---------------------------------------
+// First k pivots of QRCP of the orbital-product matrix (occ = nullptr: all
+// orbitals on both sides), sorted ascending into piv_out (device); rdiag_out
+// (host, nullable) gets the k pivot magnitudes |R_jj| in pivot order.
+size_t qrcp_work_bytes(long n, int k);
+int qrcp_points(const double* phi, const double* occ, long n, int m, int mo, int k, int* piv_out, double* rdiag_out,
+                Arena& ws, hipStream_t s);
+
 // ---- FFT (fft.hip) ----------------------------------------------------------
 size_t fft_work_bytes(int k, const int n[3], int prec);
 // x: k x n0 n1 n2 double; arithmetic and out (k x g2 weighted half spectrum) in

@@ -186,6 +186,55 @@ void KMeansSolver::kernel(ContextHandler::BlacsHandler& BH, std::vector<int>& in
 
 }  // namespace KMeans
 
+namespace QRCP {
+
+QRCPSolver::QRCPSolver(thc::Json& input, ContextHandler::BlacsHandler& BH) {
+  (void)BH;
+  input_file = input.at("orbital_file").as_string();
+  const thc::Json& ip = input.at("interpolating_points").at("qrcp");
+  if (ip.contains("sub_sample")) sub_sample = ip.at("sub_sample").as_bool();
+  else std::cout << " * Performing full QRCP decomposition." << std::endl;
+}
+
+// First thc_cfac * M pivots, ascending (src/qrcp.cpp:100-173); the pivot
+// magnitudes are printed as the reference prints the diagonal of R, for the
+// pivots that are computed.
+void QRCPSolver::kernel(ContextHandler::BlacsHandler& BH, std::vector<int>& interp_indxs, int thc_cfac,
+                        bool half_rotate) {
+  std::cout << " * Finding interpolating points via QRCP." << std::endl;
+  std::vector<double> aoR, aoR_half;
+  long n = 0;
+  int m = 0, mocc = 0;
+  {
+    thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
+    const auto d = fin.dims("aoR");
+    n = (long)d.at(0);
+    m = (int)d.at(1);
+    aoR = real_part_checked(fin.read_complex("aoR"), "aoR");
+    if (half_rotate) {
+      const auto dh = fin.dims("aoR_half");
+      mocc = (int)dh.at(1);
+      aoR_half = real_part_checked(fin.read_complex("aoR_half"), "aoR_half");
+    }
+  }
+  const int k = thc_cfac * m;
+  std::cout << " * Constructing Z matrix." << std::endl;
+  double* d_ao = to_dev(aoR);
+  DevFree fa(d_ao);
+  double* d_half = half_rotate ? to_dev(aoR_half) : nullptr;
+  DevFree fh(d_half);
+  int* d_idx = dev_alloc<int>((size_t)k);
+  DevFree fi(d_idx);
+  std::vector<double> rdiag(k);
+  status_ok(thc_qrcp_points(BH.ctx, d_ao, d_half, n, m, mocc, k, d_idx, rdiag.data()), "thc_qrcp_points");
+  interp_indxs = to_host(d_idx, (size_t)k);
+  const std::string prefix = half_rotate ? "half_ix" : "full_ix";
+  for (int i = 0; i < k; ++i)
+    std::cout << prefix + "_diag" << " " << i << " " << std::setprecision(16) << rdiag[i] << std::endl;
+}
+
+}  // namespace QRCP
+
 namespace InterpolatingPoints {
 
 IPoints::IPoints(thc::Json& input, ContextHandler::BlacsHandler& BH) {
@@ -208,7 +257,11 @@ IPoints::IPoints(thc::Json& input, ContextHandler::BlacsHandler& BH) {
 std::vector<int> IPoints::kernel(thc::Json& input, ContextHandler::BlacsHandler& BH, int thc_cfac, bool half_rotate) {
   std::vector<int> interp_indxs;
   if (solver == "qrcp") {
-    throw std::runtime_error("the QRCP interpolating-point solver is not part of this build (k-means only)");
+    std::cout << " * Using QRCP solver." << std::endl;
+    std::cout << std::endl;
+    QRCP::QRCPSolver QR(input, BH);
+    QR.kernel(BH, interp_indxs, thc_cfac, half_rotate);
+    return interp_indxs;
   }
   // the k-means solver does not depend on half_rotate (src/interpolating_points.cpp:50-58)
   (void)half_rotate;

@@ -46,6 +46,20 @@ class KMeansSolver {
 };
 }  // namespace KMeans
 
+namespace QRCP {
+// QR with column pivoting of the orbital-product matrix (src/qrcp.h, src/qrcp.cpp).
+class QRCPSolver {
+ public:
+  QRCPSolver(thc::Json& input, ContextHandler::BlacsHandler& BH);
+  void kernel(ContextHandler::BlacsHandler& BH, std::vector<int>& interp_indxs, int thc_cfac,
+              bool half_rotate = false);
+
+ private:
+  std::string input_file;
+  bool sub_sample = false;  // read for compatibility; the reference does not use it either
+};
+}  // namespace QRCP
+
 namespace InterpolatingPoints {
 class IPoints {
  public:
