@@ -197,6 +197,23 @@ int thc_factorize(thc_ctx *ctx, const double *grid, const double *rho, const dou
                   int *interp_idx, double *v, double *luv,
                   thc_kmeans_stats *kstats, thc_isdf_stats *istats);
 
+/* ---------------------------------------------------------------- complex orbitals */
+
+/* ZC^T for complex orbitals (k-point supercells; the reference's native
+ * complex128 aoR): P = Phi_mu^H Phi, ZC^T = P o conj(P) = |P|^2, which is real
+ * (IVecs::setup_pseudo_dm + setup_CZt, src/interpolating_vectors.cpp:59-141).
+ * aoR_ri: n x m complex orbitals, (re, im) interleaved (n x 2m doubles, the
+ * orbital file's compound layout); czt: k x n real out. */
+int thc_czt_complex(thc_ctx *ctx, const double *aoR_ri, long n, int m, const int *idx, int k, double *czt);
+
+/* thc_isdf for complex orbitals: ZC^T and CC^T = |P|^2 are real, so the
+ * interpolating vectors, the FFT, V and Luv are the real path's
+ * (src/interpolating_vectors.cpp:17-57, 406-455).  aoR_ri as in
+ * thc_czt_complex. */
+int thc_isdf_complex(thc_ctx *ctx, const double *aoR_ri, long n, int m, const int *idx, int k,
+                     const int fft_grid[3], const double *coulG, const thc_isdf_opts *opts,
+                     double *v, double *luv, double *xi, thc_isdf_stats *stats);
+
 /* ---------------------------------------------------------------- half-rotated THC */
 
 /* ZC^T of the half-rotated factorisation, k x n row-major:

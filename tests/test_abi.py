@@ -22,7 +22,7 @@ def declared_functions():
 
 def test_library_exports_every_declared_symbol():
     names = declared_functions()
-    assert len(names) >= 23
+    assert len(names) >= 25
     lib = L.lib()
     for n in names:
         assert hasattr(lib, n), n

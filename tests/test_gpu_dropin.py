@@ -22,7 +22,7 @@ vp = ctypes.c_void_p
 def host():
     h = ctypes.CDLL(os.path.join(PKG, "libthcpp_amd_host.so"))
     h.thc_host_write_orbital_file.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, vp, vp, vp, vp, vp, vp,
-                                              vp, ctypes.c_double, vp, ctypes.c_int]
+                                              vp, ctypes.c_double, vp, ctypes.c_int, vp]
     h.thc_host_read_f64.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.c_long]
     h.thc_host_read_f64.restype = ctypes.c_long
     h.thc_host_read_int.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.c_long]
@@ -48,7 +48,7 @@ def test_dropin_driver(tmp_path):
     hc = np.arange(s.M * s.M, dtype=np.float64).reshape(s.M, s.M)
     ne = np.array([4, 4], dtype=np.int32)
     assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(s.aoR), P(fg), P(s.coulG),
-                                         P(hc), P(ne), -2.5, None, 0) == 0
+                                         P(hc), P(ne), -2.5, None, 0, None) == 0
     inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac,
            "interpolating_points": {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}}}
     (tmp_path / "input.json").write_text(json.dumps(inp))
@@ -92,7 +92,7 @@ def test_dropin_driver_half_rotated(tmp_path):
     hc = np.eye(s.M)
     ne = np.array([4, 4], dtype=np.int32)
     assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(s.aoR), P(fg), P(s.coulG),
-                                         P(hc), P(ne), 0.0, P(occ), occ.shape[1]) == 0
+                                         P(hc), P(ne), 0.0, P(occ), occ.shape[1], None) == 0
     inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac, "half_rotated": True,
            "thc_half_cfac": half_cfac,
            "interpolating_points": {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}}}
@@ -138,7 +138,7 @@ def test_dropin_driver_qrcp(tmp_path):
     hc = np.eye(s.M)
     ne = np.array([4, 4], dtype=np.int32)
     assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(s.aoR), P(fg), P(s.coulG),
-                                         P(hc), P(ne), 0.0, None, 0) == 0
+                                         P(hc), P(ne), 0.0, None, 0, None) == 0
     inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac, "interpolating_points": {"qrcp": {}}}
     (tmp_path / "input.json").write_text(json.dumps(inp))
     r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,
@@ -154,6 +154,40 @@ def test_dropin_driver_qrcp(tmp_path):
     diag = [l for l in r.stdout.splitlines() if l.startswith("full_ix_diag")]
     assert len(diag) == s.K
     np.testing.assert_allclose([float(l.split()[2]) for l in diag], ref_rd, rtol=1e-7)
+
+
+def test_dropin_driver_complex_orbitals(tmp_path):
+    """aoR with imaginary parts (k-point orbitals, the reference's complex128
+    input): Orbitals are written complex, ZC^T = |P|^2 and V follow the oracle."""
+    h = host()
+    s = synthetic.make_system(reps=1, n=16, M=8, thc_cfac=3, max_it=5)
+    theta = np.random.default_rng(9).uniform(0, 2 * np.pi, s.M)
+    a = s.aoR * np.exp(1j * theta)[None, :]
+    re, im = np.ascontiguousarray(a.real), np.ascontiguousarray(a.imag)
+    orb = str(tmp_path / "orbitals.h5")
+    thc = str(tmp_path / "thc.h5")
+    P = lambda x: x.ctypes.data_as(vp)
+    fg = np.array(s.n, dtype=np.int32)
+    hc = np.eye(s.M)
+    ne = np.array([4, 4], dtype=np.int32)
+    assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(re), P(fg), P(s.coulG),
+                                         P(hc), P(ne), 0.0, None, 0, P(im)) == 0
+    inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac,
+           "interpolating_points": {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}}}
+    (tmp_path / "input.json").write_text(json.dumps(inp))
+    r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    idx = read(h, thc, "/Hamiltonian/THC/InterpIndx", np.int32)
+    np.testing.assert_array_equal(idx, KO.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, 0)["interp_idx"])
+    orbs = read(h, thc, "/Hamiltonian/THC/Orbitals").reshape(s.M, s.K, 2)
+    np.testing.assert_array_equal(orbs[:, :, 0], re[idx].T)
+    np.testing.assert_array_equal(orbs[:, :, 1], im[idx].T)
+    luv = read(h, thc, "/Hamiltonian/THC/Luv").reshape(s.K, s.K, 2)[:, :, 0]
+    v_ref = IO.isdf(a, idx, s.n, s.coulG)["V"].real
+    assert np.linalg.norm(luv @ luv.T - v_ref) / np.linalg.norm(v_ref) < 1e-9
+    assert "complex orbitals" in r.stdout
 
 
 def test_dropin_missing_key_fails(tmp_path):

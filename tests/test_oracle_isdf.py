@@ -169,3 +169,30 @@ def test_isdf_half_reduces_to_full_and_interpolates():
     m = half["Muv"]
     assert rel(m, m.T) < 1e-13
     assert np.linalg.eigvalsh(m).min() > -1e-10 * np.abs(m).max()
+
+
+# ---------------------------------------------------------------- complex orbitals
+
+def test_complex_orbitals_phase_invariance():
+    """ZC^T = |Phi_mu^H Phi|^2 is unchanged by a phase per grid point
+    (phi_i(r) -> e^{i theta(r)} phi_i(r)) and by a phase per orbital, so the
+    oracle's complex path must reproduce the real path's V; a conjugation slip
+    (Phi_mu^T Phi, which a per-point phase cannot expose) breaks the
+    per-orbital case."""
+    s = synthetic.make_system(reps=1, n=12, M=8, thc_cfac=2, max_it=1)
+    idx = np.sort(np.random.default_rng(1).choice(s.N, s.K, replace=False))
+    rng = np.random.default_rng(2)
+    theta = rng.uniform(0, 2 * np.pi, s.N)
+    alpha = rng.uniform(0, 2 * np.pi, s.M)
+    a_pt = s.aoR * np.exp(1j * theta)[:, None]
+    a_orb = s.aoR * np.exp(1j * alpha)[None, :]
+    z_real = IO.czt(s.aoR, idx)
+    for a in (a_pt, a_orb):
+        z = IO.czt(a, idx)
+        assert np.abs(z.imag).max() < 1e-15 * np.abs(z).max()
+        assert rel(z.real, z_real) < 1e-13
+    wrong = a_orb[idx] @ a_orb.T
+    assert rel(np.abs(wrong) ** 2, z_real) > 1e-3
+    v_real = IO.isdf(s.aoR, idx, s.n, s.coulG)["V"]
+    v_c = IO.isdf(a_pt, idx, s.n, s.coulG)["V"]
+    assert rel(v_c.real, v_real) < 1e-9

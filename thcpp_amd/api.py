@@ -122,6 +122,33 @@ class Context:
         L.check(L.lib().thc_czt(self.h, _p(a), a.shape[0], a.shape[1], _p(i), i.shape[0], _p(out)), "thc_czt")
         return out
 
+    @staticmethod
+    def _interleave(aoR):
+        """(N, M) complex -> (N, 2M) float64, (re, im) per orbital."""
+        a = np.ascontiguousarray(aoR, dtype=np.complex128)
+        return a.view(np.float64).reshape(a.shape[0], 2 * a.shape[1])
+
+    def czt_complex(self, aoR, idx):
+        a, i = self.f64(self._interleave(aoR)), self.i32(idx)
+        n, m = a.shape[0], a.shape[1] // 2
+        out = torch.empty((i.shape[0], n), dtype=torch.float64, device=self.device)
+        L.check(L.lib().thc_czt_complex(self.h, _p(a), n, m, _p(i), i.shape[0], _p(out)), "thc_czt_complex")
+        return out
+
+    def isdf_complex(self, aoR, idx, fft_grid, coulG, precision=L.PREC_FP64, want_luv=True, want_xi=False):
+        a, i, cg = self.f64(self._interleave(aoR)), self.i32(idx), self.f64(coulG).reshape(-1)
+        n, m = a.shape[0], a.shape[1] // 2
+        k = i.shape[0]
+        v = torch.empty((k, k), dtype=torch.float64, device=self.device)
+        luv = torch.empty((k, k), dtype=torch.float64, device=self.device) if want_luv else None
+        xi = torch.empty((k, n), dtype=torch.float64, device=self.device) if want_xi else None
+        opts = L.IsdfOpts(precision, 0, 1 if want_luv else 0)
+        stats = L.IsdfStats()
+        fg = (ctypes.c_int * 3)(*[int(q) for q in fft_grid])
+        st = L.lib().thc_isdf_complex(self.h, _p(a), n, m, _p(i), k, fg, _p(cg), ctypes.byref(opts), _p(v), _p(luv),
+                                      _p(xi), ctypes.byref(stats))
+        return {"status": st, "V": v, "Luv": luv, "xi": xi, "stage_ms": list(stats.stage_ms)}
+
     def czt_half(self, aoR, aoR_half, idx):
         a, ah, i = self.f64(aoR), self.f64(aoR_half), self.i32(idx)
         out = torch.empty((i.shape[0], a.shape[0]), dtype=torch.float64, device=self.device)

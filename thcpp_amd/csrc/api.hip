@@ -78,6 +78,15 @@ __global__ __launch_bounds__(256) void gather_rows(const double* a, long n, int 
   }
 }
 
+// w = i * phi for interleaved complex values (re, im) -> (-im, re)
+__global__ __launch_bounds__(256) void times_i(const double* phi, long count, double* w) {
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < count; e += 256L * gridDim.x) {
+    const double re = phi[2 * e], im = phi[2 * e + 1];
+    w[2 * e] = -im;
+    w[2 * e + 1] = re;
+  }
+}
+
 __global__ __launch_bounds__(256) void copy_f64(const double* x, double* y, long count) {
   for (long e = blockIdx.x * 256L + threadIdx.x; e < count; e += 256L * gridDim.x) y[e] = x[e];
 }
@@ -92,15 +101,18 @@ inline int grid_for(long count) { return (int)std::min<long>(8192, std::max<long
 struct IsdfBufs {
   double *z, *x, *c, *l, *ci, *r, *v;
   double *phim, *phio;  // Phi rows at the points (k x m), occupied set (k x m_occ)
+  double* w;            // i Phi for complex orbitals (n x 2m), else null
   void* b;      // weighted half spectrum (f64 or f32)
   void *chol_w, *inv_w, *fft_w, *syrk_w;
   int* info;
 };
 
-size_t isdf_bytes(long n, int k, int m, int m_occ, const int fft[3], int prec, int refine, bool own_x) {
+// m is the real row width of Phi (2 x orbitals for interleaved complex ones)
+size_t isdf_bytes(long n, int k, int m, int m_occ, bool cplx, const int fft[3], int prec, int refine, bool own_x) {
   const long g2 = half_spectrum_len(fft);
   const size_t kk = (size_t)k * k;
   size_t b = carve_bytes<double>((size_t)k * m) + carve_bytes<double>((size_t)k * std::max(m_occ, 1));
+  if (cplx) b += carve_bytes<double>((size_t)n * m);
   b += carve_bytes<double>((size_t)k * n);                   // z
   if (own_x) b += carve_bytes<double>((size_t)k * n);        // x
   b += 4 * carve_bytes<double>(kk);                          // c, l, ci, v
@@ -113,13 +125,15 @@ size_t isdf_bytes(long n, int k, int m, int m_occ, const int fft[3], int prec, i
   return b;
 }
 
-bool isdf_carve(Arena& a, long n, int k, int m, int m_occ, const int fft[3], int prec, int refine, double* xi,
-                IsdfBufs& B) {
+bool isdf_carve(Arena& a, long n, int k, int m, int m_occ, bool cplx, const int fft[3], int prec, int refine,
+                double* xi, IsdfBufs& B) {
   const long g2 = half_spectrum_len(fft);
   const size_t kk = (size_t)k * k;
   B = IsdfBufs{};
   B.phim = a.take<double>((size_t)k * m);
   B.phio = a.take<double>((size_t)k * std::max(m_occ, 1));
+  B.w = cplx ? a.take<double>((size_t)n * m) : nullptr;
+  if (cplx && !B.w) return false;
   B.z = a.take<double>((size_t)k * n);
   B.x = xi ? xi : a.take<double>((size_t)k * n);
   B.c = a.take<double>(kk);
@@ -147,11 +161,16 @@ __global__ void zero_ints(int* p, int count) {
 
 // IVecs ctor + kernel at one rank.
 // aoR_half (n x m_occ, nullable) selects the half-rotated ZC^T (Pocc o P).
-int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m, int m_occ, const int* idx,
-             int k, const int fft[3], const double* coulG, const thc_isdf_opts* opts, double* v_out,
-             double* luv_out, double* xi_out, thc_isdf_stats* stats) {
-  if (!ctx || !aoR || !idx || !fft || !coulG || !v_out || n < 1 || m < 1 || k < 1 || k > n) return THC_ERR_ARG;
+// cplx: aoR holds m complex orbitals interleaved (n x 2m doubles); then
+// ZC^T = |P|^2 = (Re P)^2 + (Im P)^2 with Re P from Phi against Phi and Im P
+// from Phi against i Phi, both real GEMMs over the 2m interleaved columns.
+int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m_orb, int m_occ, bool cplx,
+             const int* idx, int k, const int fft[3], const double* coulG, const thc_isdf_opts* opts,
+             double* v_out, double* luv_out, double* xi_out, thc_isdf_stats* stats) {
+  if (!ctx || !aoR || !idx || !fft || !coulG || !v_out || n < 1 || m_orb < 1 || k < 1 || k > n) return THC_ERR_ARG;
   if (aoR_half && m_occ < 1) return THC_ERR_ARG;
+  if (aoR_half && cplx) return THC_ERR_UNSUPPORTED;  // Pocc o conj(P) is complex there
+  const int m = cplx ? 2 * m_orb : m_orb;           // real row width of Phi
   const int prec = opts ? opts->precision : THC_PREC_FP64;
   const int refine = opts ? std::max(0, opts->refine_steps) : 0;
   const int want_luv = opts ? opts->want_luv : 1;
@@ -163,10 +182,10 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   if (nfft != n) return THC_ERR_ARG;
   THC_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  THC_TRY(ctx->ws.reserve(isdf_bytes(n, k, m, m_occ, fft, prec, refine, xi_out == nullptr)));
+  THC_TRY(ctx->ws.reserve(isdf_bytes(n, k, m, m_occ, cplx, fft, prec, refine, xi_out == nullptr)));
   ctx->ws.reset();
   IsdfBufs B;
-  if (!isdf_carve(ctx->ws, n, k, m, m_occ, fft, prec, refine, xi_out, B)) return THC_ERR_NO_MEMORY;
+  if (!isdf_carve(ctx->ws, n, k, m, m_occ, cplx, fft, prec, refine, xi_out, B)) return THC_ERR_NO_MEMORY;
   const long g2 = half_spectrum_len(fft);
   hipEvent_t* ev = ctx->ev;
 
@@ -179,6 +198,10 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   //    equals the gathered columns of ZC^T bit for bit.  Then the Cholesky
   //    factor and the explicit inverse.
   hipStream_t sa = ctx->aux;
+  if (cplx) {
+    hipLaunchKernelGGL(times_i, dim3(grid_for((long)n * m_orb)), dim3(256), 0, s, aoR, (long)n * m_orb, B.w);
+    THC_HIP(hipGetLastError());
+  }
   THC_HIP(hipEventRecord(ctx->fork, s));
   THC_HIP(hipStreamWaitEvent(sa, ctx->fork, 0));
   hipLaunchKernelGGL(gather_rows, dim3(grid_for((long)k * m)), dim3(256), 0, sa, aoR, n, m, idx, k, B.phim);
@@ -200,6 +223,12 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
       h.a = B.phio; h.lda = m_occ;
       h.b = aoR_half; h.ldb = m_occ;
       h.epi_hadamard = 1;
+      THC_TRY(gemm_f64(h, sa));
+    }
+    if (cplx) {
+      GemmArgs h = g;
+      h.b = B.w;
+      h.epi_square = 0; h.epi_addsquare = 1;
       THC_TRY(gemm_f64(h, sa));
     }
   }
@@ -227,6 +256,12 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
       h.a = aoR_half; h.lda = m_occ;
       h.b = aoR_half; h.ldb = m_occ;
       h.epi_hadamard = 1;
+      THC_TRY(gemm_f64(h, s));
+    }
+    if (cplx) {  // Zt += (Im P)^2
+      GemmArgs h = g;
+      h.b = B.w;
+      h.epi_square = 0; h.epi_addsquare = 1;
       THC_TRY(gemm_f64(h, s));
     }
   }
@@ -454,6 +489,36 @@ int thc_czt(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int 
   return THC_OK;
 }
 
+int thc_czt_complex(thc_ctx* ctx, const double* aoR_ri, long n, int m, const int* idx, int k, double* czt) {
+  if (!ctx || !aoR_ri || !idx || !czt || n < 1 || m < 1 || k < 1) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  THC_TRY(ctx->ws.reserve(carve_bytes<double>((size_t)n * 2 * m)));
+  ctx->ws.reset();
+  double* w = ctx->ws.take<double>((size_t)n * 2 * m);
+  if (!w) return THC_ERR_NO_MEMORY;
+  hipLaunchKernelGGL(times_i, dim3(grid_for((long)n * m)), dim3(256), 0, ctx->stream, aoR_ri, (long)n * m, w);
+  THC_HIP(hipGetLastError());
+  GemmArgs g = {};
+  g.m = k; g.n = n; g.k = 2 * m;
+  g.a = aoR_ri; g.lda = 2 * m; g.a_trans = 0; g.a_rows = idx; g.a_nrows = n;
+  g.b = aoR_ri; g.ldb = 2 * m; g.b_trans = 1;
+  g.c = czt; g.ldc = n;
+  g.alpha = 1.0; g.beta = 0.0; g.epi_square = 1; g.batch = 1;
+  THC_TRY(gemm_f64(g, ctx->stream));
+  GemmArgs h = g;
+  h.b = w;
+  h.epi_square = 0; h.epi_addsquare = 1;
+  THC_TRY(gemm_f64(h, ctx->stream));
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  return THC_OK;
+}
+
+int thc_isdf_complex(thc_ctx* ctx, const double* aoR_ri, long n, int m, const int* idx, int k,
+                     const int fft_grid[3], const double* coulG, const thc_isdf_opts* opts, double* v, double* luv,
+                     double* xi, thc_isdf_stats* stats) {
+  return isdf_run(ctx, aoR_ri, nullptr, n, m, 0, true, idx, k, fft_grid, coulG, opts, v, luv, xi, stats);
+}
+
 int thc_qrcp_points(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m, int m_occ, int k,
                     int* interp_idx, double* rdiag) {
   if (!ctx || !aoR || !interp_idx || n < 1 || m < 1 || k < 1 || k > n) return THC_ERR_ARG;
@@ -582,14 +647,14 @@ int thc_construct_muv(thc_ctx* ctx, const double* b, int k, long g2, double* v) 
 int thc_isdf(thc_ctx* ctx, const double* aoR, long n, int m, const int* idx, int k, const int fft_grid[3],
              const double* coulG, const thc_isdf_opts* opts, double* v, double* luv, double* xi,
              thc_isdf_stats* stats) {
-  return isdf_run(ctx, aoR, nullptr, n, m, 0, idx, k, fft_grid, coulG, opts, v, luv, xi, stats);
+  return isdf_run(ctx, aoR, nullptr, n, m, 0, false, idx, k, fft_grid, coulG, opts, v, luv, xi, stats);
 }
 
 int thc_isdf_half(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m, int m_occ,
                   const int* idx, int k, const int fft_grid[3], const double* coulG, const thc_isdf_opts* opts,
                   double* muv, double* xi, thc_isdf_stats* stats) {
   if (!aoR_half) return THC_ERR_ARG;
-  return isdf_run(ctx, aoR, aoR_half, n, m, m_occ, idx, k, fft_grid, coulG, opts, muv, nullptr, xi, stats);
+  return isdf_run(ctx, aoR, aoR_half, n, m, m_occ, false, idx, k, fft_grid, coulG, opts, muv, nullptr, xi, stats);
 }
 
 int thc_factorize(thc_ctx* ctx, const double* grid, const double* rho, const double* aoR, long n, int m,
@@ -602,8 +667,8 @@ int thc_factorize(thc_ctx* ctx, const double* grid, const double* rho, const dou
   const int st = thc_kmeans(ctx, grid, rho, n, (int)k, max_it, threshold, seed, nullptr, interp_idx, nullptr,
                             nullptr, kstats);
   if (st != THC_OK && st < 100) return st;
-  const int st2 = isdf_run(ctx, aoR, nullptr, n, m, 0, interp_idx, (int)k, fft_grid, coulG, opts, v, luv, nullptr,
-                           istats);
+  const int st2 = isdf_run(ctx, aoR, nullptr, n, m, 0, false, interp_idx, (int)k, fft_grid, coulG, opts, v, luv,
+                           nullptr, istats);
   if (st2 != THC_OK) return st2;
   return st;
 }

@@ -89,6 +89,7 @@ int ensure_pinned(thc_ctx* ctx, size_t bytes);
 //   epi_square: C = (alpha * acc)^2 (beta ignored)
 //   epi_hadamard: C = (alpha * acc) * C elementwise (beta ignored; gathered
 //     operand, both operands k-contiguous)
+//   epi_addsquare: C = C + (alpha * acc)^2 (same layouts as epi_hadamard)
 //   lower: only tiles on/below the diagonal are computed (m, n square)
 //   batch: blockIdx.z, strides in elements
 struct GemmArgs {
@@ -102,6 +103,7 @@ struct GemmArgs {
   double alpha, beta;
   int epi_square;
   int epi_hadamard;
+  int epi_addsquare;
   int lower;
   int batch;
   long stride_a, stride_b, stride_c;

@@ -44,7 +44,7 @@ template <> struct Mfma<float> {
   __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-enum { EPI_STORE = 0, EPI_SQUARE = 1, EPI_PARTIAL = 2, EPI_HADAMARD = 3 };
+enum { EPI_STORE = 0, EPI_SQUARE = 1, EPI_PARTIAL = 2, EPI_HADAMARD = 3, EPI_ADDSQUARE = 4 };
 
 struct KArgs {
   long m, n, k;
@@ -332,6 +332,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs p) {
           } else if (EPI == EPI_HADAMARD) {
             const T x = alpha * acc[i][j][v];
             *dst = x * *dst;
+          } else if (EPI == EPI_ADDSQUARE) {
+            const T x = alpha * acc[i][j][v];
+            *dst = *dst + x * x;
           } else {
             T x = alpha * acc[i][j][v];
             if (beta != T(0)) x += beta * *dst;
@@ -428,14 +431,21 @@ int launch(const GemmArgs& g, hipStream_t s) {
   if (nblk > 0x7fffffffL) return THC_ERR_ARG;
   if ((g.a_rows || g.b_rows) && g.lower) return THC_ERR_ARG;
   dim3 grid((unsigned)nblk, 1, (unsigned)g.batch);
-  if (g.epi_hadamard) {
+  if (g.epi_hadamard || g.epi_addsquare) {
     // only the gathered, k-contiguous layouts of the orbital products
-    if (g.epi_square || g.a_trans != 0 || g.b_trans != 1 || (!g.a_rows == !g.b_rows)) return THC_ERR_ARG;
+    if (g.epi_square || (g.epi_hadamard && g.epi_addsquare) || g.a_trans != 0 || g.b_trans != 1 ||
+        (!g.a_rows == !g.b_rows))
+      return THC_ERR_ARG;
     const int vw = 16 / (int)sizeof(T);
     const bool vec = aligned16(g.a) && aligned16(g.b) && g.lda % vw == 0 && g.ldb % vw == 0 &&
                      g.stride_a % vw == 0 && g.stride_b % vw == 0;
-    if (g.a_rows) launch_vec<T, C, EPI_HADAMARD, true, true, true, false>(p, vec, grid, s);
-    else launch_vec<T, C, EPI_HADAMARD, true, true, false, true>(p, vec, grid, s);
+    if (g.epi_hadamard) {
+      if (g.a_rows) launch_vec<T, C, EPI_HADAMARD, true, true, true, false>(p, vec, grid, s);
+      else launch_vec<T, C, EPI_HADAMARD, true, true, false, true>(p, vec, grid, s);
+    } else {
+      if (g.a_rows) launch_vec<T, C, EPI_ADDSQUARE, true, true, true, false>(p, vec, grid, s);
+      else launch_vec<T, C, EPI_ADDSQUARE, true, true, false, true>(p, vec, grid, s);
+    }
     THC_HIP(hipGetLastError());
     return THC_OK;
   }

@@ -8,18 +8,22 @@
 
 #include "h5io.h"
 
-// aoR_half (n x m_occ, nullable) adds the occupied orbitals the half-rotated path reads.
+// aoR_half (n x m_occ, nullable) adds the occupied orbitals the half-rotated path reads;
+// aoR_im (n x m, nullable) the imaginary parts of complex orbitals.
 extern "C" int thc_host_write_orbital_file(const char* path, long n, int m, const double* grid, const double* rho,
                                            const double* aoR, const int* fft_grid, const double* coulG,
                                            const double* hcore, const int* nelec, double e0,
-                                           const double* aoR_half, int m_occ) {
+                                           const double* aoR_half, int m_occ, const double* aoR_im) {
   try {
     thc::h5::File f(path, thc::h5::File::Truncate);
     const uint64_t N = n, M = m;
     f.write_f64("real_space_grid", grid, {N, 3});
     f.write_f64("density", rho, {N, 1});
     std::vector<double> z(2 * N * M, 0.0);
-    for (uint64_t i = 0; i < N * M; ++i) z[2 * i] = aoR[i];
+    for (uint64_t i = 0; i < N * M; ++i) {
+      z[2 * i] = aoR[i];
+      if (aoR_im) z[2 * i + 1] = aoR_im[i];
+    }
     f.write_compound_complex("aoR", z.data(), {N, M});
     if (aoR_half && m_occ > 0) {
       const uint64_t MO = m_occ;

@@ -56,6 +56,15 @@ void status_ok(int st, const char* where) {
 }
 
 // Real orbitals from the complex aoR the reference reads (src/h5helper.cpp:19-26).
+bool has_imag(const std::vector<double>& z) {
+  double im = 0.0, re = 0.0;
+  for (size_t i = 0; i + 1 < z.size(); i += 2) {
+    re = std::max(re, std::fabs(z[i]));
+    im = std::max(im, std::fabs(z[i + 1]));
+  }
+  return im > 1e-12 * std::max(1.0, re);
+}
+
 std::vector<double> real_part_checked(const std::vector<double>& z, const std::string& name) {
   std::vector<double> r(z.size() / 2);
   double im = 0.0, re = 0.0;
@@ -65,7 +74,8 @@ std::vector<double> real_part_checked(const std::vector<double>& z, const std::s
     im = std::max(im, std::fabs(z[2 * i + 1]));
   }
   if (im > 1e-12 * std::max(1.0, re))
-    throw std::runtime_error(name + " has imaginary parts; complex orbitals are not supported by this build");
+    throw std::runtime_error(name + " has imaginary parts; this path takes real orbitals only (complex orbitals "
+                             "are supported by the full, non-half-rotated k-means path)");
   return r;
 }
 
@@ -309,11 +319,19 @@ IVecs::IVecs(thc::Json& input, ContextHandler::BlacsHandler& BH, std::vector<int
 
 // <prfx>Orbitals (M, K) and <prfx>InterpIndx (src/interpolating_vectors.cpp:72-90)
 void IVecs::write_orbitals_at_points(const std::vector<double>& ao, int m, const std::vector<int>& interp_indxs,
-                                     const std::string& prfx) {
+                                     const std::string& prfx, bool interleaved) {
   const int k = (int)interp_indxs.size();
   std::vector<double> orb(2 * (size_t)m * k, 0.0);
   for (int i = 0; i < m; ++i)
-    for (int mu = 0; mu < k; ++mu) orb[2 * ((size_t)i * k + mu)] = ao[(size_t)interp_indxs[mu] * m + i];
+    for (int mu = 0; mu < k; ++mu) {
+      const size_t o = 2 * ((size_t)i * k + mu), src = (size_t)interp_indxs[mu] * m + i;
+      if (interleaved) {
+        orb[o] = ao[2 * src];
+        orb[o + 1] = ao[2 * src + 1];
+      } else {
+        orb[o] = ao[src];
+      }
+    }
   thc::h5::File fo(output_file, thc::h5::File::ReadWrite);
   fo.ensure_group("/Hamiltonian/THC");
   std::cout << " * Writing aoR_mu to file" << std::endl;
@@ -357,22 +375,28 @@ IVecs::~IVecs() {
   if (d_idx) (void)hipFree(d_idx);
 }
 
+// ZC^T = P o conj(P) (src/interpolating_vectors.cpp:127-141); complex orbitals
+// (any non-zero imaginary part) take the complex product, which is still real.
 void IVecs::setup_CZt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH) {
-  std::vector<double> aoR;
+  std::vector<double> z;
   {
     thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
-    aoR = real_part_checked(fin.read_complex("aoR"), "aoR");
+    z = fin.read_complex("aoR");
   }
+  const bool cplx = has_imag(z);
   const int k = nmu, m = nbasis;
   const long n = ngrid;
-  write_orbitals_at_points(aoR, m, interp_indxs, prefix);
-  std::cout << " * Constructing CZt matrix" << std::endl;
+  std::vector<double> aoR;
+  if (!cplx) aoR = real_part_checked(z, "aoR");
+  write_orbitals_at_points(cplx ? z : aoR, m, interp_indxs, prefix, cplx);
+  std::cout << " * Constructing CZt matrix" << (cplx ? " (complex orbitals)" : "") << std::endl;
   std::cout << " * Matrix Shape: (" << k << ", " << n << ")" << std::endl;
-  double* d_ao = to_dev(aoR);
+  double* d_ao = to_dev(cplx ? z : aoR);
   DevFree fa(d_ao);
   d_idx = to_dev(interp_indxs);
   d_czt = dev_alloc<double>((size_t)k * n);
-  status_ok(thc_czt(BH.ctx, d_ao, n, m, d_idx, k, d_czt), "thc_czt");
+  if (cplx) status_ok(thc_czt_complex(BH.ctx, d_ao, n, m, d_idx, k, d_czt), "thc_czt_complex");
+  else status_ok(thc_czt(BH.ctx, d_ao, n, m, d_idx, k, d_czt), "thc_czt");
 }
 
 void IVecs::setup_CCt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH) {

@@ -85,7 +85,7 @@ class IVecs {
   void setup_CZt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH);
   void setup_CZt_half(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH);
   void write_orbitals_at_points(const std::vector<double>& ao, int m, const std::vector<int>& interp_indxs,
-                                const std::string& prfx);
+                                const std::string& prfx, bool interleaved = false);
   void setup_CCt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH);
   void check_rank(ContextHandler::BlacsHandler& BH);
   std::string input_file, output_file;
