@@ -152,7 +152,10 @@ def main():
     from thcpp_amd import synthetic
     from thcpp_amd.api import Context
 
+    t0 = time.perf_counter()
     s = synthetic.make_config(a.config)
+    print("bench: %s generated in %.1f s (N=%d, M=%d, N_mu=%d)" % (a.config, time.perf_counter() - t0, s.N, s.M, s.K),
+          file=sys.stderr, flush=True)
     ctx = Context(0)
     g, r, ao, cg = ctx.f64(s.grid), ctx.f64(s.rho), ctx.f64(s.aoR), ctx.f64(s.coulG)
 
@@ -175,6 +178,7 @@ def main():
             print("factorisation status %d" % out["status"], file=sys.stderr)
             return 1
         results[name] = {"ms": dt * 1e3, "stages": np.mean(np.array(stages), axis=0), "V": out["V"].double().cpu()}
+        print("bench: %s path %.3f ms per factorisation" % (name, dt * 1e3), file=sys.stderr, flush=True)
     v64, vmx = results["fp64"]["V"], results["mixed"]["V"]
     relv = float(torch.linalg.norm(vmx - v64) / torch.linalg.norm(v64))
     mx = results["mixed"]
