@@ -34,7 +34,7 @@ def test_czt_cct(ctx, n, m, k):
     np.testing.assert_array_equal(c, z.cpu().numpy()[:, idx])
 
 
-@pytest.mark.parametrize("k", [1, 64, 200, 777, 1100])
+@pytest.mark.parametrize("k", [1, 63, 64, 65, 128, 200, 777, 1100, 2048, 2113])
 def test_cholesky(ctx, k):
     rng = np.random.default_rng(k)
     b = rng.standard_normal((k, k + 10))
@@ -46,11 +46,27 @@ def test_cholesky(ctx, k):
     assert not np.triu(l.cpu().numpy(), 1).any()
 
 
-def test_cholesky_not_posdef(ctx):
-    a = np.eye(100)
-    a[70, 70] = -1.0
+def test_cholesky_ill_conditioned(ctx):
+    """Eigenvalues spread over 1e-11 .. 1: the backward error L L^T - A stays
+    at rounding level whatever the conditioning."""
+    k = 700
+    rng = np.random.default_rng(9)
+    q, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    a = (q * np.logspace(-11, 0, k)) @ q.T
+    a = 0.5 * (a + a.T)
+    l, info = ctx.cholesky(a)
+    assert info == 0
+    l = l.cpu().numpy()
+    assert np.linalg.norm(l @ l.T - a) / np.linalg.norm(a) < 1e-14
+
+
+@pytest.mark.parametrize("k,bad", [(100, 70), (64, 63), (300, 0), (300, 5), (300, 200), (257, 256)])
+def test_cholesky_not_posdef(ctx, k, bad):
+    """info is the order of the first non-positive leading minor (LAPACK potrf)."""
+    a = np.eye(k)
+    a[bad, bad] = -1.0
     _, info = ctx.cholesky(a)
-    assert info == 71
+    assert info == bad + 1
 
 
 @pytest.mark.parametrize("k,n", [(100, 5000), (513, 20000)])

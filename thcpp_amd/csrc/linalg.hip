@@ -3,11 +3,9 @@
 // src/matrix_operations.h:259-267) and the SPD inverse used to apply
 // CC^T^-1 to ZC^T as one MFMA GEMM.
 //
-// Cholesky: right-looking, 64-column panels.  Per panel: one wave factors the
-// 64 x 64 diagonal block (row per lane, in registers); the panel below is
-// solved against it row by row (L21 = A21 L11^-T, one row per thread); the
-// trailing lower triangle takes the rank-64 update (MFMA GEMM, lower tiles
-// only).
+// Cholesky: right-looking, 64-column panels, one fused kernel per panel
+// (panel solve by the inverse of the diagonal block, trailing update, and the
+// next diagonal block's factor and inverse); layout below.
 // Inverse: the lower factor is padded with an identity to 64 * 2^p, its
 // 64 x 64 diagonal blocks are inverted in parallel, then recursive doubling
 // forms Linv21 = -Linv22 (L21 Linv11) for every pair of blocks of one size in
@@ -50,20 +48,6 @@ __device__ inline void load_lower_block(const double* a, long lda, int i0, int j
   }
 }
 
-// Rows r0 .. r0 + NB - 1 (below 'rows': zero) of the rows x jb panel p (row
-// stride lda) into Xs, columns >= jb zero; batched loads as above.
-__device__ inline void load_panel(const double* p, long lda, long r0, long rows, int jb, double (*Xs)[NB + 1]) {
-  const int t = threadIdx.x;
-  double v[NB];
-#pragma unroll
-  for (int rr = 0; rr < NB; ++rr) {
-    const bool ok = r0 + rr < rows && t < jb;
-    v[rr] = p[ok ? (r0 + rr) * lda + t : 0];
-  }
-#pragma unroll
-  for (int rr = 0; rr < NB; ++rr) Xs[rr][t] = (r0 + rr < rows && t < jb) ? v[rr] : 0.0;
-}
-
 // Forward substitution for one row held in registers: x L^T = b, with L given
 // transposed (Lt[c][c2] = L[c2][c], a broadcast read for the whole wave).
 // Right-looking: once x[c] is final every later entry drops its x[c] term, so
@@ -77,6 +61,28 @@ __device__ inline void solve_row(double (&x)[NB], const double (*Lt)[NB + 1]) {
   }
 }
 
+// ---- fused right-looking Cholesky ------------------------------------------
+// The matrix is taken in 64 x 64 blocks, extended by the identity beyond k.
+// Step j (one kernel) owns panel j: its diagonal block was factored by step
+// j - 1, which also left W_j = L_jj^-1.  Every workgroup takes one lower tile
+// (I, J) of the trailing matrix (I >= J > j) and
+//   1. forms L_I = A_Ij W_j^T and L_J = A_Jj W_j^T (MFMA, from the unsolved
+//      panel rows, so the panel itself is never rewritten during the step);
+//   2. applies A_IJ -= L_I L_J^T (MFMA);
+//   3. for I = J, parks L_I transposed in the unused upper block (j, I);
+//   4. tile (j+1, j+1) -- workgroup 0 -- then factors its block: wave 0 runs
+//      the pivot steps with lane r owning row r in registers, and wave 1,
+//      in lockstep one barrier behind each pivot, forms W_{j+1} column by
+//      column (lane c solves L x = e_c, right-looking).
+// A last kernel moves the parked panels into the lower triangle and zeroes
+// the upper one.  Step -1 only factors block 0.
+constexpr int CB = NB;      // block size
+constexpr int CT = 4 * 64;  // threads of a step workgroup
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ inline d4 mfma_f64(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
 __device__ inline double readlane_d(double v, int lane) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, lane);
@@ -84,110 +90,236 @@ __device__ inline double readlane_d(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Factor the jb x jb diagonal block at (j0, j0) in place (lower part only).
-// Right-looking, lane r owning row r in registers: at step k the pivot
-// L[k][k] = sqrt(A[k][k]) is read from lane k, column k is scaled by its
-// reciprocal and published in LDS as row k of Lt, and every lane drops
-// L[r][k] L[c][k] from its entries c > k (broadcast reads of Lt[k][c]).
-// Rows/columns >= jb are an identity extension.  Also writes lt (NB x NB,
-// lt[c][r] = L[r][c], zero above the diagonal of L) and rinv[c] = 1 / L[c][c]
-// for the panel solve.  info: first failing leading minor (1-based, global),
-// kept if already set.  (A rolled form with the rows in LDS is smaller code
-// but 2.3x slower: its read-modify-write chains are LDS-latency bound.)
-__global__ __launch_bounds__(64) void potrf_diag(double* a, long lda, int j0, int jb, int* info, double* lt,
-                                                 double* rinv) {
-  __shared__ double Ls[NB][NB + 1];
-  __shared__ __attribute__((aligned(16))) double Lt[NB][NB];
-  const int r = threadIdx.x;
-  load_lower_block<false>(a, lda, j0, j0, jb, Ls);
-  wave_sync_lds();
-  double row[NB];
-#pragma unroll
-  for (int c = 0; c < NB; ++c) row[c] = Ls[r][c];
-  int bad = 0;
-  double my_rp = 1.0;
-#pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    const double d = readlane_d(row[k], k);
-    if (!(d > 0.0) && bad == 0 && k < jb) bad = k + 1;
-    const double piv = sqrt(d);
-    const double rp = 1.0 / piv;
-    if (r == k) my_rp = rp;
-    const double v = (r == k) ? piv : row[k] * rp;
-    row[k] = v;
-    Lt[k][r] = v;
-    wave_sync_lds();
-#pragma unroll
-    for (int c = k + 1; c < NB; ++c) row[c] = fma(-v, Lt[k][c], row[c]);
-  }
-  if (r == 0 && bad && *info == 0) *info = j0 + bad;
-  rinv[r] = my_rp;
-#pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    lt[c * NB + r] = (c <= r) ? row[c] : 0.0;
-    Ls[r][c] = row[c];
-  }
-  wave_sync_lds();
-  for (int rr = 0; rr < jb; ++rr)
-    if (r <= rr) a[(long)(j0 + rr) * lda + j0 + r] = Ls[rr][r];
+// sqrt(d) and 1/sqrt(d) from the hardware reciprocal square root, two
+// coupled Newton steps (g -> sqrt, h -> 1/(2 sqrt)) and a final residual
+// correction of g; non-positive or NaN d give NaN or inf, as sqrt would.
+__device__ inline void sqrt_rsqrt(double d, double& s, double& rs) {
+  const double y = __builtin_amdgcn_rsq(d);
+  double g = d * y, h = 0.5 * y;
+  double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  s = fma(fma(-g, g, d), h, g);
+  rs = 2.0 * h;
 }
 
-// L21 = A21 L11^-T in place for the 'rows' rows below the diagonal block at
-// (j0, j0): one row per thread (x L11^T = a, forward substitution,
-// right-looking, each x[c] scaled by the stored reciprocal of L[c][c]), 64 rows
-// per workgroup.  L11^T and the reciprocals come from potrf_diag's output.
-__global__ __launch_bounds__(64) void trsm_panel(double* a, long lda, int j0, int jb, long rows,
-                                                 const double* __restrict__ lt, const double* __restrict__ rinv) {
-  __shared__ double Lt[NB][NB + 1];
-  __shared__ double Xs[NB][NB + 1];
-  __shared__ double Ri[NB];
-  const int t = threadIdx.x;
-  const long r0 = (long)blockIdx.x * NB;
-  double* p = a + (long)(j0 + jb) * lda + j0;
-  {
-    double v[NB];
+// a[r][c] of the matrix extended by the identity beyond k
+__device__ inline double ext_at(const double* a, long lda, int k, long r, long c) {
+  const bool in = r < k && c < k;
+  const double v = a[in ? r * lda + c : 0];
+  return in ? v : (r == c ? 1.0 : 0.0);
+}
+
+// Block at (r0, c0) into X; each wave reads whole 64-element rows, all loads
+// issued before the first LDS store.
+__device__ inline void load_tile(const double* a, long lda, int k, long r0, long c0, double (*X)[CB + 1]) {
+  const int c = threadIdx.x & 63, rb = threadIdx.x >> 6;
+  double v[CB / 4];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) v[c] = lt[c * NB + t];
-    const double ri = rinv[t];
-    load_panel(p, lda, r0, rows, jb, Xs);
+  for (int i = 0; i < CB / 4; ++i) v[i] = ext_at(a, lda, k, r0 + rb + 4 * i, c0 + c);
 #pragma unroll
-    for (int c = 0; c < NB; ++c) Lt[c][t] = v[c];
-    Ri[t] = ri;
-  }
-  wave_sync_lds();
-  double x[NB];
+  for (int i = 0; i < CB / 4; ++i) X[rb + 4 * i][c] = v[i];
+}
+
+// Rows 16 w .. 16 w + 15 of X become X W^T (wave w; only its own rows are
+// read and then rewritten).  f64 MFMA D layout: lane l holds column l % 16 of
+// rows l / 16 + 4 v.
+__device__ inline void panel_rows(double (*X)[CB + 1], const double (*W)[CB + 1]) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  d4 acc[4] = {};
 #pragma unroll
-  for (int c = 0; c < NB; ++c) x[c] = Xs[t][c];
-  // col holds step c's factors: col[c] = 1 / L[c][c], col[c2] = L[c2][c] (c2 > c).
-  // The next step's column is read before this step's FMAs, behind a compiler
-  // fence, so the LDS latency overlaps the arithmetic.
-  double col[NB];
-  col[0] = Ri[0];
+  for (int q0 = 0; q0 < CB; q0 += 4) {
+    const double av = X[16 * w + (l & 15)][q0 + (l >> 4)];
 #pragma unroll
-  for (int c2 = 1; c2 < NB; ++c2) col[c2] = Lt[0][c2];
-#pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    double nxt[NB];
-    if (c + 1 < NB) {
-      nxt[c + 1] = Ri[c + 1];
-#pragma unroll
-      for (int c2 = c + 2; c2 < NB; ++c2) nxt[c2] = Lt[c + 1][c2];
-    }
-    __asm__ volatile("" ::: "memory");
-    x[c] *= col[c];
-#pragma unroll
-    for (int c2 = c + 1; c2 < NB; ++c2) x[c2] = fma(-x[c], col[c2], x[c2]);
-    if (c + 1 < NB) {
-#pragma unroll
-      for (int c2 = c + 1; c2 < NB; ++c2) col[c2] = nxt[c2];
-    }
+    for (int cb = 0; cb < 4; ++cb) acc[cb] = mfma_f64(av, W[16 * cb + (l & 15)][q0 + (l >> 4)], acc[cb]);
   }
 #pragma unroll
-  for (int c = 0; c < NB; ++c) Xs[t][c] = x[c];
-  wave_sync_lds();
-  for (int rr = 0; rr < NB; ++rr) {
-    const long pr = r0 + rr;
-    if (pr < rows && t < jb) p[pr * lda + t] = Xs[rr][t];
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) X[16 * w + (l >> 4) + 4 * v][16 * cb + (l & 15)] = acc[cb][v];
+}
+
+// Factor the block in C (LDS, lower part read) at global rows/cols base ..
+// base + 63 (jb of them inside the matrix): L (lower) to adiag, W = L^-1 to
+// w_out (row-major 64 x 64, zero above the diagonal) when need_w.  Lt (row kk
+// = column kk of L) and Rs (1 / L[kk][kk]) are LDS scratch.  All four waves
+// call this (the barriers are workgroup-wide).  info: first non-positive
+// pivot (1-based, global), kept if already set.
+__device__ void factor_block(double (*C)[CB + 1], double (*Lt)[CB + 1], double* Rs, bool need_w, double* adiag,
+                             long lda, int jb, double* w_out, int* info, int base) {
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  double x[CB];
+  int bad = 0;
+  double v = 0.0;
+  if (w == 0) {
+#pragma unroll
+    for (int c = 0; c < CB; ++c) x[c] = C[l][c];
+    const double d = readlane_d(x[0], 0);
+    if (!(d > 0.0)) bad = 1;
+    double s, rs;
+    sqrt_rsqrt(d, s, rs);
+    v = (l == 0) ? s : x[0] * rs;
+    x[0] = v;
+    Lt[0][l] = v;
+    if (l == 0) Rs[0] = rs;
+  } else if (w == 1) {
+#pragma unroll
+    for (int c = 0; c < CB; ++c) x[c] = (c == l) ? 1.0 : 0.0;
+  }
+#pragma unroll
+  for (int kk = 0; kk < CB; ++kk) {
+    __syncthreads();
+    if (w == 0) {
+      if (kk + 1 < CB) {
+        // column kk + 1 first: the next pivot's chain overlaps the other updates
+        x[kk + 1] = fma(-v, Lt[kk][kk + 1], x[kk + 1]);
+        const double d = readlane_d(x[kk + 1], kk + 1);
+        if (!(d > 0.0) && bad == 0) bad = kk + 2;
+        double s, rs;
+        sqrt_rsqrt(d, s, rs);
+        const double vn = (l == kk + 1) ? s : x[kk + 1] * rs;
+#pragma unroll
+        for (int c = kk + 2; c < CB; ++c) x[c] = fma(-v, Lt[kk][c], x[c]);
+        x[kk + 1] = vn;
+        Lt[kk + 1][l] = vn;
+        if (l == kk + 1) Rs[kk + 1] = rs;
+        v = vn;
+      }
+    } else if (w == 1 && need_w) {
+      x[kk] *= Rs[kk];
+#pragma unroll
+      for (int c = kk + 1; c < CB; ++c) x[c] = fma(-Lt[kk][c], x[kk], x[c]);
+    }
+  }
+  // after the last barrier only wave 1 reads Lt (row 63, no entries) and Rs
+  if (w == 0) {
+    if (l == 0 && bad && *info == 0) *info = base + bad;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) C[l][c] = x[c];
+  } else if (w == 1 && need_w) {
+#pragma unroll
+    for (int c = 0; c < CB; ++c) Lt[c][l] = x[c];
+  }
+  __syncthreads();
+  const int c = l, rb = w;
+#pragma unroll
+  for (int i = 0; i < CB / 4; ++i) {
+    const int r = rb + 4 * i;
+    if (r < jb && c <= r) adiag[(long)r * lda + c] = C[r][c];
+    if (need_w) w_out[r * CB + c] = Lt[r][c];
+  }
+}
+
+__device__ inline void lower_tile_of(int t, int& ti, int& tj) {
+  int i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((i + 1) * (i + 2) / 2 <= t) ++i;
+  while (i * (i + 1) / 2 > t) --i;
+  ti = i;
+  tj = t - i * (i + 1) / 2;
+}
+
+__global__ __launch_bounds__(CT) void chol_step(double* a, long lda, int k, int nb, int j,
+                                                const double* __restrict__ w_in, double* __restrict__ w_out, int* info) {
+  __shared__ double Ws[CB][CB + 1];
+  __shared__ double Xi[CB][CB + 1];
+  __shared__ double Xj[CB][CB + 1];
+  __shared__ double Rs[CB];
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  int ti, tj;
+  lower_tile_of((int)blockIdx.x, ti, tj);
+  const int I = j + 1 + ti, J = j + 1 + tj;
+  const long rI = (long)I * CB, rJ = (long)J * CB;
+  const bool factor = blockIdx.x == 0;  // tile (j + 1, j + 1)
+  if (j >= 0) {
+    const long cj = (long)j * CB;
+    load_tile(a, lda, k, rI, cj, Xi);
+    if (J != I) load_tile(a, lda, k, rJ, cj, Xj);
+    {
+      double v[CB / 4];
+#pragma unroll
+      for (int i = 0; i < CB / 4; ++i) v[i] = w_in[(w + 4 * i) * CB + l];
+#pragma unroll
+      for (int i = 0; i < CB / 4; ++i) Ws[w + 4 * i][l] = v[i];
+    }
+    __syncthreads();
+    panel_rows(Xi, Ws);
+    if (J != I) panel_rows(Xj, Ws);
+    __syncthreads();
+    const double(*Lj)[CB + 1] = (J != I) ? Xj : Xi;
+    if (I == J) {
+      // L_I^T parked in the upper block (j, I): a[cj + c][rI + r] = L_I[r][c]
+#pragma unroll
+      for (int i = 0; i < CB / 4; ++i) {
+        const int c = w + 4 * i;
+        if (rI + l < k) a[(cj + c) * lda + rI + l] = Xi[l][c];
+      }
+    }
+    d4 acc[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[cb][v] = ext_at(a, lda, k, rI + 16 * w + (l >> 4) + 4 * v, rJ + 16 * cb + (l & 15));
+#pragma unroll
+    for (int q0 = 0; q0 < CB; q0 += 4) {
+      const double av = -Xi[16 * w + (l & 15)][q0 + (l >> 4)];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) acc[cb] = mfma_f64(av, Lj[16 * cb + (l & 15)][q0 + (l >> 4)], acc[cb]);
+    }
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long r = rI + 16 * w + (l >> 4) + 4 * v, c = rJ + 16 * cb + (l & 15);
+        if (r < k && c < k) a[r * lda + c] = acc[cb][v];
+      }
+    if (!factor) return;
+    // I == J here, so Xj is free; the other waves read only Xi
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Xj[16 * w + (l >> 4) + 4 * v][16 * cb + (l & 15)] = acc[cb][v];
+  } else {
+    if (!factor) return;  // step -1 is launched as one workgroup
+    load_tile(a, lda, k, 0, 0, Xj);
+  }
+  __syncthreads();
+  factor_block(Xj, Ws, Rs, j + 2 < nb, a + rI * lda + rI, lda, (int)std::min<long>(CB, k - rI), w_out, info,
+               (int)rI);
+}
+
+// Parked panels into the lower triangle, upper triangle zeroed: workgroup
+// per lower block (I, p); the upper block (p, I) holds L(I, p) transposed.
+__global__ __launch_bounds__(CT) void chol_finish(double* a, long lda, int k) {
+  __shared__ double S[CB][CB + 1];
+  int I, p;
+  lower_tile_of((int)blockIdx.x, I, p);
+  const int c = threadIdx.x & 63, rb = threadIdx.x >> 6;
+  const long rI = (long)I * CB, rp = (long)p * CB;
+  if (I == p) {
+#pragma unroll
+    for (int i = 0; i < CB / 4; ++i) {
+      const int r = rb + 4 * i;
+      if (c > r && rI + c < k) a[(rI + r) * lda + rI + c] = 0.0;
+    }
+    return;
+  }
+  const bool col_ok = rI + c < k;
+  double v[CB / 4];
+#pragma unroll
+  for (int i = 0; i < CB / 4; ++i) v[i] = a[col_ok ? (rp + rb + 4 * i) * lda + rI + c : 0];
+#pragma unroll
+  for (int i = 0; i < CB / 4; ++i) {
+    S[rb + 4 * i][c] = v[i];
+    if (col_ok) a[(rp + rb + 4 * i) * lda + rI + c] = 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < CB / 4; ++i) {
+    const int r = rb + 4 * i;
+    if (rI + r < k) a[(rI + r) * lda + rp + c] = S[c][r];
   }
 }
 
@@ -242,29 +374,21 @@ inline int grid_for(long count) { return (int)std::min<long>(4096, std::max<long
 
 }  // namespace
 
-size_t chol_work_bytes(int) { return align_up(sizeof(double) * NB * NB, 256) + align_up(sizeof(double) * NB, 256); }
+size_t chol_work_bytes(int) { return 2 * align_up(sizeof(double) * CB * CB, 256); }
 
 int cholesky_f64(double* a, int k, long lda, int* d_info, void* work, hipStream_t s) {
-  double* lt = reinterpret_cast<double*>(work);
-  double* rinv = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + align_up(sizeof(double) * NB * NB, 256));
-  for (int j0 = 0; j0 < k; j0 += NB) {
-    const int jb = std::min(NB, k - j0);
-    hipLaunchKernelGGL(potrf_diag, dim3(1), dim3(64), 0, s, a, lda, j0, jb, d_info, lt, rinv);
+  if (k < 1) return THC_OK;
+  // W_j lives in wb[j & 1]: step j - 1 writes it, step j reads it
+  double* wb[2] = {reinterpret_cast<double*>(work),
+                   reinterpret_cast<double*>(reinterpret_cast<char*>(work) + align_up(sizeof(double) * CB * CB, 256))};
+  const int nb = ceil_div(k, CB);
+  for (int j = -1; j < nb - 1; ++j) {
+    const long tb = j < 0 ? 1 : nb - j - 1;  // step -1 has only the factor of block 0
+    hipLaunchKernelGGL(chol_step, dim3((unsigned)(tb * (tb + 1) / 2)), dim3(CT), 0, s, a, lda, k, nb, j,
+                       wb[(j + 2) & 1], wb[(j + 1) & 1], d_info);
     THC_HIP(hipGetLastError());
-    const long rows = k - j0 - jb;
-    if (rows <= 0) continue;
-    hipLaunchKernelGGL(trsm_panel, dim3(ceil_div(rows, NB)), dim3(64), 0, s, a, lda, j0, jb, rows, lt, rinv);
-    THC_HIP(hipGetLastError());
-    // trailing lower update A22 -= L21 L21^T
-    GemmArgs u = {};
-    u.m = rows; u.n = rows; u.k = jb;
-    u.a = a + (long)(j0 + jb) * lda + j0; u.lda = lda; u.a_trans = 0;
-    u.b = u.a; u.ldb = lda; u.b_trans = 1;
-    u.c = a + (long)(j0 + jb) * lda + (j0 + jb); u.ldc = lda;
-    u.alpha = -1.0; u.beta = 1.0; u.lower = 1; u.batch = 1;
-    THC_TRY(gemm_f64(u, s));
   }
-  hipLaunchKernelGGL(zero_upper_kernel, dim3(grid_for((long)k * k)), dim3(256), 0, s, a, k, lda);
+  hipLaunchKernelGGL(chol_finish, dim3((unsigned)((long)nb * (nb + 1) / 2)), dim3(CT), 0, s, a, lda, k);
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
