@@ -69,7 +69,7 @@ def test_cholesky_not_posdef(ctx, k, bad):
     assert info == bad + 1
 
 
-@pytest.mark.parametrize("k,n", [(100, 5000), (513, 20000)])
+@pytest.mark.parametrize("k,n", [(100, 5000), (513, 20000), (1100, 3000), (2048, 2500)])
 def test_least_squares(ctx, k, n):
     rng = np.random.default_rng(k)
     b = rng.standard_normal((k, 3 * k))

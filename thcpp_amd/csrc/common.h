@@ -105,8 +105,16 @@ struct GemmArgs {
   int epi_hadamard;
   int epi_addsquare;
   int lower;
+  int tri;       // GEMM_TRI_* bits: known zero blocks of A / B that bound each tile's k range
   int batch;
   long stride_a, stride_b, stride_c;
+};
+// Triangular operands: the k range of the output tile at rows m0 .. m0+BM-1,
+// columns n0 .. n0+BN-1 is cut to where both operands can be nonzero.
+enum {
+  GEMM_TRI_A_KGE_M = 1,  // A[i][q] = 0 for q < i  (A = L^T of a lower L)
+  GEMM_TRI_B_KGE_N = 2,  // B[q][j] = 0 for q < j  (B lower triangular)
+  GEMM_TRI_A_KLE_M = 4,  // A[i][q] = 0 for q > i  (A lower triangular)
 };
 int gemm_f64(const GemmArgs& g, hipStream_t s);
 int gemm_f32(const GemmArgs& g, hipStream_t s);

@@ -15,7 +15,7 @@
 // complex row, transformed along z, unpacked into the half spectrum, then
 // transformed along y.  Pass B, one workgroup per (mu, y-index): transform along
 // x, weight, store.  Both passes run a Stockham autosort FFT in LDS with radix
-// 4, 2, 3 and 5 stages; twiddles come from a table of e^{-2 pi i j/n}.
+// 8, 4, 2, 3 and 5 stages; twiddles come from a table of e^{-2 pi i j/n}.
 #include <algorithm>
 
 #include "common.h"
@@ -35,6 +35,17 @@ template <class T>
 __device__ inline C2<T> csub(C2<T> a, C2<T> b) { return C2<T>{a.x - b.x, a.y - b.y}; }
 
 constexpr int MAXR = 8;  // max stages
+
+// 4-point forward DFT (e^{-2 pi i / 4} = -i)
+template <class T>
+__device__ inline void dft4(C2<T> x0, C2<T> x1, C2<T> x2, C2<T> x3, C2<T>& y0, C2<T>& y1, C2<T>& y2, C2<T>& y3) {
+  const C2<T> a = cadd(x0, x2), b = csub(x0, x2);
+  const C2<T> c = cadd(x1, x3), d = csub(x1, x3);
+  y0 = cadd(a, c);
+  y2 = csub(a, c);
+  y1 = C2<T>{b.x + d.y, b.y - d.x};  // b - i d
+  y3 = C2<T>{b.x - d.y, b.y + d.x};  // b + i d
+}
 
 struct Plan1D {
   int n;
@@ -82,12 +93,24 @@ __device__ inline void stage(const C2<T>* __restrict__ src, C2<T>* __restrict__ 
       y[0] = cadd(x[0], x[1]);
       y[1] = csub(x[0], x[1]);
     } else if (R == 4) {
-      const C2<T> a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
-      const C2<T> c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
-      y[0] = cadd(a, c);
-      y[2] = csub(a, c);
-      y[1] = C2<T>{b.x + d.y, b.y - d.x};  // b - i d
-      y[3] = C2<T>{b.x - d.y, b.y + d.x};  // b + i d
+      dft4(x[0], x[1], x[2], x[3], y[0], y[1], y[2], y[3]);
+    } else if (R == 8) {
+      // even / odd halves by radix 4, joined with w8^p = e^{-2 pi i p / 8}
+      C2<T> e[4], o[4];
+      dft4(x[0], x[2], x[4], x[6], e[0], e[1], e[2], e[3]);
+      dft4(x[1], x[3], x[5], x[7], o[0], o[1], o[2], o[3]);
+      const T h = T(0.70710678118654752440);
+      const C2<T> w1 = C2<T>{(o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h};  // w8 o1
+      const C2<T> w2 = C2<T>{o[2].y, -o[2].x};                                // -i o2
+      const C2<T> w3 = C2<T>{(o[3].y - o[3].x) * h, -(o[3].x + o[3].y) * h};  // w8^3 o3
+      y[0] = cadd(e[0], o[0]);
+      y[4] = csub(e[0], o[0]);
+      y[1] = cadd(e[1], w1);
+      y[5] = csub(e[1], w1);
+      y[2] = cadd(e[2], w2);
+      y[6] = csub(e[2], w2);
+      y[3] = cadd(e[3], w3);
+      y[7] = csub(e[3], w3);
     } else {
       const int step = n / R;
 #pragma unroll
@@ -110,7 +133,8 @@ __device__ C2<T>* fft_lines(C2<T>* a, C2<T>* b, const Plan1D& pl, int nlines, in
   int ns = 1;
   for (int s = 0; s < pl.nst; ++s) {
     const int r = pl.rad[s];
-    if (r == 4) stage<T, 4>(a, b, pl.n, ns, nlines, es, ls, tw);
+    if (r == 8) stage<T, 8>(a, b, pl.n, ns, nlines, es, ls, tw);
+    else if (r == 4) stage<T, 4>(a, b, pl.n, ns, nlines, es, ls, tw);
     else if (r == 2) stage<T, 2>(a, b, pl.n, ns, nlines, es, ls, tw);
     else if (r == 3) stage<T, 3>(a, b, pl.n, ns, nlines, es, ls, tw);
     else stage<T, 5>(a, b, pl.n, ns, nlines, es, ls, tw);
@@ -227,6 +251,7 @@ bool make_plan(int n, Plan1D& pl) {
   pl.n = n;
   pl.nst = 0;
   int m = n;
+  while (m % 8 == 0 && pl.nst < MAXR) { pl.rad[pl.nst++] = 8; m /= 8; }
   while (m % 4 == 0 && pl.nst < MAXR) { pl.rad[pl.nst++] = 4; m /= 4; }
   while (m % 2 == 0 && pl.nst < MAXR) { pl.rad[pl.nst++] = 2; m /= 2; }
   while (m % 3 == 0 && pl.nst < MAXR) { pl.rad[pl.nst++] = 3; m /= 3; }

@@ -53,6 +53,7 @@ struct KArgs {
   void* c; long ldc; int c_trans;
   double alpha, beta;
   int lower;
+  int tri;
   long stride_a, stride_b, stride_c;
   long kchunk;  // split-K chunk (EPI_PARTIAL)
 };
@@ -165,6 +166,12 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs p) {
   if (EPI == EPI_PARTIAL) {
     kbeg = (long)blockIdx.y * p.kchunk;
     kend = min(p.k, kbeg + p.kchunk);
+  }
+  if (p.tri) {
+    if (p.tri & GEMM_TRI_A_KGE_M) kbeg = max(kbeg, m0);
+    if (p.tri & GEMM_TRI_B_KGE_N) kbeg = max(kbeg, n0);
+    if (p.tri & GEMM_TRI_A_KLE_M) kend = min(kend, m0 + BM);
+    kbeg = kbeg / BK * BK;
   }
 
   const int tid = threadIdx.x;
@@ -373,6 +380,7 @@ KArgs make_args(const GemmArgs& g) {
   p.c = g.c; p.ldc = g.ldc; p.c_trans = g.c_trans;
   p.alpha = g.alpha; p.beta = g.beta;
   p.lower = g.lower;
+  p.tri = g.tri;
   p.stride_a = g.stride_a; p.stride_b = g.stride_b; p.stride_c = g.stride_c;
   return p;
 }

@@ -11,6 +11,7 @@
 // forms Linv21 = -Linv22 (L21 Linv11) for every pair of blocks of one size in
 // two batched GEMMs per level; finally A^-1 = Linv^T Linv.
 #include <algorithm>
+#include <utility>
 
 #include "common.h"
 
@@ -90,20 +91,17 @@ __device__ inline double readlane_d(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// sqrt(d) and 1/sqrt(d) from the hardware reciprocal square root, two
-// coupled Newton steps (g -> sqrt, h -> 1/(2 sqrt)) and a final residual
-// correction of g; non-positive or NaN d give NaN or inf, as sqrt would.
+// 1/sqrt(d) from the hardware reciprocal square root (about 23 bits) and two
+// Newton steps y <- y (3/2 - d/2 y^2) (quadratic: 2^-23 -> 2^-45 -> rounding
+// level), sqrt(d) = d / sqrt(d); seven dependent operations after the
+// estimate.  Non-positive or NaN d give NaN, as sqrt would.
 __device__ inline void sqrt_rsqrt(double d, double& s, double& rs) {
-  const double y = __builtin_amdgcn_rsq(d);
-  double g = d * y, h = 0.5 * y;
-  double r = fma(-h, g, 0.5);
-  g = fma(g, r, g);
-  h = fma(h, r, h);
-  r = fma(-h, g, 0.5);
-  g = fma(g, r, g);
-  h = fma(h, r, h);
-  s = fma(fma(-g, g, d), h, g);
-  rs = 2.0 * h;
+  const double h = 0.5 * d;
+  double y = __builtin_amdgcn_rsq(d);
+  y = y * fma(-h, y * y, 1.5);
+  y = y * fma(-h, y * y, 1.5);
+  rs = y;
+  s = d * y;
 }
 
 // a[r][c] of the matrix extended by the identity beyond k
@@ -142,12 +140,69 @@ __device__ inline void panel_rows(double (*X)[CB + 1], const double (*W)[CB + 1]
     for (int v = 0; v < 4; ++v) X[16 * w + (l >> 4) + 4 * v][16 * cb + (l & 15)] = acc[cb][v];
 }
 
+// One pivot step KK of factor_block (after its barrier).  Wave 0 (lane r
+// owns row r of the block in x): column KK + 1 takes step KK's update first,
+// with L[KK+1][KK] read from lane KK + 1's register, so the next pivot, its
+// Newton steps and the scaled column KK + 1 (published as row KK + 1 of Lt)
+// follow without an LDS round trip; the other columns read row KK of Lt, up
+// to RA entries of it issued before that dependent chain so that the LDS
+// latency hides behind it.  Wave 1 (lane c owns column c of W = L^-1 in x)
+// takes substitution step KK with row KK of Lt and Rs[KK].
+constexpr int RA = 32;
+
+template <int KK>
+__device__ __forceinline__ void pivot_step(double (&x)[CB], double& v, int& bad, double (*Lt)[CB + 1], double* Rs,
+                                           int w, int l, bool need_w) {
+  if (w == 0) {
+    if (KK + 1 < CB) {
+      constexpr int c0 = KK + 2, na = CB - c0 < RA ? CB - c0 : RA;
+      double ra[na > 0 ? na : 1];
+#pragma unroll
+      for (int i = 0; i < na; ++i) ra[i] = Lt[KK][c0 + i];
+      __asm__ volatile("" ::: "memory");
+      x[KK + 1] = fma(-v, readlane_d(v, KK + 1), x[KK + 1]);
+      const double d = readlane_d(x[KK + 1], KK + 1);
+      if (!(d > 0.0) && bad == 0) bad = KK + 2;
+      double s, rs;
+      sqrt_rsqrt(d, s, rs);
+      const double vn = (l == KK + 1) ? s : x[KK + 1] * rs;
+#pragma unroll
+      for (int i = 0; i < na; ++i) x[c0 + i] = fma(-v, ra[i], x[c0 + i]);
+#pragma unroll
+      for (int c = c0 + na; c < CB; ++c) x[c] = fma(-v, Lt[KK][c], x[c]);
+      x[KK + 1] = vn;
+      Lt[KK + 1][l] = vn;
+      if (l == KK + 1) Rs[KK + 1] = rs;
+      v = vn;
+    }
+  } else if (w == 1 && need_w) {
+    constexpr int c0 = KK + 1, na = CB - c0 < RA ? CB - c0 : RA;
+    const double r = Rs[KK];
+    double ra[na > 0 ? na : 1];
+#pragma unroll
+    for (int i = 0; i < na; ++i) ra[i] = Lt[KK][c0 + i];
+    __asm__ volatile("" ::: "memory");
+    x[KK] *= r;
+#pragma unroll
+    for (int i = 0; i < na; ++i) x[c0 + i] = fma(-ra[i], x[KK], x[c0 + i]);
+#pragma unroll
+    for (int c = c0 + na; c < CB; ++c) x[c] = fma(-Lt[KK][c], x[KK], x[c]);
+  }
+}
+
+template <int... K>
+__device__ __forceinline__ void pivot_steps(std::integer_sequence<int, K...>, double (&x)[CB], double& v, int& bad,
+                                            double (*Lt)[CB + 1], double* Rs, int w, int l, bool need_w) {
+  ((__syncthreads(), pivot_step<K>(x, v, bad, Lt, Rs, w, l, need_w)), ...);
+}
+
 // Factor the block in C (LDS, lower part read) at global rows/cols base ..
 // base + 63 (jb of them inside the matrix): L (lower) to adiag, W = L^-1 to
 // w_out (row-major 64 x 64, zero above the diagonal) when need_w.  Lt (row kk
 // = column kk of L) and Rs (1 / L[kk][kk]) are LDS scratch.  All four waves
-// call this (the barriers are workgroup-wide).  info: first non-positive
-// pivot (1-based, global), kept if already set.
+// call this (the barriers are workgroup-wide); waves 0 and 1 work in
+// lockstep, one barrier per pivot.  info: first non-positive pivot (1-based,
+// global), kept if already set.
 __device__ void factor_block(double (*C)[CB + 1], double (*Lt)[CB + 1], double* Rs, bool need_w, double* adiag,
                              long lda, int jb, double* w_out, int* info, int base) {
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
@@ -169,31 +224,7 @@ __device__ void factor_block(double (*C)[CB + 1], double (*Lt)[CB + 1], double* 
 #pragma unroll
     for (int c = 0; c < CB; ++c) x[c] = (c == l) ? 1.0 : 0.0;
   }
-#pragma unroll
-  for (int kk = 0; kk < CB; ++kk) {
-    __syncthreads();
-    if (w == 0) {
-      if (kk + 1 < CB) {
-        // column kk + 1 first: the next pivot's chain overlaps the other updates
-        x[kk + 1] = fma(-v, Lt[kk][kk + 1], x[kk + 1]);
-        const double d = readlane_d(x[kk + 1], kk + 1);
-        if (!(d > 0.0) && bad == 0) bad = kk + 2;
-        double s, rs;
-        sqrt_rsqrt(d, s, rs);
-        const double vn = (l == kk + 1) ? s : x[kk + 1] * rs;
-#pragma unroll
-        for (int c = kk + 2; c < CB; ++c) x[c] = fma(-v, Lt[kk][c], x[c]);
-        x[kk + 1] = vn;
-        Lt[kk + 1][l] = vn;
-        if (l == kk + 1) Rs[kk + 1] = rs;
-        v = vn;
-      }
-    } else if (w == 1 && need_w) {
-      x[kk] *= Rs[kk];
-#pragma unroll
-      for (int c = kk + 1; c < CB; ++c) x[c] = fma(-Lt[kk][c], x[kk], x[c]);
-    }
-  }
+  pivot_steps(std::make_integer_sequence<int, CB>{}, x, v, bad, Lt, Rs, w, l, need_w);
   // after the last barrier only wave 1 reads Lt (row 63, no entries) and Rs
   if (w == 0) {
     if (l == 0 && bad && *info == 0) *info = base + bad;
@@ -425,6 +456,7 @@ int spd_inverse_from_chol(const double* l, int k, long lda, double* ainv, void* 
     g.b = linv; g.ldb = kp; g.b_trans = 0;
     g.c = tmp; g.ldc = sz;
     g.alpha = 1.0; g.beta = 0.0; g.batch = pairs;
+    g.tri = GEMM_TRI_B_KGE_N;  // Linv11 is lower triangular
     g.stride_a = step; g.stride_b = step; g.stride_c = (long)sz * sz;
     THC_TRY(gemm_f64(g, s));
     // Linv21 = -Linv22 tmp_p
@@ -434,17 +466,23 @@ int spd_inverse_from_chol(const double* l, int k, long lda, double* ainv, void* 
     h.b = tmp; h.ldb = sz; h.b_trans = 0;
     h.c = linv + (long)sz * kp; h.ldc = kp;
     h.alpha = -1.0; h.beta = 0.0; h.batch = pairs;
+    h.tri = GEMM_TRI_A_KLE_M;  // Linv22 is lower triangular
     h.stride_a = step; h.stride_b = (long)sz * sz; h.stride_c = step;
     THC_TRY(gemm_f64(h, s));
   }
-  // A^-1 = Linv^T Linv (top-left k x k)
+  // A^-1 = Linv^T Linv (top-left k x k): lower tiles only, each over the k
+  // range where both triangular operands are nonzero, then mirrored
   GemmArgs f = {};
   f.m = k; f.n = k; f.k = k;
   f.a = linv; f.lda = kp; f.a_trans = 1;
   f.b = linv; f.ldb = kp; f.b_trans = 0;
   f.c = ainv; f.ldc = k;
   f.alpha = 1.0; f.beta = 0.0; f.batch = 1;
+  f.lower = 1;
+  f.tri = GEMM_TRI_A_KGE_M | GEMM_TRI_B_KGE_N;
   THC_TRY(gemm_f64(f, s));
+  hipLaunchKernelGGL(symmetrize_kernel, dim3(grid_for((long)k * k)), dim3(256), 0, s, ainv, k, (long)k);
+  THC_HIP(hipGetLastError());
   return THC_OK;
 }
 
