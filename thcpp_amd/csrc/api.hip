@@ -402,6 +402,7 @@ int thc_ctx_create(int device, void* stream, thc_ctx** out) {
   hipError_t r = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
   if (r == hipSuccess) r = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
   if (r == hipSuccess) r = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+  for (int q = 0; q < 3 && r == hipSuccess; ++q) r = hipEventCreateWithFlags(&c->kev[q], hipEventDisableTiming);
   if (r != hipSuccess) {
     set_last_error(hipGetErrorString(r), __FILE__, __LINE__);
     thc_ctx_destroy(c);
@@ -429,6 +430,8 @@ int thc_ctx_destroy(thc_ctx* c) {
   }
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->join) (void)hipEventDestroy(c->join);
+  for (hipEvent_t e : c->kev)
+    if (e) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   for (thc_ctx* l : c->lanes) thc_ctx_destroy(l);
@@ -685,6 +688,11 @@ int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isd
     c->use_graphs = ctx->use_graphs;
     ctx->lanes.push_back(c);
   }
+  // with several lanes the streams outnumber the hardware queues: keep each
+  // lane's k-means on its own stream
+  const bool overlap0 = ctx->km_overlap;
+  ctx->km_overlap = nl == 1;
+  for (thc_ctx* c : ctx->lanes) c->km_overlap = nl == 1;
   // longest first onto the least loaded lane (cost ~ solve + k-means work)
   std::vector<int> order(nsys);
   std::vector<double> cost(nsys);
@@ -727,6 +735,7 @@ int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isd
   }
   work(0);
   for (auto& t : threads) t.join();
+  ctx->km_overlap = overlap0;
   int first_warn = THC_OK;
   for (int i = 0; i < nsys; ++i) {
     if (sys[i].status != THC_OK && sys[i].status < 100) {

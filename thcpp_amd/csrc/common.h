@@ -63,6 +63,8 @@ struct thc_ctx {
   hipEvent_t ev[10] = {};
   hipStream_t aux = nullptr;      // second stream: CC^T factorisation beside the ZC^T GEMM
   hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t kev[3] = {};  // k-means: centroid sums done, loop control done (by iteration parity)
+  bool km_overlap = true;  // k-means loop control on 'aux' beside the next iteration (off in multi-lane batches)
   void* pinned = nullptr;  // small pinned host buffer for status read-back
   size_t pinned_bytes = 0;
   std::vector<thc_ctx*> lanes;  // extra stream contexts for batched calls (owned)

@@ -14,14 +14,14 @@ enum { MODE_LATEST = 0, MODE_FINAL = 1, MODE_ZEROS = 2 };
 
 // Device-side loop state (one per run).
 struct State {
-  int done;       // loop ended: diff < threshold, or an error
-  int final_buf;  // centroid buffer with the converged centroids (MODE_FINAL)
-  int error;      // 1: a cluster weight < 1e-8
-  int changed;    // points whose cluster changed in the current iteration
-  int iters;      // iterations executed
-  int mode;       // which centroids map_to_grid uses (MODE_*)
-  int dups;       // repeated indices after map_to_grid
-  int pad;
+  int done;        // loop ended: diff < threshold, or an error
+  int final_buf;   // centroid buffer with the converged centroids (MODE_FINAL)
+  int error;       // 1: a cluster weight < 1e-8 in an iteration the loop ran; 2: bad cluster key
+  int changed[2];  // points whose cluster changed, by iteration parity
+  int serr[2];     // a cluster weight < 1e-8 in the centroid sums, by iteration parity
+  int iters;       // iterations executed
+  int mode;        // which centroids map_to_grid uses (MODE_*)
+  int dups;        // repeated indices after map_to_grid
   double last_diff;
 };
 
@@ -37,6 +37,7 @@ struct KmGraphCache {
   int k = 0, max_it = 0;
   double threshold = 0;
   void* base = nullptr;
+  bool overlap = false;
   void reset() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);

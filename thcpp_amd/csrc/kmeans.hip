@@ -169,7 +169,8 @@ __global__ void init_state(State* st, int mode) {
     st->done = 0;
     st->final_buf = 0;
     st->error = 0;
-    st->changed = 0;
+    st->changed[0] = st->changed[1] = 0;
+    st->serr[0] = st->serr[1] = 0;
     st->iters = 0;
     st->mode = mode;
     st->dups = 0;
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(256) void cs_scatter(const int* keys, long n, int n
 __global__ __launch_bounds__(256) void assign(const double* sx, const double* sy, const double* sz,
                                               const int* perm, long n, const double* boxes,
                                               const double* cen, int k, int* map_new, const int* map_old,
-                                              State* st) {
+                                              State* st, int par) {
   if (st && st->done) return;
   __shared__ int cidx[MAXB];
   __shared__ double ccx[CAND_CAP], ccy[CAND_CAP], ccz[CAND_CAP];
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(256) void assign(const double* sx, const double* sy
   }
   if (st && map_old) {
     changed = wave_sum(changed);
-    if (lane == 0 && changed) atomicAdd(&st->changed, changed);
+    if (lane == 0 && changed) atomicAdd(&st->changed[par], changed);
   }
 }
 
@@ -411,7 +412,7 @@ constexpr int CS_CHUNK = 256;
 
 __global__ __launch_bounds__(256) void centroid_sums(const int* order, const int* start, const int* total,
                                                      const double* rho, const double* grid, int k,
-                                                     double* cen_out, State* st) {
+                                                     double* cen_out, State* st, int par) {
   if (st && st->done) return;
   __shared__ double buf[4][4][CS_CHUNK];  // [wave][w, x, y, z][member]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -441,7 +442,7 @@ __global__ __launch_bounds__(256) void centroid_sums(const int* order, const int
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
   const double w = __shfl(acc, 0);
-  if (lane == 0 && w < 1e-8) atomicOr(&st->error, 1);
+  if (lane == 0 && w < 1e-8) atomicOr(&st->serr[par], 1);
   if (lane >= 1 && lane < 4) cen_out[c * 3 + lane - 1] = acc / w;
 }
 
@@ -454,9 +455,9 @@ __global__ __launch_bounds__(1024) void diff_control(const double* nxt, const do
   extern __shared__ double terms[];
   __shared__ double red[4][16];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int len = 3 * k;
-  if (st->error) {
-    if (tid == 0) { st->done = 1; st->iters = it + 1; }
+  const int len = 3 * k, par = it & 1;
+  if (st->serr[par]) {
+    if (tid == 0) { st->error = 1; st->done = 1; st->iters = it + 1; }
     return;
   }
   double mxa = -DBL_MAX, mna = DBL_MAX, mxb = -DBL_MAX, mnb = DBL_MAX;
@@ -506,12 +507,12 @@ __global__ __launch_bounds__(1024) void diff_control(const double* nxt, const do
     }
     for (int i = nb * 16; i < len; ++i) acc += terms[i];
     const double diff = scale * sqrt(acc);
-    const double pct = 100 * ((double)st->changed) / (double)n;
+    const double pct = 100 * ((double)st->changed[par]) / (double)n;
     diffs[it] = diff;
     pcts[it] = pct;
     st->iters = it + 1;
     st->last_diff = diff;
-    st->changed = 0;
+    st->changed[par] = 0;  // the next iteration of this parity waits for this kernel
     if (diff < threshold) {
       st->done = 1;
       st->mode = MODE_FINAL;
@@ -528,7 +529,7 @@ __global__ __launch_bounds__(1024) void diff_control(const double* nxt, const do
 // the grid tiles the bounds cannot exclude).
 __global__ __launch_bounds__(256) void map_to_grid(const double* sx, const double* sy, const double* sz,
                                                    const int* perm, long n, const double* boxes, int ntiles,
-                                                   const double* cen_a, const double* cen_b, int latest_buf,
+                                                   const double* cen3, long cstride, int latest_buf,
                                                    const State* st, int* out) {
   const int c = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -538,7 +539,7 @@ __global__ __launch_bounds__(256) void map_to_grid(const double* sx, const doubl
     return;
   }
   const int buf = (mode == MODE_FINAL) ? st->final_buf : latest_buf;
-  const double* cen = buf ? cen_b : cen_a;
+  const double* cen = cen3 + buf * cstride;
   const double cx = cen[c * 3], cy = cen[c * 3 + 1], cz = cen[c * 3 + 2];
   __shared__ double redd[4];
   __shared__ int redi[4];
@@ -621,13 +622,13 @@ __global__ __launch_bounds__(1024) void sort_indices(int* idx, int k, State* st)
 namespace thc {
 namespace km {
 
-__global__ void copy_final(const int* map0, const int* map1, long n, const double* cen0, const double* cen1,
-                           int k, int latest_buf, const State* st, int* map_out, double* cen_out) {
+__global__ void copy_final(const int* map0, const int* map1, long n, const double* cen3, int k, int latest_buf,
+                           const State* st, int* map_out, double* cen_out) {
   const int iters = st->iters;
   const int mbuf = iters > 0 ? (iters - 1) & 1 : 0;
   const int cbuf = st->mode == MODE_FINAL ? st->final_buf : latest_buf;
   const int* m = mbuf ? map1 : map0;
-  const double* c = cbuf ? cen1 : cen0;
+  const double* c = cen3 + cbuf * 3L * k;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x) {
     if (map_out) map_out[i] = m[i];
     if (cen_out && i < 3L * k) cen_out[i] = c[i];
@@ -670,7 +671,7 @@ static Layout plan(long n, int k, int max_it) {
 }
 
 struct Bufs {
-  double *part, *box, *sx, *sy, *sz, *boxes, *cen[2], *diffs, *pcts;
+  double *part, *box, *sx, *sy, *sz, *boxes, *cen3, *diffs, *pcts;  // cen3: three centroid buffers
   int *keys, *perm, *map[2], *order, *tc, *total, *start, *idx0;
   State* st;
 };
@@ -685,7 +686,7 @@ static size_t bytes_for(const Layout& L, bool full) {
   b += carve_bytes<int>((size_t)L.ncs * nb) + 2 * carve_bytes<int>(nb);
   b += carve_bytes<State>(1);
   if (full) {
-    b += 2 * carve_bytes<double>(3L * L.k) + 2 * carve_bytes<double>(std::max(1, L.max_it));
+    b += carve_bytes<double>(9L * L.k) + 2 * carve_bytes<double>(std::max(1, L.max_it));
     b += 3 * carve_bytes<int>(n) + carve_bytes<int>(L.k);
   }
   return b;
@@ -710,15 +711,14 @@ static bool carve(Arena& a, const Layout& L, Bufs& b, bool full) {
   bool ok = b.part && b.box && b.sx && b.sy && b.sz && b.boxes && b.keys && b.perm && b.tc && b.total &&
             b.start && b.st;
   if (full) {
-    b.cen[0] = a.take<double>(3L * L.k);
-    b.cen[1] = a.take<double>(3L * L.k);
+    b.cen3 = a.take<double>(9L * L.k);
     b.diffs = a.take<double>(std::max(1, L.max_it));
     b.pcts = a.take<double>(std::max(1, L.max_it));
     b.map[0] = a.take<int>(n);
     b.map[1] = a.take<int>(n);
     b.order = a.take<int>(n);
     b.idx0 = a.take<int>(L.k);
-    ok = ok && b.cen[0] && b.cen[1] && b.diffs && b.pcts && b.map[0] && b.map[1] && b.order && b.idx0;
+    ok = ok && b.cen3 && b.diffs && b.pcts && b.map[0] && b.map[1] && b.order && b.idx0;
   }
   return ok;
 }
@@ -742,31 +742,54 @@ static void enqueue_setup(const Layout& L, const Bufs& b, const double* grid, hi
   hipLaunchKernelGGL(tile_boxes, dim3(L.ntiles), dim3(256), 0, s, b.sx, b.sy, b.sz, L.n, b.boxes);
 }
 
-static void enqueue_kmeans(const Layout& L, const Bufs& b, const double* grid, const double* rho, double threshold,
-                           int* interp_idx, int* map_out, double* cen_out, hipStream_t s) {
+// The loop control of iteration it (diff_control: the sequential normed
+// difference, the stop test) runs on the second stream beside iteration
+// it + 1's assignment, sort and centroid sums, which do not need its result:
+// if iteration it stops the loop, iteration it + 1 (already started) only
+// writes buffers the final state does not use, and iteration it + 2 waits for
+// it and returns at once.  Hence three centroid buffers (iteration it reads
+// it % 3, writes (it+1) % 3, and diff_control(it) still reads both while
+// iteration it + 1 writes (it+2) % 3), and change counts / empty-cluster flags
+// kept per iteration parity (diff_control(it) reads and clears its parity's
+// before iteration it + 2 runs).  With sa == s the same code runs in order on
+// one stream.
+static int enqueue_kmeans(const Layout& L, const Bufs& b, const double* grid, const double* rho, double threshold,
+                          int* interp_idx, int* map_out, double* cen_out, hipStream_t s, hipStream_t sa,
+                          const hipEvent_t kev[3]) {
+  const long cs = 3L * L.k;
+  const bool two = sa != s;
   hipLaunchKernelGGL(init_state, dim3(1), dim3(64), 0, s, b.st, (int)MODE_LATEST);
   enqueue_setup(L, b, grid, s);
-  hipLaunchKernelGGL(init_centroids, dim3(ceil_div(L.k, 256)), dim3(256), 0, s, grid, b.idx0, L.k, b.cen[0]);
+  hipLaunchKernelGGL(init_centroids, dim3(ceil_div(L.k, 256)), dim3(256), 0, s, grid, b.idx0, L.k, b.cen3);
   const size_t dyn = (size_t)3 * L.k * sizeof(double);
   for (int it = 0; it < L.max_it; ++it) {
-    const int cur = it & 1, nxt = cur ^ 1;
-    int* mnew = b.map[it & 1];
-    const int* mold = it == 0 ? nullptr : b.map[(it + 1) & 1];
+    const int cur = it % 3, nxt = (it + 1) % 3, par = it & 1;
+    int* mnew = b.map[par];
+    const int* mold = it == 0 ? nullptr : b.map[par ^ 1];
+    if (two && it >= 2) THC_HIP(hipStreamWaitEvent(s, kev[1 + par], 0));  // diff_control(it - 2)
     hipLaunchKernelGGL(assign, dim3(L.ntiles), dim3(256), 0, s, b.sx, b.sy, b.sz, b.perm, L.n, b.boxes,
-                       b.cen[cur], L.k, mnew, mold, b.st);
+                       b.cen3 + cur * cs, L.k, mnew, mold, b.st, par);
     enqueue_sort(L, b, mnew, L.k, L.nbits_k, b.order, b.st, s);
     hipLaunchKernelGGL(centroid_sums, dim3(ceil_div(L.k, 4)), dim3(256), 0, s, b.order, b.start, b.total, rho,
-                       grid, L.k, b.cen[nxt], b.st);
-    hipLaunchKernelGGL(diff_control, dim3(1), dim3(1024), dyn, s, b.cen[nxt], b.cen[cur], L.k, L.n, threshold,
-                       it, L.max_it, nxt, b.st, b.diffs, b.pcts);
+                       grid, L.k, b.cen3 + nxt * cs, b.st, par);
+    if (two) {
+      THC_HIP(hipEventRecord(kev[0], s));
+      THC_HIP(hipStreamWaitEvent(sa, kev[0], 0));
+    }
+    hipLaunchKernelGGL(diff_control, dim3(1), dim3(1024), dyn, sa, b.cen3 + nxt * cs, b.cen3 + cur * cs, L.k, L.n,
+                       threshold, it, L.max_it, nxt, b.st, b.diffs, b.pcts);
+    if (two) THC_HIP(hipEventRecord(kev[1 + par], sa));
   }
-  const int latest = L.max_it & 1;
+  if (two) THC_HIP(hipStreamWaitEvent(s, kev[1 + ((L.max_it - 1) & 1)], 0));  // last loop control
+  const int latest = L.max_it % 3;
   hipLaunchKernelGGL(map_to_grid, dim3(L.k), dim3(256), 0, s, b.sx, b.sy, b.sz, b.perm, L.n, b.boxes, L.ntiles,
-                     b.cen[0], b.cen[1], latest, b.st, interp_idx);
+                     b.cen3, cs, latest, b.st, interp_idx);
   hipLaunchKernelGGL(sort_indices, dim3(1), dim3(1024), 0, s, interp_idx, L.k, b.st);
   if (map_out || cen_out)
-    hipLaunchKernelGGL(copy_final, dim3(256), dim3(256), 0, s, b.map[0], b.map[1], L.n, b.cen[0], b.cen[1], L.k,
-                       latest, b.st, map_out, cen_out);
+    hipLaunchKernelGGL(copy_final, dim3(256), dim3(256), 0, s, b.map[0], b.map[1], L.n, b.cen3, L.k, latest, b.st,
+                       map_out, cen_out);
+  THC_HIP(hipGetLastError());
+  return THC_OK;
 }
 
 }  // namespace km
@@ -811,22 +834,25 @@ int kmeans_run(thc_ctx* ctx, const double* grid, const double* rho, long n, int 
     KmGraphCache& gc = *ctx->kmgraph;
     const void* key[5] = {grid, rho, interp_idx, grid_map_out, centroids_out};
     bool hit = gc.exec && gc.n == n && gc.k == k && gc.max_it == max_it && gc.threshold == threshold &&
-               gc.base == ctx->kmws.base;
+               gc.base == ctx->kmws.base && gc.overlap == ctx->km_overlap;
     for (int q = 0; q < 5 && hit; ++q) hit = gc.key_ptr[q] == key[q];
     if (!hit) {
       gc.reset();
       THC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      enqueue_kmeans(L, b, grid, rho, threshold, interp_idx, grid_map_out, centroids_out, s);
+      const int est = enqueue_kmeans(L, b, grid, rho, threshold, interp_idx, grid_map_out, centroids_out, s,
+                                     ctx->km_overlap ? ctx->aux : s, ctx->kev);
       hipError_t e = hipStreamEndCapture(s, &gc.graph);
+      THC_TRY(est);
       THC_HIP(e);
       THC_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
       for (int q = 0; q < 5; ++q) gc.key_ptr[q] = key[q];
       gc.n = n; gc.k = k; gc.max_it = max_it; gc.threshold = threshold; gc.base = ctx->kmws.base;
+      gc.overlap = ctx->km_overlap;
     }
     THC_HIP(hipGraphLaunch(gc.exec, s));
   } else {
-    enqueue_kmeans(L, b, grid, rho, threshold, interp_idx, grid_map_out, centroids_out, s);
-    THC_HIP(hipGetLastError());
+    THC_TRY(enqueue_kmeans(L, b, grid, rho, threshold, interp_idx, grid_map_out, centroids_out, s,
+                           ctx->km_overlap ? ctx->aux : s, ctx->kev));
   }
   char* hp = reinterpret_cast<char*>(ctx->pinned);
   State* hst = reinterpret_cast<State*>(hp + align_up(sizeof(int) * k, 64));
@@ -867,7 +893,7 @@ int kmeans_classify(thc_ctx* ctx, const double* grid, long n, const double* cent
   hipStream_t s = ctx->stream;
   enqueue_setup(L, b, grid, s);
   hipLaunchKernelGGL(assign, dim3(L.ntiles), dim3(256), 0, s, b.sx, b.sy, b.sz, b.perm, L.n, b.boxes, centroids, k,
-                     grid_map, (const int*)nullptr, (State*)nullptr);
+                     grid_map, (const int*)nullptr, (State*)nullptr, 0);
   THC_HIP(hipGetLastError());
   THC_HIP(hipStreamSynchronize(s));
   return THC_OK;
@@ -891,13 +917,13 @@ int kmeans_update(thc_ctx* ctx, const double* rho, const double* grid, long n, c
   hipLaunchKernelGGL(check_keys, dim3(256), dim3(256), 0, s, grid_map, n, k, b.st);
   enqueue_sort(L, b, grid_map, k, L.nbits_k, order, nullptr, s);
   hipLaunchKernelGGL(centroid_sums, dim3(ceil_div(k, 4)), dim3(256), 0, s, order, b.start, b.total, rho, grid, k,
-                     centroids, b.st);
+                     centroids, b.st, 0);
   THC_HIP(hipGetLastError());
   State hst;
   THC_HIP(hipMemcpyAsync(&hst, b.st, sizeof(State), hipMemcpyDeviceToHost, s));
   THC_HIP(hipStreamSynchronize(s));
   if (hst.error & 2) return THC_ERR_ARG;
-  if (hst.error) return THC_ERR_EMPTY_CLUSTER;
+  if (hst.serr[0]) return THC_ERR_EMPTY_CLUSTER;
   return THC_OK;
 }
 
@@ -915,7 +941,7 @@ int kmeans_map_to_grid(thc_ctx* ctx, const double* grid, long n, const double* c
   hipLaunchKernelGGL(init_state, dim3(1), dim3(64), 0, s, b.st, (int)MODE_LATEST);
   enqueue_setup(L, b, grid, s);
   hipLaunchKernelGGL(map_to_grid, dim3(k), dim3(256), 0, s, b.sx, b.sy, b.sz, b.perm, L.n, b.boxes, L.ntiles,
-                     centroids, centroids, 0, (const State*)nullptr, interp_idx);
+                     centroids, 0L, 0, (const State*)nullptr, interp_idx);
   hipLaunchKernelGGL(sort_indices, dim3(1), dim3(1024), 0, s, interp_idx, k, b.st);
   THC_HIP(hipGetLastError());
   State hst;
