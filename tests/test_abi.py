@@ -43,7 +43,11 @@ def test_status_strings():
 
 def test_initial_indices_host_entry_point():
     """thc_kmeans_initial_indices runs on the host (libstdc++ shuffle)."""
-    for n, k, seed in [(32768, 128, 0), (262144, 1024, 0), (5000, 17, 99)]:
+    # both libstdc++ paths: paired draws (N <= 65535) and the one-draw-per-element
+    # path restated with a K-entry window (N > 65535), K from 1 to N
+    for n, k, seed in [(32768, 128, 0), (262144, 1024, 0), (5000, 17, 99), (65535, 40, 1), (65536, 40, 2),
+                       (65536, 65536, 3), (70001, 100, 9), (262144, 2048, 7), (100000, 1, 5),
+                       (300000, 299999, 2**32 - 1), (2097152, 8192, 0)]:
         out = np.empty(k, dtype=np.int32)
         st = L.lib().thc_kmeans_initial_indices(n, k, seed, out.ctypes.data_as(ctypes.c_void_p))
         assert st == L.THC_OK
