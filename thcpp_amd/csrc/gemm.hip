@@ -351,23 +351,42 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs p) {
       }
 }
 
+// V = FP64 sum over the split-K partials (in split order), mirrored to the
+// upper triangle: one workgroup per 32 x 32 block of a lower tile, thread t
+// owning row t / 8 and columns 4 (t % 8) .. + 3; the mirror image goes
+// through an LDS transpose so both stores are row-contiguous.
 template <class T, int BM, int BN>
-__global__ __launch_bounds__(256) void syrk_reduce_kernel(const T* work, int nsplit, long ntiles, long m,
-                                                          double* v, long ldv) {
-  const long tile_lin = blockIdx.x;
+__global__ __launch_bounds__(256) void syrk_reduce_kernel(const T* __restrict__ work, int nsplit, long ntiles, long m,
+                                                          double* __restrict__ v, long ldv) {
+  constexpr int RB = 32, SUBM = BM / RB, SUBN = BN / RB;
+  __shared__ double S[RB][RB + 1];
+  const long tile_lin = blockIdx.x / (SUBM * SUBN);
+  const int sb = blockIdx.x % (SUBM * SUBN), sr = sb / SUBN, sc = sb % SUBN;
   int tm = (int)((sqrt(8.0 * tile_lin + 1.0) - 1.0) * 0.5);
   while ((long)(tm + 1) * (tm + 2) / 2 <= tile_lin) ++tm;
   while ((long)tm * (tm + 1) / 2 > tile_lin) --tm;
   const int tn = (int)(tile_lin - (long)tm * (tm + 1) / 2);
-  for (int e = threadIdx.x; e < BM * BN; e += 256) {
-    const int r = e / BN, c = e % BN;
-    const long gr = (long)tm * BM + r, gc = (long)tn * BN + c;
-    if (gr >= m || gc >= m || gc > gr) continue;
-    double s = 0.0;
-    for (int sp = 0; sp < nsplit; ++sp) s += (double)work[((long)sp * ntiles + tile_lin) * (BM * BN) + e];
-    v[gr * ldv + gc] = s;
-    v[gc * ldv + gr] = s;
+  const int r = threadIdx.x >> 3, c0 = (threadIdx.x & 7) * 4;
+  const int rl = sr * RB + r, cl = sc * RB + c0;
+  const T* src = work + tile_lin * (BM * BN) + rl * BN + cl;
+  const long sstride = ntiles * (BM * BN);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int sp = 0; sp < nsplit; ++sp) {
+    const T* p = src + (long)sp * sstride;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] += (double)p[u];
   }
+  const long gr = (long)tm * BM + rl, gc = (long)tn * BN + cl;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    S[r][c0 + u] = acc[u];
+    if (gr < m && gc + u < m && gc + u <= gr) v[gr * ldv + gc + u] = acc[u];
+  }
+  __syncthreads();
+  const long mr = (long)tn * BN + sc * RB + r, mc = (long)tm * BM + sr * RB + c0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (mr < m && mc + u < m && mr < mc + u) v[mr * ldv + mc + u] = S[c0 + u][r];
 }
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
@@ -511,17 +530,18 @@ int syrk_splitk(const void* a, long m, long k, long lda, int prec, double* v, lo
   p.lower = 1;
   p.kchunk = kchunk;
   dim3 grid((unsigned)ntiles, (unsigned)nsplit, 1);
+  const dim3 rgrid((unsigned)(ntiles * (SY_BM / 32) * (SY_BN / 32)));  // 32 x 32 blocks of the tiles
   const int vw = prec == 0 ? 2 : 4;
   const bool vec = aligned16(a) && lda % vw == 0;
   if (prec == 0) {
     launch_vec<double, CfgLarge, EPI_PARTIAL, true, true, false, false>(p, vec, grid, s);
     THC_HIP(hipGetLastError());
-    hipLaunchKernelGGL((syrk_reduce_kernel<double, SY_BM, SY_BN>), dim3((unsigned)ntiles), dim3(256), 0, s,
+    hipLaunchKernelGGL((syrk_reduce_kernel<double, SY_BM, SY_BN>), rgrid, dim3(256), 0, s,
                        (const double*)work, nsplit, ntiles, m, v, ldv);
   } else {
     launch_vec<float, CfgLarge, EPI_PARTIAL, true, true, false, false>(p, vec, grid, s);
     THC_HIP(hipGetLastError());
-    hipLaunchKernelGGL((syrk_reduce_kernel<float, SY_BM, SY_BN>), dim3((unsigned)ntiles), dim3(256), 0, s,
+    hipLaunchKernelGGL((syrk_reduce_kernel<float, SY_BM, SY_BN>), rgrid, dim3(256), 0, s,
                        (const float*)work, nsplit, ntiles, m, v, ldv);
   }
   THC_HIP(hipGetLastError());

@@ -45,7 +45,29 @@ __device__ inline double block_max(double v, double* red) {
   return v;
 }
 
-// digits S[p][row][g] (g < g2p, zero padded) and exponents E[row][chunk]
+// four consecutive values of a row as doubles (vector loads when aligned)
+__device__ inline void load4(const float* p, bool vec, long g, long g2, double (&v)[4]) {
+  if (vec && g + 4 <= g2) {
+    const float4 q = *reinterpret_cast<const float4*>(p + g);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = g + j < g2 ? (double)p[g + j] : 0.0;
+  }
+}
+__device__ inline void load4(const double* p, bool vec, long g, long g2, double (&v)[4]) {
+  if (vec && g + 4 <= g2) {
+    const double2 q0 = *reinterpret_cast<const double2*>(p + g), q1 = *reinterpret_cast<const double2*>(p + g + 2);
+    v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = g + j < g2 ? p[g + j] : 0.0;
+  }
+}
+
+// digits S[p][row][g] (g < g2p, zero padded) and exponents E[row][chunk].
+// Thread t takes runs of 4 consecutive columns (one vector load), and stores
+// each digit plane's 4 bytes as one word.
 template <class T>
 __global__ __launch_bounds__(256) void oz_split(const T* __restrict__ b, long ld, long g2, long g2p, int k,
                                                 int nchunk, signed char* __restrict__ S, int* __restrict__ E) {
@@ -53,29 +75,37 @@ __global__ __launch_bounds__(256) void oz_split(const T* __restrict__ b, long ld
   const int c = blockIdx.x, row = blockIdx.y;
   const T* src = b + (long)row * ld;
   const long g0 = (long)c * OZ_CH;
-  constexpr int PER = OZ_CH / 256;
-  double v[PER];
+  constexpr int GRP = OZ_CH / (256 * 4);
+  const bool vec = ((reinterpret_cast<uintptr_t>(src) & 15) == 0);
+  double v[GRP][4];
   double mx = 0.0;
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const long g = g0 + e * 256 + threadIdx.x;
-    v[e] = g < g2 ? (double)src[g] : 0.0;
-    mx = fmax(mx, fabs(v[e]));
+  for (int q = 0; q < GRP; ++q) {
+    load4(src, vec, g0 + ((long)q * 256 + threadIdx.x) * 4, g2, v[q]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) mx = fmax(mx, fabs(v[q][j]));
   }
   mx = block_max(mx, red);
   int ex = 0;
   if (mx > 0.0) frexp(mx, &ex);  // mx < 2^ex
   if (threadIdx.x == 0) E[(long)row * nchunk + c] = ex;
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const long g = g0 + e * 256 + threadIdx.x;
-    double x = ldexp(v[e], -ex);
+  for (int q = 0; q < GRP; ++q) {
+    const long g = g0 + ((long)q * 256 + threadIdx.x) * 4;
+    double x[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) x[j] = ldexp(v[q][j], -ex);
 #pragma unroll
     for (int p = 0; p < OZ_S; ++p) {
-      x *= 128.0;
-      const double q = trunc(x);
-      x -= q;
-      S[((long)p * k + row) * g2p + g] = (signed char)(int)q;
+      unsigned w = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        x[j] *= 128.0;
+        const double d = trunc(x[j]);
+        x[j] -= d;
+        w |= ((unsigned)(int)d & 0xffu) << (8 * j);
+      }
+      *reinterpret_cast<unsigned*>(S + ((long)p * k + row) * g2p + g) = w;
     }
   }
 }
@@ -200,22 +230,60 @@ __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S
       }
 }
 
-__global__ __launch_bounds__(256) void oz_reduce(const double* work, int nchunk, long ntiles, int k, double* v,
-                                                 long ldv) {
-  const long tile_lin = blockIdx.x;
+// V = sum over chunks (in chunk order) of the partial tiles, mirrored to the
+// upper triangle.  One workgroup per 32 x 32 block of a lower tile (16 per
+// tile): thread t owns row t / 8, columns 4 (t % 8) .. + 3, four independent
+// chunk-ordered sums; the mirror image is written through an LDS transpose so
+// both stores are row-contiguous.
+constexpr int RB = 32;
+__global__ __launch_bounds__(256) void oz_reduce(const double* __restrict__ work, int nchunk, long ntiles, int k,
+                                                 double* __restrict__ v, long ldv) {
+  __shared__ double S[RB][RB + 1];
+  constexpr int SUB = OZ_T / RB;
+  const long tile_lin = blockIdx.x / (SUB * SUB);
+  const int sb = blockIdx.x % (SUB * SUB), sr = sb / SUB, sc = sb % SUB;
   int tm = (int)((sqrt(8.0 * tile_lin + 1.0) - 1.0) * 0.5);
   while ((long)(tm + 1) * (tm + 2) / 2 <= tile_lin) ++tm;
   while ((long)tm * (tm + 1) / 2 > tile_lin) --tm;
   const int tn = (int)(tile_lin - (long)tm * (tm + 1) / 2);
-  for (int e = threadIdx.x; e < OZ_T * OZ_T; e += 256) {
-    const int r = e / OZ_T, c = e % OZ_T;
-    const long gr = (long)tm * OZ_T + r, gc = (long)tn * OZ_T + c;
-    if (gr >= k || gc >= k || gc > gr) continue;
-    double s = 0.0;
-    for (int ch = 0; ch < nchunk; ++ch) s += work[((long)ch * ntiles + tile_lin) * (OZ_T * OZ_T) + e];
-    v[gr * ldv + gc] = s;
-    v[gc * ldv + gr] = s;
+  const int r = threadIdx.x >> 3, c0 = (threadIdx.x & 7) * 4;
+  const int rl = sr * RB + r, cl = sc * RB + c0;  // within the tile
+  const double* src = work + tile_lin * (OZ_T * OZ_T) + rl * OZ_T + cl;
+  const long cstride = ntiles * (OZ_T * OZ_T);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int ch = 0;
+  for (; ch + 4 <= nchunk; ch += 4) {
+    double4 q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = *reinterpret_cast<const double4*>(src + (long)(ch + u) * cstride);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s0 += q[u].x;
+      s1 += q[u].y;
+      s2 += q[u].z;
+      s3 += q[u].w;
+    }
   }
+  for (; ch < nchunk; ++ch) {
+    const double4 q = *reinterpret_cast<const double4*>(src + (long)ch * cstride);
+    s0 += q.x;
+    s1 += q.y;
+    s2 += q.z;
+    s3 += q.w;
+  }
+  const long gr = (long)tm * OZ_T + rl, gc = (long)tn * OZ_T + cl;
+  const double sv[4] = {s0, s1, s2, s3};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    S[r][c0 + u] = sv[u];
+    if (gr < k && gc + u < k && gc + u <= gr) v[gr * ldv + gc + u] = sv[u];
+  }
+  __syncthreads();
+  // mirror: row gc' = tn T + sc RB + r of the upper triangle, columns gr' = tm T + sr RB + c0 .. + 3
+  const long mr = (long)tn * OZ_T + sc * RB + r, mc = (long)tm * OZ_T + sr * RB + c0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (mr < k && mc + u < k && mr < mc + u) v[mr * ldv + mc + u] = S[c0 + u][r];
 }
 
 void oz_plan(int k, long g2, int& nchunk, long& g2p, long& ntiles) {
@@ -257,7 +325,8 @@ int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ld
   if (slots > 0x7fffffffL) return THC_ERR_ARG;
   hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, S, g2p, k, E, nchunk, part, ntiles);
   THC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(oz_reduce, dim3((unsigned)ntiles), dim3(256), 0, s, part, nchunk, ntiles, k, v, ldv);
+  hipLaunchKernelGGL(oz_reduce, dim3((unsigned)(ntiles * (OZ_T / RB) * (OZ_T / RB))), dim3(256), 0, s, part, nchunk,
+                     ntiles, k, v, ldv);
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
