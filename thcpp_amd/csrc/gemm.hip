@@ -413,6 +413,7 @@ struct Cfg {
 };
 typedef Cfg<128, 128, 16, 2, 4, 4, 2> CfgLarge;  // 8 waves of 32 x 64
 typedef Cfg<64, 64, 16, 2, 2, 2, 2> CfgSmall;    // 4 waves of 32 x 32
+typedef Cfg<32, 32, 16, 2, 2, 1, 1> CfgTiny;     // 1 wave of 32 x 32: four times the workgroups
 
 template <class T, class C, int EPI, bool AK, bool BKc, bool GA, bool GB>
 void launch_vec(const KArgs& p, bool vec, dim3 grid, hipStream_t s) {
@@ -485,6 +486,11 @@ int gemm_dispatch(const GemmArgs& g, hipStream_t s) {
   if (g.m <= 0 || g.n <= 0) return THC_OK;
   if (g.lower && g.m != g.n) return THC_ERR_ARG;
   const long tiles128 = ((g.m + 127) / 128) * ((g.n + 127) / 128) * g.batch;
+  long tiles64 = ((g.m + 63) / 64) * ((g.n + 63) / 64);
+  if (g.lower) tiles64 = (tiles64 + (g.m + 63) / 64) / 2;
+  tiles64 *= g.batch;
+  // too few 64 x 64 tiles to fill 256 CUs a few times over: single-wave tiles
+  if (tiles64 < 1024) return launch<T, CfgTiny>(g, s);
   if (tiles128 < 512) return launch<T, CfgSmall>(g, s);
   return launch<T, CfgLarge>(g, s);
 }
