@@ -383,7 +383,28 @@ __global__ __launch_bounds__(256) void assign(const double* sx, const double* sy
     int bj = cidx[0];
     double bd2 = dist2(ccx[0], ccy[0], ccz[0], gx, gy, gz);
     double bs = sqrt(bd2);
-    for (int c = 1; c < ncand; ++c) {
+    // eight candidates' distances at a time (their LDS reads in flight
+    // together), then the updates in ascending order as one by one
+    constexpr int U = 8;
+    const int cached = min(ncand, CAND_CAP);
+    int c = 1;
+    for (; c + U <= cached; c += U) {
+      double d2v[U];
+      int jv[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        jv[q] = cidx[c + q];
+        d2v[q] = dist2(ccx[c + q], ccy[c + q], ccz[c + q], gx, gy, gz);
+      }
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        if (d2v[q] < bd2) {
+          const double sq = sqrt(d2v[q]);
+          if (sq < bs) { bs = sq; bd2 = d2v[q]; bj = jv[q]; }
+        }
+      }
+    }
+    for (; c < ncand; ++c) {
       double cx, cy, cz;
       const int j = cidx[c];
       if (c < CAND_CAP) { cx = ccx[c]; cy = ccy[c]; cz = ccz[c]; }
