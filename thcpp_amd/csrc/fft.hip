@@ -69,6 +69,49 @@ struct FDiv {
   __device__ int div(int e) const { return d > 1 ? (int)__umulhi((unsigned)e, m) : e; }
 };
 
+// Radix-R butterfly of one Stockham stage: twiddles w^(j q tstep) on the
+// inputs (ns > 1), then the R-point DFT.
+template <class T, int R>
+__device__ inline void butterfly(C2<T> (&x)[R], C2<T> (&y)[R], int n, int ns, int j, int tstep,
+                                 const C2<T>* __restrict__ tw) {
+  if (ns > 1) {
+#pragma unroll
+    for (int q = 1; q < R; ++q) x[q] = cmul(x[q], tw[j * q * tstep]);
+  }
+  if (R == 2) {
+    y[0] = cadd(x[0], x[1]);
+    y[1] = csub(x[0], x[1]);
+  } else if (R == 4) {
+    dft4(x[0], x[1], x[2], x[3], y[0], y[1], y[2], y[3]);
+  } else if (R == 8) {
+    // even / odd halves by radix 4, joined with w8^p = e^{-2 pi i p / 8}
+    C2<T> e[4], o[4];
+    dft4(x[0], x[2], x[4], x[6], e[0], e[1], e[2], e[3]);
+    dft4(x[1], x[3], x[5], x[7], o[0], o[1], o[2], o[3]);
+    const T h = T(0.70710678118654752440);
+    const C2<T> w1 = C2<T>{(o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h};  // w8 o1
+    const C2<T> w2 = C2<T>{o[2].y, -o[2].x};                                // -i o2
+    const C2<T> w3 = C2<T>{(o[3].y - o[3].x) * h, -(o[3].x + o[3].y) * h};  // w8^3 o3
+    y[0] = cadd(e[0], o[0]);
+    y[4] = csub(e[0], o[0]);
+    y[1] = cadd(e[1], w1);
+    y[5] = csub(e[1], w1);
+    y[2] = cadd(e[2], w2);
+    y[6] = csub(e[2], w2);
+    y[3] = cadd(e[3], w3);
+    y[7] = csub(e[3], w3);
+  } else {
+    const int step = n / R;
+#pragma unroll
+    for (int p = 0; p < R; ++p) {
+      C2<T> acc = x[0];
+#pragma unroll
+      for (int q = 1; q < R; ++q) acc = cadd(acc, cmul(x[q], tw[((p * q) % R) * step]));
+      y[p] = acc;
+    }
+  }
+}
+
 // One Stockham stage over 'nlines' lines of length n, read src write dst.
 template <class T, int R>
 __device__ inline void stage(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, int n, int ns, int nlines,
@@ -81,46 +124,10 @@ __device__ inline void stage(const C2<T>* __restrict__ src, C2<T>* __restrict__ 
     const int line = fnb.div(w), t = w - line * nb;
     const int base = line * ls;
     const int tq = fns.div(t), j = t - tq * ns;
-    C2<T> x[R];
+    C2<T> x[R], y[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) x[q] = src[base + (t + q * nb) * es];
-    if (ns > 1) {
-#pragma unroll
-      for (int q = 1; q < R; ++q) x[q] = cmul(x[q], tw[j * q * tstep]);
-    }
-    C2<T> y[R];
-    if (R == 2) {
-      y[0] = cadd(x[0], x[1]);
-      y[1] = csub(x[0], x[1]);
-    } else if (R == 4) {
-      dft4(x[0], x[1], x[2], x[3], y[0], y[1], y[2], y[3]);
-    } else if (R == 8) {
-      // even / odd halves by radix 4, joined with w8^p = e^{-2 pi i p / 8}
-      C2<T> e[4], o[4];
-      dft4(x[0], x[2], x[4], x[6], e[0], e[1], e[2], e[3]);
-      dft4(x[1], x[3], x[5], x[7], o[0], o[1], o[2], o[3]);
-      const T h = T(0.70710678118654752440);
-      const C2<T> w1 = C2<T>{(o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h};  // w8 o1
-      const C2<T> w2 = C2<T>{o[2].y, -o[2].x};                                // -i o2
-      const C2<T> w3 = C2<T>{(o[3].y - o[3].x) * h, -(o[3].x + o[3].y) * h};  // w8^3 o3
-      y[0] = cadd(e[0], o[0]);
-      y[4] = csub(e[0], o[0]);
-      y[1] = cadd(e[1], w1);
-      y[5] = csub(e[1], w1);
-      y[2] = cadd(e[2], w2);
-      y[6] = csub(e[2], w2);
-      y[3] = cadd(e[3], w3);
-      y[7] = csub(e[3], w3);
-    } else {
-      const int step = n / R;
-#pragma unroll
-      for (int p = 0; p < R; ++p) {
-        C2<T> acc = x[0];
-#pragma unroll
-        for (int q = 1; q < R; ++q) acc = cadd(acc, cmul(x[q], tw[((p * q) % R) * step]));
-        y[p] = acc;
-      }
-    }
+    butterfly<T, R>(x, y, n, ns, j, tstep, tw);
     const int d0 = tq * ns * R + j;
 #pragma unroll
     for (int q = 0; q < R; ++q) dst[base + (d0 + q * ns) * es] = y[q];
@@ -145,16 +152,83 @@ __device__ C2<T>* fft_lines(C2<T>* a, C2<T>* b, const Plan1D& pl, int nlines, in
   return a;
 }
 
+// In-place variant: each thread reads the inputs of all its butterflies (at
+// most IPV / R of them) into registers, then after a barrier writes the
+// outputs into the same buffer.  One LDS buffer instead of two lets a CU hold
+// twice the workgroups, which is what these latency-bound passes need.
+constexpr int IPV = 24;  // complex values per thread per in-place stage
+constexpr int FT = 256;  // threads per workgroup of both passes
+
+template <class T, int R>
+__device__ inline void stage_ip(C2<T>* buf, int n, int ns, int nlines, int es, int ls, const C2<T>* __restrict__ tw) {
+  constexpr int NB = IPV / R;
+  const int nb = n / R;
+  const int total = nlines * nb;
+  const int tstep = n / (ns * R);
+  const FDiv fnb(nb), fns(ns);
+  C2<T> x[NB][R];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int w = threadIdx.x + i * FT;
+    if (w < total) {
+      const int line = fnb.div(w), t = w - line * nb;
+#pragma unroll
+      for (int q = 0; q < R; ++q) x[i][q] = buf[line * ls + (t + q * nb) * es];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int w = threadIdx.x + i * FT;
+    if (w < total) {
+      const int line = fnb.div(w), t = w - line * nb;
+      const int tq = fns.div(t), j = t - tq * ns;
+      C2<T> y[R];
+      butterfly<T, R>(x[i], y, n, ns, j, tstep, tw);
+      const int d0 = tq * ns * R + j;
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[line * ls + (d0 + q * ns) * es] = y[q];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T>
+__device__ void fft_lines_ip(C2<T>* buf, const Plan1D& pl, int nlines, int es, int ls, const C2<T>* tw) {
+  int ns = 1;
+  for (int s = 0; s < pl.nst; ++s) {
+    const int r = pl.rad[s];
+    if (r == 8) stage_ip<T, 8>(buf, pl.n, ns, nlines, es, ls, tw);
+    else if (r == 4) stage_ip<T, 4>(buf, pl.n, ns, nlines, es, ls, tw);
+    else if (r == 2) stage_ip<T, 2>(buf, pl.n, ns, nlines, es, ls, tw);
+    else if (r == 3) stage_ip<T, 3>(buf, pl.n, ns, nlines, es, ls, tw);
+    else stage_ip<T, 5>(buf, pl.n, ns, nlines, es, ls, tw);
+    ns *= r;
+  }
+}
+
+// Whether every stage of a plan over nlines lines keeps to IPV values per thread.
+bool fits_ip(const Plan1D& pl, int nlines) {
+  for (int s = 0; s < pl.nst; ++s) {
+    const int r = pl.rad[s];
+    const long per = ((long)nlines * (pl.n / r) + FT - 1) / FT;
+    if (per > IPV / r) return false;
+  }
+  return true;
+}
+constexpr int UPV = 12;  // elements per thread of the in-place unpack
+
 // x: (k, n0, n1, n2) real FP64 (rounded to T as it is read); t: (k, n0, n1,
 // n2h) complex.
 template <class T>
-__global__ __launch_bounds__(256) void pass_a(const double* __restrict__ x, C2<T>* __restrict__ t, FftParams p) {
+__global__ __launch_bounds__(FT) void pass_a(const double* __restrict__ x, C2<T>* __restrict__ t, FftParams p,
+                                             int ip) {
   extern __shared__ unsigned char smem_raw[];
   const int n1 = p.n1, n2 = p.n2, n2h = p.n2h;
   const int n1p = (n1 + 1) / 2;
   const int cap = max(n1p * n2, n1 * n2h);
   C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
-  C2<T>* bufb = bufa + cap;
+  C2<T>* bufb = ip ? bufa : bufa + cap;  // in-place stages use one buffer
   C2<T>* twz = bufb + cap;  // twiddle tables staged in LDS (each stage reads them)
   C2<T>* twy = twz + n2;
   const long mu = blockIdx.y, ix = blockIdx.x;
@@ -183,33 +257,62 @@ __global__ __launch_bounds__(256) void pass_a(const double* __restrict__ x, C2<T
     }
   }
   __syncthreads();
-  C2<T>* res = fft_lines<T>(bufa, bufb, p.pz, n1p, 1, n2, twz);
-  C2<T>* h = (res == bufa) ? bufb : bufa;
   // unpack: A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / (2i)
-  for (int e = threadIdx.x; e < n1p * n2h; e += blockDim.x) {
-    const int r = fn2h.div(e), kz = e - r * n2h;
-    const C2<T> z1 = res[r * n2 + kz];
-    const C2<T> z2 = res[r * n2 + (kz == 0 ? 0 : n2 - kz)];
-    const C2<T> a = C2<T>{(z1.x + z2.x) * T(0.5), (z1.y - z2.y) * T(0.5)};
-    const C2<T> bb = C2<T>{(z1.y + z2.y) * T(0.5), (z2.x - z1.x) * T(0.5)};
-    h[(2 * r) * n2h + kz] = a;
-    if (2 * r + 1 < n1) h[(2 * r + 1) * n2h + kz] = bb;
+  C2<T>* fin;
+  if (ip) {
+    fft_lines_ip<T>(bufa, p.pz, n1p, 1, n2, twz);
+    C2<T> z1[UPV], z2[UPV];
+#pragma unroll
+    for (int i = 0; i < UPV; ++i) {
+      const int e = threadIdx.x + i * FT;
+      if (e < n1p * n2h) {
+        const int r = fn2h.div(e), kz = e - r * n2h;
+        z1[i] = bufa[r * n2 + kz];
+        z2[i] = bufa[r * n2 + (kz == 0 ? 0 : n2 - kz)];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < UPV; ++i) {
+      const int e = threadIdx.x + i * FT;
+      if (e < n1p * n2h) {
+        const int r = fn2h.div(e), kz = e - r * n2h;
+        bufa[(2 * r) * n2h + kz] = C2<T>{(z1[i].x + z2[i].x) * T(0.5), (z1[i].y - z2[i].y) * T(0.5)};
+        if (2 * r + 1 < n1)
+          bufa[(2 * r + 1) * n2h + kz] = C2<T>{(z1[i].y + z2[i].y) * T(0.5), (z2[i].x - z1[i].x) * T(0.5)};
+      }
+    }
+    __syncthreads();
+    fft_lines_ip<T>(bufa, p.py, n2h, n2h, 1, twy);
+    fin = bufa;
+  } else {
+    C2<T>* res = fft_lines<T>(bufa, bufb, p.pz, n1p, 1, n2, twz);
+    C2<T>* h = (res == bufa) ? bufb : bufa;
+    for (int e = threadIdx.x; e < n1p * n2h; e += blockDim.x) {
+      const int r = fn2h.div(e), kz = e - r * n2h;
+      const C2<T> z1 = res[r * n2 + kz];
+      const C2<T> z2 = res[r * n2 + (kz == 0 ? 0 : n2 - kz)];
+      const C2<T> a = C2<T>{(z1.x + z2.x) * T(0.5), (z1.y - z2.y) * T(0.5)};
+      const C2<T> bb = C2<T>{(z1.y + z2.y) * T(0.5), (z2.x - z1.x) * T(0.5)};
+      h[(2 * r) * n2h + kz] = a;
+      if (2 * r + 1 < n1) h[(2 * r + 1) * n2h + kz] = bb;
+    }
+    __syncthreads();
+    C2<T>* other = (h == bufa) ? bufb : bufa;
+    fin = fft_lines<T>(h, other, p.py, n2h, n2h, 1, twy);
   }
-  __syncthreads();
-  C2<T>* other = (h == bufa) ? bufb : bufa;
-  C2<T>* fin = fft_lines<T>(h, other, p.py, n2h, n2h, 1, twy);
   C2<T>* dst = t + (mu * p.n0 + ix) * (long)n1 * n2h;
   for (int e = threadIdx.x; e < n1 * n2h; e += blockDim.x) dst[e] = fin[e];
 }
 
 // t: (k, n0, n1, n2h) complex -> out (k, n0, n1, n2h, 2) weighted.
 template <class T>
-__global__ __launch_bounds__(256) void pass_b(const C2<T>* __restrict__ t, const double* __restrict__ coulG,
-                                              T* __restrict__ out, FftParams p) {
+__global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const double* __restrict__ coulG,
+                                             T* __restrict__ out, FftParams p, int ip) {
   extern __shared__ unsigned char smem_raw[];
   const int n0 = p.n0, n1 = p.n1, n2 = p.n2, n2h = p.n2h;
   C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
-  C2<T>* bufb = bufa + n0 * n2h;
+  C2<T>* bufb = ip ? bufa : bufa + n0 * n2h;
   C2<T>* twx = bufb + n0 * n2h;  // twiddle table staged in LDS
   for (int j = threadIdx.x; j < n0; j += blockDim.x) twx[j] = reinterpret_cast<const C2<T>*>(p.twx)[j];
   const long mu = blockIdx.y;
@@ -233,7 +336,9 @@ __global__ __launch_bounds__(256) void pass_b(const C2<T>* __restrict__ t, const
     }
   }
   __syncthreads();
-  C2<T>* fin = fft_lines<T>(bufa, bufb, p.px, n2h, n2h, 1, twx);
+  C2<T>* fin = bufa;
+  if (ip) fft_lines_ip<T>(bufa, p.px, n2h, n2h, 1, twx);
+  else fin = fft_lines<T>(bufa, bufb, p.px, n2h, n2h, 1, twx);
   T* dst = out + mu * (long)n0 * n1 * n2h * 2;
   for (int e = threadIdx.x; e < n0 * n2h; e += blockDim.x) {
     const int kx = fn2h.div(e), kz = e - kx * n2h;
@@ -273,13 +378,13 @@ constexpr int MAXN = 128;
 
 // two ping-pong buffers plus the twiddle tables of the pass
 template <class T>
-size_t lds_a(int n1, int n2) {
+size_t lds_a(int n1, int n2, bool ip) {
   const int n2h = n2 / 2 + 1, n1p = (n1 + 1) / 2;
-  return 2 * sizeof(C2<T>) * (size_t)std::max(n1p * n2, n1 * n2h) + sizeof(C2<T>) * (size_t)(n1 + n2);
+  return (ip ? 1 : 2) * sizeof(C2<T>) * (size_t)std::max(n1p * n2, n1 * n2h) + sizeof(C2<T>) * (size_t)(n1 + n2);
 }
 template <class T>
-size_t lds_b(int n0, int n2) {
-  return 2 * sizeof(C2<T>) * (size_t)n0 * (n2 / 2 + 1) + sizeof(C2<T>) * (size_t)n0;
+size_t lds_b(int n0, int n2, bool ip) {
+  return (ip ? 1 : 2) * sizeof(C2<T>) * (size_t)n0 * (n2 / 2 + 1) + sizeof(C2<T>) * (size_t)n0;
 }
 
 template <class T>
@@ -287,7 +392,12 @@ int run(const void* x, int k, const int n[3], const double* coulG, void* out, vo
   FftParams p;
   p.k = k; p.n0 = n[0]; p.n1 = n[1]; p.n2 = n[2]; p.n2h = n[2] / 2 + 1;
   if (!make_plan(n[2], p.pz) || !make_plan(n[1], p.py) || !make_plan(n[0], p.px)) return THC_ERR_UNSUPPORTED;
-  const size_t la = lds_a<T>(n[1], n[2]), lb = lds_b<T>(n[0], n[2]);
+  // in-place stages (one LDS buffer) when every stage and the unpack keep to
+  // their per-thread register budgets
+  const int n1p = (n[1] + 1) / 2, n2h = n[2] / 2 + 1;
+  const bool ipa = fits_ip(p.pz, n1p) && fits_ip(p.py, n2h) && ((long)n1p * n2h + FT - 1) / FT <= UPV;
+  const bool ipb = fits_ip(p.px, n2h);
+  const size_t la = lds_a<T>(n[1], n[2], ipa), lb = lds_b<T>(n[0], n[2], ipb);
   if (la > 160 * 1024 || lb > 160 * 1024) return THC_ERR_UNSUPPORTED;
   // twiddle tables at the head of the workspace, then the intermediate spectrum
   char* w = reinterpret_cast<char*>(work);
@@ -303,9 +413,11 @@ int run(const void* x, int k, const int n[3], const double* coulG, void* out, vo
   p.twz = dz; p.twy = dy; p.twx = dx;
   THC_HIP(hipFuncSetAttribute((const void*)pass_a<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la));
   THC_HIP(hipFuncSetAttribute((const void*)pass_b<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
-  hipLaunchKernelGGL(pass_a<T>, dim3(n[0], k), dim3(256), la, s, reinterpret_cast<const double*>(x), tmid, p);
+  hipLaunchKernelGGL(pass_a<T>, dim3(n[0], k), dim3(FT), la, s, reinterpret_cast<const double*>(x), tmid, p,
+                     (int)ipa);
   THC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(pass_b<T>, dim3(n[1], k), dim3(256), lb, s, tmid, coulG, reinterpret_cast<T*>(out), p);
+  hipLaunchKernelGGL(pass_b<T>, dim3(n[1], k), dim3(FT), lb, s, tmid, coulG, reinterpret_cast<T*>(out), p,
+                     (int)ipb);
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
