@@ -155,13 +155,14 @@ __device__ C2<T>* fft_lines(C2<T>* a, C2<T>* b, const Plan1D& pl, int nlines, in
 // In-place variant: each thread reads the inputs of all its butterflies (at
 // most IPV / R of them) into registers, then after a barrier writes the
 // outputs into the same buffer.  One LDS buffer instead of two lets a CU hold
-// twice the workgroups, which is what these latency-bound passes need.
-constexpr int IPV = 24;  // complex values per thread per in-place stage
+// up to twice the workgroups, which is what these latency-bound passes need;
+// the smaller IPV budget (16 complex values) keeps FP32 passes at about 64
+// VGPRs and seven workgroups per CU, 24 covers 96-point lines.
 constexpr int FT = 256;  // threads per workgroup of both passes
 
-template <class T, int R>
+template <class T, int R, int IPV>
 __device__ inline void stage_ip(C2<T>* buf, int n, int ns, int nlines, int es, int ls, const C2<T>* __restrict__ tw) {
-  constexpr int NB = IPV / R;
+  constexpr int NB = IPV / R > 0 ? IPV / R : 1;
   const int nb = n / R;
   const int total = nlines * nb;
   const int tstep = n / (ns * R);
@@ -193,26 +194,26 @@ __device__ inline void stage_ip(C2<T>* buf, int n, int ns, int nlines, int es, i
   __syncthreads();
 }
 
-template <class T>
+template <class T, int IPV>
 __device__ void fft_lines_ip(C2<T>* buf, const Plan1D& pl, int nlines, int es, int ls, const C2<T>* tw) {
   int ns = 1;
   for (int s = 0; s < pl.nst; ++s) {
     const int r = pl.rad[s];
-    if (r == 8) stage_ip<T, 8>(buf, pl.n, ns, nlines, es, ls, tw);
-    else if (r == 4) stage_ip<T, 4>(buf, pl.n, ns, nlines, es, ls, tw);
-    else if (r == 2) stage_ip<T, 2>(buf, pl.n, ns, nlines, es, ls, tw);
-    else if (r == 3) stage_ip<T, 3>(buf, pl.n, ns, nlines, es, ls, tw);
-    else stage_ip<T, 5>(buf, pl.n, ns, nlines, es, ls, tw);
+    if (r == 8) stage_ip<T, 8, IPV>(buf, pl.n, ns, nlines, es, ls, tw);
+    else if (r == 4) stage_ip<T, 4, IPV>(buf, pl.n, ns, nlines, es, ls, tw);
+    else if (r == 2) stage_ip<T, 2, IPV>(buf, pl.n, ns, nlines, es, ls, tw);
+    else if (r == 3) stage_ip<T, 3, IPV>(buf, pl.n, ns, nlines, es, ls, tw);
+    else stage_ip<T, 5, IPV>(buf, pl.n, ns, nlines, es, ls, tw);
     ns *= r;
   }
 }
 
-// Whether every stage of a plan over nlines lines keeps to IPV values per thread.
-bool fits_ip(const Plan1D& pl, int nlines) {
+// Whether every stage of a plan over nlines lines keeps to ipv values per thread.
+bool fits_ip(const Plan1D& pl, int nlines, int ipv) {
   for (int s = 0; s < pl.nst; ++s) {
     const int r = pl.rad[s];
     const long per = ((long)nlines * (pl.n / r) + FT - 1) / FT;
-    if (per > IPV / r) return false;
+    if (per > ipv / r) return false;
   }
   return true;
 }
@@ -220,9 +221,9 @@ constexpr int UPV = 12;  // elements per thread of the in-place unpack
 
 // x: (k, n0, n1, n2) real FP64 (rounded to T as it is read); t: (k, n0, n1,
 // n2h) complex.
-template <class T>
-__global__ __launch_bounds__(FT) void pass_a(const double* __restrict__ x, C2<T>* __restrict__ t, FftParams p,
-                                             int ip) {
+template <class T, int IPV>  // IPV = 0: ping-pong stages
+__global__ __launch_bounds__(FT) void pass_a(const double* __restrict__ x, C2<T>* __restrict__ t, FftParams p) {
+  constexpr bool ip = IPV > 0;
   extern __shared__ unsigned char smem_raw[];
   const int n1 = p.n1, n2 = p.n2, n2h = p.n2h;
   const int n1p = (n1 + 1) / 2;
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(FT) void pass_a(const double* __restrict__ x, C2<T>
   // unpack: A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / (2i)
   C2<T>* fin;
   if (ip) {
-    fft_lines_ip<T>(bufa, p.pz, n1p, 1, n2, twz);
+    fft_lines_ip<T, IPV>(bufa, p.pz, n1p, 1, n2, twz);
     C2<T> z1[UPV], z2[UPV];
 #pragma unroll
     for (int i = 0; i < UPV; ++i) {
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(FT) void pass_a(const double* __restrict__ x, C2<T>
       }
     }
     __syncthreads();
-    fft_lines_ip<T>(bufa, p.py, n2h, n2h, 1, twy);
+    fft_lines_ip<T, IPV>(bufa, p.py, n2h, n2h, 1, twy);
     fin = bufa;
   } else {
     C2<T>* res = fft_lines<T>(bufa, bufb, p.pz, n1p, 1, n2, twz);
@@ -306,9 +307,10 @@ __global__ __launch_bounds__(FT) void pass_a(const double* __restrict__ x, C2<T>
 }
 
 // t: (k, n0, n1, n2h) complex -> out (k, n0, n1, n2h, 2) weighted.
-template <class T>
+template <class T, int IPV>  // IPV = 0: ping-pong stages
 __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const double* __restrict__ coulG,
-                                             T* __restrict__ out, FftParams p, int ip) {
+                                             T* __restrict__ out, FftParams p) {
+  constexpr bool ip = IPV > 0;
   extern __shared__ unsigned char smem_raw[];
   const int n0 = p.n0, n1 = p.n1, n2 = p.n2, n2h = p.n2h;
   C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const 
   }
   __syncthreads();
   C2<T>* fin = bufa;
-  if (ip) fft_lines_ip<T>(bufa, p.px, n2h, n2h, 1, twx);
+  if (ip) fft_lines_ip<T, IPV>(bufa, p.px, n2h, n2h, 1, twx);
   else fin = fft_lines<T>(bufa, bufb, p.px, n2h, n2h, 1, twx);
   T* dst = out + mu * (long)n0 * n1 * n2h * 2;
   for (int e = threadIdx.x; e < n0 * n2h; e += blockDim.x) {
@@ -387,6 +389,22 @@ size_t lds_b(int n0, int n2, bool ip) {
   return (ip ? 1 : 2) * sizeof(C2<T>) * (size_t)n0 * (n2 / 2 + 1) + sizeof(C2<T>) * (size_t)n0;
 }
 
+template <class T, int IPV>
+int launch_a(dim3 g, size_t la, const double* x, C2<T>* tmid, const FftParams& p, hipStream_t s) {
+  THC_HIP(hipFuncSetAttribute((const void*)pass_a<T, IPV>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la));
+  hipLaunchKernelGGL((pass_a<T, IPV>), g, dim3(FT), la, s, x, tmid, p);
+  THC_HIP(hipGetLastError());
+  return THC_OK;
+}
+
+template <class T, int IPV>
+int launch_b(dim3 g, size_t lb, const C2<T>* tmid, const double* coulG, T* out, const FftParams& p, hipStream_t s) {
+  THC_HIP(hipFuncSetAttribute((const void*)pass_b<T, IPV>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
+  hipLaunchKernelGGL((pass_b<T, IPV>), g, dim3(FT), lb, s, tmid, coulG, out, p);
+  THC_HIP(hipGetLastError());
+  return THC_OK;
+}
+
 template <class T>
 int run(const void* x, int k, const int n[3], const double* coulG, void* out, void* work, hipStream_t s) {
   FftParams p;
@@ -395,9 +413,10 @@ int run(const void* x, int k, const int n[3], const double* coulG, void* out, vo
   // in-place stages (one LDS buffer) when every stage and the unpack keep to
   // their per-thread register budgets
   const int n1p = (n[1] + 1) / 2, n2h = n[2] / 2 + 1;
-  const bool ipa = fits_ip(p.pz, n1p) && fits_ip(p.py, n2h) && ((long)n1p * n2h + FT - 1) / FT <= UPV;
-  const bool ipb = fits_ip(p.px, n2h);
-  const size_t la = lds_a<T>(n[1], n[2], ipa), lb = lds_b<T>(n[0], n[2], ipb);
+  auto fa = [&](int v) { return fits_ip(p.pz, n1p, v) && fits_ip(p.py, n2h, v) && ((long)n1p * n2h + FT - 1) / FT <= UPV; };
+  const int ipa = fa(16) ? 16 : (fa(24) ? 24 : 0);
+  const int ipb = fits_ip(p.px, n2h, 16) ? 16 : (fits_ip(p.px, n2h, 24) ? 24 : 0);
+  const size_t la = lds_a<T>(n[1], n[2], ipa > 0), lb = lds_b<T>(n[0], n[2], ipb > 0);
   if (la > 160 * 1024 || lb > 160 * 1024) return THC_ERR_UNSUPPORTED;
   // twiddle tables at the head of the workspace, then the intermediate spectrum
   char* w = reinterpret_cast<char*>(work);
@@ -411,13 +430,14 @@ int run(const void* x, int k, const int n[3], const double* coulG, void* out, vo
   hipLaunchKernelGGL(twiddle_kernel<T>, dim3(1), dim3(128), 0, s, n[0], dx);
   THC_HIP(hipGetLastError());
   p.twz = dz; p.twy = dy; p.twx = dx;
-  THC_HIP(hipFuncSetAttribute((const void*)pass_a<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la));
-  THC_HIP(hipFuncSetAttribute((const void*)pass_b<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
-  hipLaunchKernelGGL(pass_a<T>, dim3(n[0], k), dim3(FT), la, s, reinterpret_cast<const double*>(x), tmid, p,
-                     (int)ipa);
-  THC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(pass_b<T>, dim3(n[1], k), dim3(FT), lb, s, tmid, coulG, reinterpret_cast<T*>(out), p,
-                     (int)ipb);
+  const double* xd = reinterpret_cast<const double*>(x);
+  T* o = reinterpret_cast<T*>(out);
+  if (ipa == 16) THC_TRY((launch_a<T, 16>(dim3(n[0], k), la, xd, tmid, p, s)));
+  else if (ipa == 24) THC_TRY((launch_a<T, 24>(dim3(n[0], k), la, xd, tmid, p, s)));
+  else THC_TRY((launch_a<T, 0>(dim3(n[0], k), la, xd, tmid, p, s)));
+  if (ipb == 16) THC_TRY((launch_b<T, 16>(dim3(n[1], k), lb, tmid, coulG, o, p, s)));
+  else if (ipb == 24) THC_TRY((launch_b<T, 24>(dim3(n[1], k), lb, tmid, coulG, o, p, s)));
+  else THC_TRY((launch_b<T, 0>(dim3(n[1], k), lb, tmid, coulG, o, p, s)));
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
