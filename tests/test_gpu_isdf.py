@@ -209,3 +209,22 @@ def test_isdf_mixed_uneven_sizes(ctx, n, m, k):
     out = ctx.isdf(a, idx, n, cg, precision=L.PREC_MIXED)
     assert out["status"] in (L.THC_OK, L.THC_WARN_LUV_NOT_POSDEF)
     assert rel(out["V"].cpu(), ref["V"].cpu()) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [L.PREC_MIXED, L.PREC_FP32])
+@pytest.mark.parametrize("n,m,k", [((15, 12, 10), 9, 37), ((40, 36, 30), 24, 150)])
+def test_isdf_f32_x_store_bitwise(ctx, prec, n, m, k):
+    """Below FP64 the solve stores X as FP32 when no xi output is asked for (the
+    FFT rounds X to FP32 as it reads it either way), so V with and without the
+    FP64 xi output is bitwise the same."""
+    from thcpp_amd import synthetic
+    rng = np.random.default_rng(11)
+    N = int(np.prod(n))
+    a = np.linalg.qr(rng.standard_normal((N, m)))[0]
+    idx = np.sort(rng.choice(N, k, replace=False)).astype(np.int32)
+    cg = synthetic.coulomb_kernel(6.5, n)
+    with_xi = ctx.isdf(a, idx, n, cg, precision=prec, want_xi=True)
+    without = ctx.isdf(a, idx, n, cg, precision=prec)
+    assert with_xi["status"] == without["status"]
+    assert with_xi["status"] in (L.THC_OK, L.THC_WARN_LUV_NOT_POSDEF)
+    assert torch.equal(with_xi["V"].cpu(), without["V"].cpu())

@@ -269,7 +269,10 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   // join: the solve needs (CC^T)^-1
   THC_HIP(hipStreamWaitEvent(s, ctx->join, 0));
   THC_HIP(hipEventRecord(ev[2], s));
-  // 3. X = CC^T^-1 ZC^T
+  // 3. X = CC^T^-1 ZC^T.  Below FP64 the FFT rounds X to FP32 as it reads it,
+  // so when nothing else reads X (no xi output, no refinement) the solve
+  // stores it as FP32 directly: the same values, half the bytes.
+  const int x_f32 = prec != THC_PREC_FP64 && xi_out == nullptr && refine == 0;
   {
     GemmArgs g = {};
     g.m = k; g.n = n; g.k = k;
@@ -277,6 +280,7 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
     g.b = B.z; g.ldb = k; g.b_trans = 1;
     g.c = B.x; g.ldc = n;
     g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
+    g.c_f32 = x_f32;
     THC_TRY(gemm_f64(g, s));
   }
   THC_HIP(hipEventRecord(ev[3], s));
@@ -301,11 +305,7 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   }
   THC_HIP(hipEventRecord(ev[4], s));
   // 4. weighted half spectrum
-  if (prec == THC_PREC_FP64) {
-    THC_TRY(fft_coulomb(B.x, k, fft, coulG, B.b, 0, B.fft_w, s));
-  } else {
-    THC_TRY(fft_coulomb(B.x, k, fft, coulG, B.b, 1, B.fft_w, s));
-  }
+  THC_TRY(fft_coulomb(B.x, x_f32, k, fft, coulG, B.b, prec == THC_PREC_FP64 ? 0 : 1, B.fft_w, s));
   THC_HIP(hipEventRecord(ev[5], s));
   // 5. V = B B^T, FP64 sum of the split-K partials (exact int8 digit products on the mixed path)
   if (prec == THC_PREC_MIXED) {
@@ -630,7 +630,7 @@ int thc_fft_vectors(thc_ctx* ctx, const double* x, int k, const int fft_grid[3],
   ctx->ws.reset();
   void* w = ctx->ws.take<char>(fft_work_bytes(k, fft_grid, 0));
   if (!w) return THC_ERR_NO_MEMORY;
-  THC_TRY(fft_coulomb(x, k, fft_grid, coulG, out, 0, w, ctx->stream));
+  THC_TRY(fft_coulomb(x, 0, k, fft_grid, coulG, out, 0, w, ctx->stream));
   THC_HIP(hipStreamSynchronize(ctx->stream));
   return THC_OK;
 }

@@ -108,6 +108,7 @@ struct GemmArgs {
   int epi_addsquare;
   int lower;
   int tri;       // GEMM_TRI_* bits: known zero blocks of A / B that bound each tile's k range
+  int c_f32;     // FP64 GEMM storing C as float (plain store, beta = 0)
   int batch;
   long stride_a, stride_b, stride_c;
 };
@@ -153,9 +154,9 @@ int qrcp_points(const double* phi, const double* occ, long n, int m, int mo, int
 
 // ---- FFT (fft.hip) ----------------------------------------------------------
 size_t fft_work_bytes(int k, const int n[3], int prec);
-// x: k x n0 n1 n2 double; arithmetic and out (k x g2 weighted half spectrum) in
-// double (prec 0) or float (prec 1)
-int fft_coulomb(const void* x, int k, const int n[3], const double* coulG, void* out, int prec,
+// x: k x n0 n1 n2, double or (x_f32, prec 1 only) float; arithmetic and out
+// (k x g2 weighted half spectrum) in double (prec 0) or float (prec 1)
+int fft_coulomb(const void* x, int x_f32, int k, const int n[3], const double* coulG, void* out, int prec,
                 void* work, hipStream_t s);
 inline long half_spectrum_len(const int n[3]) { return 2L * n[0] * n[1] * (n[2] / 2 + 1); }
 

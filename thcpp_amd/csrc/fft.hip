@@ -219,10 +219,10 @@ bool fits_ip(const Plan1D& pl, int nlines, int ipv) {
 }
 constexpr int UPV = 12;  // elements per thread of the in-place unpack
 
-// x: (k, n0, n1, n2) real FP64 (rounded to T as it is read); t: (k, n0, n1,
-// n2h) complex.
-template <class T, int IPV>  // IPV = 0: ping-pong stages
-__global__ __launch_bounds__(FT) void pass_a(const double* __restrict__ x, C2<T>* __restrict__ t, FftParams p) {
+// x: (k, n0, n1, n2) real, FP64 or FP32 (rounded to T as it is read); t: (k,
+// n0, n1, n2h) complex.
+template <class T, int IPV, class X>  // IPV = 0: ping-pong stages
+__global__ __launch_bounds__(FT) void pass_a(const X* __restrict__ x, C2<T>* __restrict__ t, FftParams p) {
   constexpr bool ip = IPV > 0;
   extern __shared__ unsigned char smem_raw[];
   const int n1 = p.n1, n2 = p.n2, n2h = p.n2h;
@@ -235,21 +235,21 @@ __global__ __launch_bounds__(FT) void pass_a(const double* __restrict__ x, C2<T>
   const long mu = blockIdx.y, ix = blockIdx.x;
   for (int j = threadIdx.x; j < n2; j += blockDim.x) twz[j] = reinterpret_cast<const C2<T>*>(p.twz)[j];
   for (int j = threadIdx.x; j < n1; j += blockDim.x) twy[j] = reinterpret_cast<const C2<T>*>(p.twy)[j];
-  const double* src = x + (mu * p.n0 + ix) * (long)n1 * n2;
+  const X* src = x + (mu * p.n0 + ix) * (long)n1 * n2;
   // pack rows (2r, 2r+1) as re/im; LB loads per thread are issued before the
   // first LDS store so the plane is fetched in one or two memory round trips
   constexpr int LB = 8;
   const int tot = n1p * n2;
   const FDiv fn2(n2), fn2h(n2h);
   for (int e0 = threadIdx.x; e0 < tot; e0 += LB * blockDim.x) {
-    double re[LB], im[LB];
+    X re[LB], im[LB];
 #pragma unroll
     for (int b = 0; b < LB; ++b) {
       const int e = e0 + b * (int)blockDim.x;
       const int r = fn2.div(e), z = e - r * n2;
       const bool ok = e < tot, ok1 = ok && 2 * r + 1 < n1;
       re[b] = src[ok ? (long)(2 * r) * n2 + z : 0];
-      im[b] = ok1 ? src[(long)(2 * r + 1) * n2 + z] : 0.0;
+      im[b] = ok1 ? src[(long)(2 * r + 1) * n2 + z] : X(0);
     }
 #pragma unroll
     for (int b = 0; b < LB; ++b) {
@@ -389,10 +389,10 @@ size_t lds_b(int n0, int n2, bool ip) {
   return (ip ? 1 : 2) * sizeof(C2<T>) * (size_t)n0 * (n2 / 2 + 1) + sizeof(C2<T>) * (size_t)n0;
 }
 
-template <class T, int IPV>
-int launch_a(dim3 g, size_t la, const double* x, C2<T>* tmid, const FftParams& p, hipStream_t s) {
-  THC_HIP(hipFuncSetAttribute((const void*)pass_a<T, IPV>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la));
-  hipLaunchKernelGGL((pass_a<T, IPV>), g, dim3(FT), la, s, x, tmid, p);
+template <class T, int IPV, class X>
+int launch_a(dim3 g, size_t la, const X* x, C2<T>* tmid, const FftParams& p, hipStream_t s) {
+  THC_HIP(hipFuncSetAttribute((const void*)pass_a<T, IPV, X>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la));
+  hipLaunchKernelGGL((pass_a<T, IPV, X>), g, dim3(FT), la, s, x, tmid, p);
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
@@ -405,8 +405,8 @@ int launch_b(dim3 g, size_t lb, const C2<T>* tmid, const double* coulG, T* out, 
   return THC_OK;
 }
 
-template <class T>
-int run(const void* x, int k, const int n[3], const double* coulG, void* out, void* work, hipStream_t s) {
+template <class T, class X>
+int run(const X* x, int k, const int n[3], const double* coulG, void* out, void* work, hipStream_t s) {
   FftParams p;
   p.k = k; p.n0 = n[0]; p.n1 = n[1]; p.n2 = n[2]; p.n2h = n[2] / 2 + 1;
   if (!make_plan(n[2], p.pz) || !make_plan(n[1], p.py) || !make_plan(n[0], p.px)) return THC_ERR_UNSUPPORTED;
@@ -430,11 +430,10 @@ int run(const void* x, int k, const int n[3], const double* coulG, void* out, vo
   hipLaunchKernelGGL(twiddle_kernel<T>, dim3(1), dim3(128), 0, s, n[0], dx);
   THC_HIP(hipGetLastError());
   p.twz = dz; p.twy = dy; p.twx = dx;
-  const double* xd = reinterpret_cast<const double*>(x);
   T* o = reinterpret_cast<T*>(out);
-  if (ipa == 16) THC_TRY((launch_a<T, 16>(dim3(n[0], k), la, xd, tmid, p, s)));
-  else if (ipa == 24) THC_TRY((launch_a<T, 24>(dim3(n[0], k), la, xd, tmid, p, s)));
-  else THC_TRY((launch_a<T, 0>(dim3(n[0], k), la, xd, tmid, p, s)));
+  if (ipa == 16) THC_TRY((launch_a<T, 16, X>(dim3(n[0], k), la, x, tmid, p, s)));
+  else if (ipa == 24) THC_TRY((launch_a<T, 24, X>(dim3(n[0], k), la, x, tmid, p, s)));
+  else THC_TRY((launch_a<T, 0, X>(dim3(n[0], k), la, x, tmid, p, s)));
   if (ipb == 16) THC_TRY((launch_b<T, 16>(dim3(n[1], k), lb, tmid, coulG, o, p, s)));
   else if (ipb == 24) THC_TRY((launch_b<T, 24>(dim3(n[1], k), lb, tmid, coulG, o, p, s)));
   else THC_TRY((launch_b<T, 0>(dim3(n[1], k), lb, tmid, coulG, o, p, s)));
@@ -450,13 +449,17 @@ size_t fft_work_bytes(int k, const int n[3], int prec) {
   return 3 * tb + align_up(2 * es * (size_t)k * n[0] * n[1] * (n[2] / 2 + 1), 256) + 256;
 }
 
-int fft_coulomb(const void* x, int k, const int n[3], const double* coulG, void* out, int prec, void* work,
-                hipStream_t s) {
+int fft_coulomb(const void* x, int x_f32, int k, const int n[3], const double* coulG, void* out, int prec,
+                void* work, hipStream_t s) {
   if (k <= 0) return THC_OK;
   for (int d = 0; d < 3; ++d)
     if (n[d] < 1 || n[d] > MAXN) return THC_ERR_UNSUPPORTED;
-  if (prec == 0) return run<double>(x, k, n, coulG, out, work, s);
-  return run<float>(x, k, n, coulG, out, work, s);
+  if (prec == 0) {
+    if (x_f32) return THC_ERR_ARG;  // FP64 arithmetic keeps FP64 input
+    return run<double>(reinterpret_cast<const double*>(x), k, n, coulG, out, work, s);
+  }
+  if (x_f32) return run<float>(reinterpret_cast<const float*>(x), k, n, coulG, out, work, s);
+  return run<float>(reinterpret_cast<const double*>(x), k, n, coulG, out, work, s);
 }
 
 }  // namespace thc

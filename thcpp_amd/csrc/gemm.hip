@@ -54,6 +54,7 @@ struct KArgs {
   double alpha, beta;
   int lower;
   int tri;
+  int c_f32;
   long stride_a, stride_b, stride_c;
   long kchunk;  // split-K chunk (EPI_PARTIAL)
 };
@@ -334,8 +335,11 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs p) {
         const long r = m0 + wm * WTM + i * 16 + Mfma<T>::row(lane, v);
         const long cc = n0 + wn * WTN + j * 16 + l16;
         if (r < p.m && cc < p.n && (!p.lower || cc <= r)) {
-          T* dst = p.c_trans ? Cb + cc * p.ldc + r : Cb + r * p.ldc + cc;
-          if (EPI == EPI_SQUARE) {
+          const long off = p.c_trans ? cc * p.ldc + r : r * p.ldc + cc;
+          T* dst = Cb + off;
+          if (EPI == EPI_STORE && p.c_f32) {
+            reinterpret_cast<float*>(p.c)[(long)bz * p.stride_c + off] = (float)(alpha * acc[i][j][v]);
+          } else if (EPI == EPI_SQUARE) {
             const T x = alpha * acc[i][j][v];
             *dst = x * x;
           } else if (EPI == EPI_HADAMARD) {
@@ -402,6 +406,7 @@ KArgs make_args(const GemmArgs& g) {
   p.alpha = g.alpha; p.beta = g.beta;
   p.lower = g.lower;
   p.tri = g.tri;
+  p.c_f32 = g.c_f32;
   p.stride_a = g.stride_a; p.stride_b = g.stride_b; p.stride_c = g.stride_c;
   return p;
 }
@@ -488,6 +493,8 @@ template <class T>
 int gemm_dispatch(const GemmArgs& g, hipStream_t s) {
   if (g.m <= 0 || g.n <= 0) return THC_OK;
   if (g.lower && g.m != g.n) return THC_ERR_ARG;
+  if (g.c_f32 && (sizeof(T) != 8 || g.beta != 0.0 || g.epi_square || g.epi_hadamard || g.epi_addsquare))
+    return THC_ERR_ARG;
   const long tiles128 = ((g.m + 127) / 128) * ((g.n + 127) / 128) * g.batch;
   long tiles64 = ((g.m + 63) / 64) * ((g.n + 63) / 64);
   if (g.lower) tiles64 = (tiles64 + (g.m + 63) / 64) / 2;
