@@ -212,7 +212,7 @@ def test_isdf_mixed_uneven_sizes(ctx, n, m, k):
 
 
 @pytest.mark.parametrize("prec", [L.PREC_MIXED, L.PREC_FP32])
-@pytest.mark.parametrize("n,m,k", [((15, 12, 10), 9, 37), ((40, 36, 30), 24, 150)])
+@pytest.mark.parametrize("n,m,k", [((15, 12, 10), 9, 37), ((40, 36, 30), 24, 150), ((40, 36, 30), 64, 272)])
 def test_isdf_f32_x_store_bitwise(ctx, prec, n, m, k):
     """Below FP64 the solve stores X as FP32 when no xi output is asked for (the
     FFT rounds X to FP32 as it reads it either way), so V with and without the
@@ -228,3 +228,27 @@ def test_isdf_f32_x_store_bitwise(ctx, prec, n, m, k):
     assert with_xi["status"] == without["status"]
     assert with_xi["status"] in (L.THC_OK, L.THC_WARN_LUV_NOT_POSDEF)
     assert torch.equal(with_xi["V"].cpu(), without["V"].cpu())
+
+
+@pytest.mark.parametrize("k", [256, 272])
+def test_isdf_solve_large_tiles(ctx, k):
+    """The solve at sizes that take the direct-to-LDS 128 x 128 x 16 FP64 tile (A
+    and B contiguous along k, k a multiple of 16, at least 512 tiles): k = 272
+    leaves a 16-row last tile row, N = 43200 a 64-column last tile column.  xi
+    against scipy's solve of CC^T X = ZC^T, within the conditioning of CC^T."""
+    from thcpp_amd import synthetic
+    n, m = (40, 36, 30), 64
+    N = int(np.prod(n))
+    rng = np.random.default_rng(k)
+    a = np.linalg.qr(rng.standard_normal((N, m)))[0]
+    idx = np.sort(rng.choice(N, k, replace=False)).astype(np.int32)
+    cg = synthetic.coulomb_kernel(6.5, n)
+    out = ctx.isdf(a, idx, n, cg, want_xi=True)
+    assert out["status"] in (L.THC_OK, L.THC_WARN_LUV_NOT_POSDEF)
+    z = IO.czt(a, idx)
+    c = IO.cct(z, idx)
+    xr = scipy.linalg.solve(c, z, assume_a="pos")
+    cond = np.linalg.cond(c)
+    e = rel(out["xi"].cpu().numpy(), xr)
+    print("k %d cond(CC^T) %.2e rel xi %.3e" % (k, cond, e))
+    assert e < max(1e-13, 1e-15 * cond) and e < 1e-6

@@ -357,6 +357,125 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs p) {
       }
 }
 
+// FP64 128 x 128 x 16 tile for A (m x k) and B (n x k) both contiguous along
+// k, k a multiple of 16: the tiles reach LDS by direct global-to-LDS loads
+// (global_load_lds_dwordx4: lane L of a wave writes bytes 16 L .. 16 L + 15 of
+// the wave's 1 KB block), so no registers stage them and no LDS stores wait on
+// the global loads.  An LDS tile row is 16 doubles (128 bytes, eight 16-byte
+// chunks, no padding); logical chunk c of row r sits at chunk c ^ ((r >> 1) & 7),
+// which keeps each 16-row fragment read free of bank conflicts.  The MFMA k
+// order is permuted so one 16-byte read feeds two k-steps: in half h, k-step s,
+// lane group l4 contributes k = 8 h + 2 l4 + s of both operands.  Rows past m
+// (n) read row m - 1 (n - 1); they only feed outputs that are never stored.
+typedef __attribute__((address_space(3))) void lds_void;
+constexpr int DTL_BM = 128, DTL_BN = 128, DTL_BK = 16, DTL_WN = 2, DTL_THREADS = 512;
+
+__global__ __launch_bounds__(DTL_THREADS) void gemm_dtl_kernel(KArgs p) {
+  constexpr int BM = DTL_BM, BN = DTL_BN, BK = DTL_BK, TM = 2, TN = 4, WN = DTL_WN;
+  constexpr int WTM = TM * 16, WTN = TN * 16;
+  static_assert((BM / 8) % (DTL_THREADS / 64) == 0 && BM == BN, "8-row blocks must split evenly over the waves");
+  // one variable per buffer: the compiler then knows that the direct loads
+  // into one buffer do not alias the fragment reads of the other, and does not
+  // wait for the loads before reading
+  __shared__ __attribute__((aligned(16))) double As0[BM * BK], As1[BM * BK];
+  __shared__ __attribute__((aligned(16))) double Bs0[BN * BK], Bs1[BN * BK];
+  int tm, tn;
+  if (!tile_coords(p, BM, BN, tm, tn)) return;
+  const long m0 = (long)tm * BM, n0 = (long)tn * BN;
+  const int bz = blockIdx.z;
+  const double* A = reinterpret_cast<const double*>(p.a) + (long)bz * p.stride_a;
+  const double* B = reinterpret_cast<const double*>(p.b) + (long)bz * p.stride_b;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int l16 = lane & 15, l4 = lane >> 4;
+
+  // wave w loads 8-row blocks w and w + 8 of each operand; lane L fills row
+  // 8 b + L / 8, physical chunk L % 8
+  constexpr int NQ = BM / 8 / (DTL_THREADS / 64);
+  const double* ap[NQ];
+  const double* bp[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int r = 8 * (wave + 8 * q) + (lane >> 3);
+    const int lc = (lane & 7) ^ ((r >> 1) & 7);
+    ap[q] = A + min(m0 + r, p.m - 1) * p.lda + 2 * lc;
+    bp[q] = B + min(n0 + r, p.n - 1) * p.ldb + 2 * lc;
+  }
+  auto issue = [&](long k0, double* as, double* bs) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = wave + 8 * q;
+      __builtin_amdgcn_global_load_lds(ap[q] + k0, (lds_void*)(as + b * 8 * BK), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(bp[q] + k0, (lds_void*)(bs + b * 8 * BK), 16, 0, 0);
+    }
+  };
+
+  d4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const long nk = p.k / BK;
+  // one tile: fragments from (as, bs) while the next tile streams into (na, nb)
+  auto step = [&](long t, const double* as, const double* bs, double* na, double* nb) {
+    if (t + 1 < nk) issue((t + 1) * BK, na, nb);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      double2 af[TM], bf[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int r = wm * WTM + i * 16 + l16;
+        af[i] = *reinterpret_cast<const double2*>(as + r * BK + 2 * ((4 * h + l4) ^ ((r >> 1) & 7)));
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int r = wn * WTN + j * 16 + l16;
+        bf[j] = *reinterpret_cast<const double2*>(bs + r * BK + 2 * ((4 * h + l4) ^ ((r >> 1) & 7)));
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc[i][j] = Mfma<double>::op(af[i].x, bf[j].x, acc[i][j]);
+          acc[i][j] = Mfma<double>::op(af[i].y, bf[j].y, acc[i][j]);
+        }
+    }
+    // this wave's next tiles have landed, and every wave is done reading the
+    // current ones before they are refilled
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  };
+  issue(0, As0, Bs0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (long t = 0; t < nk; t += 2) {
+    step(t, As0, Bs0, As1, Bs1);
+    if (t + 1 < nk) step(t + 1, As1, Bs1, As0, Bs0);
+  }
+
+  double* Cb = reinterpret_cast<double*>(p.c) + (long)bz * p.stride_c;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long r = m0 + wm * WTM + i * 16 + Mfma<double>::row(lane, v);
+        const long cc = n0 + wn * WTN + j * 16 + l16;
+        if (r < p.m && cc < p.n) {
+          const long off = p.c_trans ? cc * p.ldc + r : r * p.ldc + cc;
+          double x = p.alpha * acc[i][j][v];
+          if (p.c_f32) {
+            reinterpret_cast<float*>(p.c)[(long)bz * p.stride_c + off] = (float)x;
+          } else {
+            if (p.beta != 0.0) x += p.beta * Cb[off];
+            Cb[off] = x;
+          }
+        }
+      }
+}
+
 // V = FP64 sum over the split-K partials (in split order), mirrored to the
 // upper triangle: one workgroup per 32 x 32 block of a lower tile, thread t
 // owning row t / 8 and columns 4 (t % 8) .. + 3; the mirror image goes
@@ -489,6 +608,24 @@ int launch(const GemmArgs& g, hipStream_t s) {
   return launch_layout<T, C, EPI_STORE>(g, p, grid, s);
 }
 
+// The direct-to-LDS FP64 tile: plain store epilogue, A and B contiguous along
+// k with 16-byte aligned rows, k a multiple of the 16-deep tile.
+bool dtl_fits(const GemmArgs& g) {
+  return !g.lower && !g.tri && g.a_trans == 0 && g.b_trans == 1 && !g.a_rows && !g.b_rows && !g.epi_square &&
+         !g.epi_hadamard && !g.epi_addsquare && g.k > 0 && g.k % DTL_BK == 0 && g.lda % 2 == 0 &&
+         g.ldb % 2 == 0 && g.stride_a % 2 == 0 && g.stride_b % 2 == 0 && aligned16(g.a) && aligned16(g.b);
+}
+
+int launch_dtl(const GemmArgs& g, hipStream_t s) {
+  const KArgs p = make_args(g);
+  const long nblk = ((g.m + DTL_BM - 1) / DTL_BM) * ((g.n + DTL_BN - 1) / DTL_BN);
+  if (nblk <= 0 || g.batch <= 0) return THC_OK;
+  if (nblk > 0x7fffffffL) return THC_ERR_ARG;
+  hipLaunchKernelGGL(gemm_dtl_kernel, dim3((unsigned)nblk, 1, (unsigned)g.batch), dim3(DTL_THREADS), 0, s, p);
+  THC_HIP(hipGetLastError());
+  return THC_OK;
+}
+
 template <class T>
 int gemm_dispatch(const GemmArgs& g, hipStream_t s) {
   if (g.m <= 0 || g.n <= 0) return THC_OK;
@@ -508,6 +645,7 @@ int gemm_dispatch(const GemmArgs& g, hipStream_t s) {
   // too few 64 x 64 tiles to fill 256 CUs a few times over: single-wave tiles
   if (tiles64 < 1024) return launch<T, CfgTiny>(g, s);
   if (tiles128 < 512) return launch<T, CfgSmall>(g, s);
+  if (sizeof(T) == 8 && dtl_fits(g)) return launch_dtl(g, s);
   return launch<T, CfgLarge>(g, s);
 }
 
