@@ -420,6 +420,7 @@ __global__ __launch_bounds__(DTL_THREADS) void gemm_dtl_kernel(KArgs p) {
   // one tile: fragments from (as, bs) while the next tile streams into (na, nb)
   auto step = [&](long t, const double* as, const double* bs, double* na, double* nb) {
     if (t + 1 < nk) issue((t + 1) * BK, na, nb);
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of this tile's MFMAs
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       double2 af[TM], bf[TN];

@@ -121,11 +121,15 @@ typedef __attribute__((address_space(3))) void lds_void;
 
 __device__ inline int oz_seg(int r, int s) { return (s + (r >> 2)) & 3; }
 
-__global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S, long g2p, int k,
+__global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S, long g2, long g2p, int k,
                                                const int* __restrict__ E, int nchunk, double* __restrict__ work,
                                                long ntiles) {
-  __shared__ __attribute__((aligned(16))) signed char As[2][OZ_S][OZ_T][OZ_KS];
-  __shared__ __attribute__((aligned(16))) signed char Bs[2][OZ_S][OZ_T][OZ_KS];
+  // one variable per buffer: the compiler then knows that the direct loads
+  // into one buffer do not alias the fragment reads of the other, and does not
+  // wait for the loads before reading
+  constexpr int SLAB = OZ_S * OZ_T * OZ_KS;
+  __shared__ __attribute__((aligned(16))) signed char As0[SLAB], As1[SLAB];
+  __shared__ __attribute__((aligned(16))) signed char Bs0[SLAB], Bs1[SLAB];
   const long total = ntiles * nchunk;
   const long per_xcd = (total + 7) / 8;
   const long w = (long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
@@ -148,44 +152,40 @@ __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[d][i][j] = i4{0, 0, 0, 0};
 
-  // A slab is OZ_S * OZ_T rows of 64 bytes = 40 blocks of 16 rows (1 KB); wave
-  // wv loads blocks wv, wv + 8, ... of both operands.  Lane L fills row
-  // 16 b + L / 4 of the block, physical segment L % 4, from the logical segment
-  // that maps there.  Rows past k read row k - 1 (valid memory); they only feed
-  // output rows or columns >= k, which are never stored.
-  constexpr int NBLK = OZ_S * OZ_T / 16;
-  static_assert(NBLK % 8 == 0, "slab blocks must split evenly over the 8 waves");
-  const long kbase = (long)c * OZ_CH;
-  const int lrow = lane >> 2, pseg = lane & 3;
-  auto issue = [&](int ks, int buf) {
-    const long k0 = kbase + (long)ks * OZ_KS;
+  // A slab is OZ_S * OZ_T rows of 64 bytes = 40 blocks of 16 rows (1 KB);
+  // block b = wave + 8 p holds rows 16 wave .. 16 wave + 15 of digit plane p.
+  // Lane L fills row 16 wave + L / 4 of every plane, physical segment L % 4,
+  // from the logical segment that maps there, so one base pointer per operand
+  // and the plane stride address all its loads.  Rows past k read row k - 1
+  // (valid memory); they only feed output rows or columns >= k, which are never
+  // stored.
+  static_assert(OZ_T == 16 * 8, "one 16-row block of every plane per wave");
+  const int r_ld = 16 * wave + (lane >> 2);
+  const int ls = ((lane & 3) - (r_ld >> 2)) & 3;  // logical segment stored at physical segment L % 4
+  const long plane = (long)k * g2p, kbase = (long)c * OZ_CH;
+  const signed char* sa = S + (long)min(tm * OZ_T + r_ld, k - 1) * g2p + kbase + ls * 16;
+  const signed char* sb = S + (long)min(tn * OZ_T + r_ld, k - 1) * g2p + kbase + ls * 16;
+  auto issue = [&](int ks, signed char* as, signed char* bs) {
+    const long k0 = (long)ks * OZ_KS;
 #pragma unroll
-    for (int q = 0; q < NBLK / 8; ++q) {
-      const int b = wave + 8 * q;
-      const int fr = 16 * b + lrow;             // flattened (plane, row)
-      const int p = fr / OZ_T, r = fr % OZ_T;
-      const int ls = (pseg - (r >> 2)) & 3;     // logical segment stored at pseg
-      const int ra = min(tm * OZ_T + r, k - 1), rb = min(tn * OZ_T + r, k - 1);
-      __builtin_amdgcn_global_load_lds(S + ((long)p * k + ra) * g2p + k0 + ls * 16,
-                                       (lds_void*)(&As[buf][0][0][0] + b * 1024), 16, 0, 0);
-      __builtin_amdgcn_global_load_lds(S + ((long)p * k + rb) * g2p + k0 + ls * 16,
-                                       (lds_void*)(&Bs[buf][0][0][0] + b * 1024), 16, 0, 0);
+    for (int p = 0; p < OZ_S; ++p) {
+      const int b = wave + 8 * p;
+      __builtin_amdgcn_global_load_lds(sa + p * plane + k0, (lds_void*)(as + b * 1024), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(sb + p * plane + k0, (lds_void*)(bs + b * 1024), 16, 0, 0);
     }
   };
-  constexpr int NKS = OZ_CH / OZ_KS;
-  issue(0, 0);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  for (int ks = 0; ks < NKS; ++ks) {
-    const int cur = ks & 1;
-    if (ks + 1 < NKS) issue(ks + 1, cur ^ 1);
+  // k-steps of this chunk that hold columns < g2 (the rest are zero padding)
+  const int nks = (int)min((long)(OZ_CH / OZ_KS), (g2 - kbase + OZ_KS - 1) / OZ_KS);
+  // one k-step: fragments from (as, bs) while the next slabs stream into (na, nb)
+  auto step = [&](int ks, const signed char* as, const signed char* bs, signed char* na, signed char* nb) {
+    if (ks + 1 < nks) issue(ks + 1, na, nb);
     i4 bq[OZ_S][2];
 #pragma unroll
     for (int q = 0; q < OZ_S; ++q)
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int r = wn * 32 + j * 16 + l16;
-        bq[q][j] = *reinterpret_cast<const i4*>(&Bs[cur][q][r][oz_seg(r, l4) * 16]);
+        bq[q][j] = *reinterpret_cast<const i4*>(bs + (q * OZ_T + r) * OZ_KS + oz_seg(r, l4) * 16);
       }
 #pragma unroll
     for (int p = 0; p < OZ_S; ++p) {
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = wm * 64 + i * 16 + l16;
-        ap[i] = *reinterpret_cast<const i4*>(&As[cur][p][r][oz_seg(r, l4) * 16]);
+        ap[i] = *reinterpret_cast<const i4*>(as + (p * OZ_T + r) * OZ_KS + oz_seg(r, l4) * 16);
       }
 #pragma unroll
       for (int q = 0; q < OZ_S - p; ++q)
@@ -207,6 +207,13 @@ __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S
     // reading the current ones before they are refilled
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+  };
+  issue(0, As0, Bs0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int ks = 0; ks < nks; ks += 2) {
+    step(ks, As0, Bs0, As1, Bs1);
+    if (ks + 1 < nks) step(ks + 1, As1, Bs1, As0, Bs0);
   }
   // scale and write the partial tile: D[4 (l/16) + v][l % 16]
   double* W = work + ((long)c * ntiles + tile_lin) * (OZ_T * OZ_T);
@@ -323,7 +330,7 @@ int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ld
   THC_HIP(hipGetLastError());
   const long slots = (ntiles * nchunk + 7) / 8 * 8;  // every w < ntiles * nchunk gets one workgroup
   if (slots > 0x7fffffffL) return THC_ERR_ARG;
-  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, S, g2p, k, E, nchunk, part, ntiles);
+  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, S, g2, g2p, k, E, nchunk, part, ntiles);
   THC_HIP(hipGetLastError());
   hipLaunchKernelGGL(oz_reduce, dim3((unsigned)(ntiles * (OZ_T / RB) * (OZ_T / RB))), dim3(256), 0, s, part, nchunk,
                      ntiles, k, v, ldv);
