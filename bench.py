@@ -26,10 +26,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"hbm_gbs": 8000.0, "fp64_tflops": 78.6, "fp32_tflops": 157.3, "bf16_tflops": 2500.0}
-# HBM bytes per launch of the solve GEMM on diamond64_n64_k2048, from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# counters scaled by the same counters on a 1 GiB device copy (fetch x 2.0, write x 1.0); the summary
-# with its commands is profiles/isdf_diamond64_n64_k2048_hbm_traffic.txt.  Other workloads report null.
-SOLVE_TRAFFIC_BYTES = {"diamond64_n64_k2048": 30.11e9}
+# HBM bytes per launch of the mixed-path solve GEMM on diamond64_n64_k2048, from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE counters scaled by the same counters on a 1 GiB device copy (fetch x 2.0, write x 1.0); the
+# summary with its commands is profiles/isdf_diamond64_n64_k2048_hbm_traffic.txt.  Other workloads
+# report null.
+SOLVE_TRAFFIC_BYTES = {"diamond64_n64_k2048": 29.17e9}
 METRIC = "ISDF factorizations/s + % roofline; mixed vs FP64 GPU at <=1e-6 rel V error"
 
 
@@ -206,12 +207,13 @@ def main():
         "rel_V_error_mixed_vs_fp64": relv,
         "stage_ms": {k: round(float(v), 3) for k, v in zip(
             ["zct", "cct_chol_inv", "solve", "fft", "V", "luv", "isdf_total", "refine", "kmeans"], st)},
-        "roofline": {"kernel": "solve GEMM X = CC^T^-1 ZC^T (FP64 MFMA)", "bound": "mfma",
+        "roofline": {"kernel": "solve GEMM X = CC^T^-1 ZC^T (gemm_dtl_kernel, FP64 MFMA)", "bound": "mfma",
                      "achieved": round(achieved, 2), "peak": PEAK["fp64_tflops"], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["fp64_tflops"], 4),
                      "traffic": SOLVE_TRAFFIC_BYTES.get(a.config),
                      "traffic_unit": "bytes per launch",
-                     "algorithmic_bytes": 8.0 * (2 * K * N + K * K)},
+                     # Zt and (CC^T)^-1 read in FP64, X written in FP32 (nothing else reads X)
+                     "algorithmic_bytes": 8.0 * (K * N + K * K) + 4.0 * K * N},
     }
     if not a.no_batch:
         line["batched"] = batched_leg(ctx, max(1, min(a.steps, 3)), 1)
