@@ -118,8 +118,11 @@ def test_fft_vectors_matches_oracle_muv(ctx):
     assert rel(b @ b.T, muv.real) < 1e-13
 
 
-@pytest.mark.parametrize("k,g2", [(50, 1000), (300, 70000), (1030, 5000)])
+@pytest.mark.parametrize("k,g2", [(50, 1000), (300, 70000), (1030, 5000), (520, 64000)])
 def test_construct_muv(ctx, k, g2):
+    """FP64 V = B B^T by split-K partial tiles; G2 a multiple of 16 (70000, 64000)
+    takes the direct-to-LDS tile (520 rows: an 8-row last tile row, 134 splits of
+    480 columns with a 160-column last one)."""
     rng = np.random.default_rng(k)
     b = rng.standard_normal((k, g2))
     v = ctx.construct_muv(b).cpu().numpy()

@@ -367,9 +367,19 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs p) {
 // order is permuted so one 16-byte read feeds two k-steps: in half h, k-step s,
 // lane group l4 contributes k = 8 h + 2 l4 + s of both operands.  Rows past m
 // (n) read row m - 1 (n - 1); they only feed outputs that are never stored.
+// PARTIAL: split-K partial tiles of the lower triangle (blockIdx.y = split,
+// k range [y kchunk, min(k, (y + 1) kchunk))), stored whole as in gemm_kernel.
 typedef __attribute__((address_space(3))) void lds_void;
 constexpr int DTL_BM = 128, DTL_BN = 128, DTL_BK = 16, DTL_WN = 2, DTL_THREADS = 512;
 
+// one lane's 16 bytes from global memory into LDS (a plain function: with
+// template-dependent operands the builtin is not instantiated for the host
+// pass, and the kernel's launch stub goes missing)
+__device__ inline void load_lds16(const double* src, double* dst) {
+  __builtin_amdgcn_global_load_lds(src, (lds_void*)dst, 16, 0, 0);
+}
+
+template <bool PARTIAL>
 __global__ __launch_bounds__(DTL_THREADS) void gemm_dtl_kernel(KArgs p) {
   constexpr int BM = DTL_BM, BN = DTL_BN, BK = DTL_BK, TM = 2, TN = 4, WN = DTL_WN;
   constexpr int WTM = TM * 16, WTN = TN * 16;
@@ -405,8 +415,8 @@ __global__ __launch_bounds__(DTL_THREADS) void gemm_dtl_kernel(KArgs p) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int b = wave + 8 * q;
-      __builtin_amdgcn_global_load_lds(ap[q] + k0, (lds_void*)(as + b * 8 * BK), 16, 0, 0);
-      __builtin_amdgcn_global_load_lds(bp[q] + k0, (lds_void*)(bs + b * 8 * BK), 16, 0, 0);
+      load_lds16(ap[q] + k0, as + b * 8 * BK);
+      load_lds16(bp[q] + k0, bs + b * 8 * BK);
     }
   };
 
@@ -416,10 +426,15 @@ __global__ __launch_bounds__(DTL_THREADS) void gemm_dtl_kernel(KArgs p) {
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  const long nk = p.k / BK;
+  long kbeg = 0, kend = p.k;
+  if (PARTIAL) {
+    kbeg = (long)blockIdx.y * p.kchunk;
+    kend = min(p.k, kbeg + p.kchunk);
+  }
+  const long nk = (kend - kbeg) / BK;
   // one tile: fragments from (as, bs) while the next tile streams into (na, nb)
   auto step = [&](long t, const double* as, const double* bs, double* na, double* nb) {
-    if (t + 1 < nk) issue((t + 1) * BK, na, nb);
+    if (t + 1 < nk) issue(kbeg + (t + 1) * BK, na, nb);
     __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of this tile's MFMAs
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -447,7 +462,7 @@ __global__ __launch_bounds__(DTL_THREADS) void gemm_dtl_kernel(KArgs p) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   };
-  issue(0, As0, Bs0);
+  if (nk > 0) issue(kbeg, As0, Bs0);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   for (long t = 0; t < nk; t += 2) {
@@ -455,6 +470,21 @@ __global__ __launch_bounds__(DTL_THREADS) void gemm_dtl_kernel(KArgs p) {
     if (t + 1 < nk) step(t + 1, As1, Bs1, As0, Bs0);
   }
 
+  if (PARTIAL) {
+    const long tile_lin = (long)tm * (tm + 1) / 2 + tn;
+    double* W = reinterpret_cast<double*>(p.c) + ((long)blockIdx.y * gridDim.x + tile_lin) * (BM * BN);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = wm * WTM + i * 16 + Mfma<double>::row(lane, v);
+          const int cc = wn * WTN + j * 16 + l16;
+          W[r * BN + cc] = acc[i][j][v];
+        }
+    return;
+  }
   double* Cb = reinterpret_cast<double*>(p.c) + (long)bz * p.stride_c;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -622,7 +652,8 @@ int launch_dtl(const GemmArgs& g, hipStream_t s) {
   const long nblk = ((g.m + DTL_BM - 1) / DTL_BM) * ((g.n + DTL_BN - 1) / DTL_BN);
   if (nblk <= 0 || g.batch <= 0) return THC_OK;
   if (nblk > 0x7fffffffL) return THC_ERR_ARG;
-  hipLaunchKernelGGL(gemm_dtl_kernel, dim3((unsigned)nblk, 1, (unsigned)g.batch), dim3(DTL_THREADS), 0, s, p);
+  hipLaunchKernelGGL(gemm_dtl_kernel<false>, dim3((unsigned)nblk, 1, (unsigned)g.batch), dim3(DTL_THREADS), 0, s,
+                     p);
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
@@ -695,7 +726,12 @@ int syrk_splitk(const void* a, long m, long k, long lda, int prec, double* v, lo
   const int vw = prec == 0 ? 2 : 4;
   const bool vec = aligned16(a) && lda % vw == 0;
   if (prec == 0) {
-    launch_vec<double, CfgLarge, EPI_PARTIAL, true, true, false, false>(p, vec, grid, s);
+    // direct-to-LDS tiles when every split covers whole 16-deep k-steps
+    static_assert(SY_BM == DTL_BM && SY_BN == DTL_BN && SY_BK == DTL_BK, "same tile as the direct-to-LDS kernel");
+    if (vec && k % DTL_BK == 0)
+      hipLaunchKernelGGL(gemm_dtl_kernel<true>, grid, dim3(DTL_THREADS), 0, s, p);
+    else
+      launch_vec<double, CfgLarge, EPI_PARTIAL, true, true, false, false>(p, vec, grid, s);
     THC_HIP(hipGetLastError());
     hipLaunchKernelGGL((syrk_reduce_kernel<double, SY_BM, SY_BN>), rgrid, dim3(256), 0, s,
                        (const double*)work, nsplit, ntiles, m, v, ldv);
