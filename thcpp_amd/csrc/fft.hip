@@ -57,6 +57,7 @@ struct FftParams {
   int k, n0, n1, n2, n2h;
   Plan1D pz, py, px;
   const void* twz; const void* twy; const void* twx;
+  int vec_out;  // output aligned to two elements: one vector store per complex value
 };
 
 // Division by a run-time divisor d >= 1 of dividends e with e * d < 2^32:
@@ -306,9 +307,23 @@ __global__ __launch_bounds__(FT) void pass_a(const X* __restrict__ x, C2<T>* __r
   for (int e = threadIdx.x; e < n1 * n2h; e += blockDim.x) dst[e] = fin[e];
 }
 
-// t: (k, n0, n1, n2h) complex -> out (k, n0, n1, n2h, 2) weighted.
+// Half-spectrum weights sqrt(w(kz) coulG(kx, ky, kz)) rounded to T, w = 1 for
+// kz = 0 and kz = n2 / 2, else 2 (the conjugate half the R2C layout omits),
+// computed once per call for all k vectors.
+template <class T>
+__global__ void weight_kernel(const double* __restrict__ coulG, int n1, int n2, int n2h, long total,
+                              T* __restrict__ wt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long row = i / n2h;  // kx * n1 + ky
+  const int kz = (int)(i - row * n2h);
+  const double w = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
+  wt[i] = (T)sqrt(w * coulG[row * n2 + kz]);
+}
+
+// t: (k, n0, n1, n2h) complex -> out (k, n0, n1, n2h, 2) weighted by wt.
 template <class T, int IPV>  // IPV = 0: ping-pong stages
-__global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const double* __restrict__ coulG,
+__global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const T* __restrict__ wt,
                                              T* __restrict__ out, FftParams p) {
   constexpr bool ip = IPV > 0;
   extern __shared__ unsigned char smem_raw[];
@@ -341,16 +356,19 @@ __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const 
   C2<T>* fin = bufa;
   if (ip) fft_lines_ip<T, IPV>(bufa, p.px, n2h, n2h, 1, twx);
   else fin = fft_lines<T>(bufa, bufb, p.px, n2h, n2h, 1, twx);
+  typedef T v2 __attribute__((ext_vector_type(2)));
   T* dst = out + mu * (long)n0 * n1 * n2h * 2;
   for (int e = threadIdx.x; e < n0 * n2h; e += blockDim.x) {
     const int kx = fn2h.div(e), kz = e - kx * n2h;
-    const long g = ((long)kx * n1 + iy) * n2 + kz;
-    const double w = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
-    const T f = (T)sqrt(w * coulG[g]);
+    const long h = ((long)kx * n1 + iy) * n2h + kz;
+    const T f = wt[h];
     const C2<T> v = fin[e];
-    const long o = (((long)kx * n1 + iy) * n2h + kz) * 2;
-    dst[o] = v.x * f;
-    dst[o + 1] = v.y * f;
+    if (p.vec_out) {
+      *reinterpret_cast<v2*>(dst + 2 * h) = v2{v.x * f, v.y * f};
+    } else {
+      dst[2 * h] = v.x * f;
+      dst[2 * h + 1] = v.y * f;
+    }
   }
 }
 
@@ -398,9 +416,9 @@ int launch_a(dim3 g, size_t la, const X* x, C2<T>* tmid, const FftParams& p, hip
 }
 
 template <class T, int IPV>
-int launch_b(dim3 g, size_t lb, const C2<T>* tmid, const double* coulG, T* out, const FftParams& p, hipStream_t s) {
+int launch_b(dim3 g, size_t lb, const C2<T>* tmid, const T* wt, T* out, const FftParams& p, hipStream_t s) {
   THC_HIP(hipFuncSetAttribute((const void*)pass_b<T, IPV>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
-  hipLaunchKernelGGL((pass_b<T, IPV>), g, dim3(FT), lb, s, tmid, coulG, out, p);
+  hipLaunchKernelGGL((pass_b<T, IPV>), g, dim3(FT), lb, s, tmid, wt, out, p);
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
@@ -424,19 +442,24 @@ int run(const X* x, int k, const int n[3], const double* coulG, void* out, void*
   C2<T>* dz = reinterpret_cast<C2<T>*>(w);
   C2<T>* dy = reinterpret_cast<C2<T>*>(w + tb);
   C2<T>* dx = reinterpret_cast<C2<T>*>(w + 2 * tb);
-  C2<T>* tmid = reinterpret_cast<C2<T>*>(w + 3 * tb);
+  const long nh = (long)n[0] * n[1] * n2h;  // half-spectrum points
+  T* wt = reinterpret_cast<T*>(w + 3 * tb);
+  C2<T>* tmid = reinterpret_cast<C2<T>*>(w + 3 * tb + align_up(sizeof(T) * (size_t)nh, 256));
+  hipLaunchKernelGGL(weight_kernel<T>, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, coulG, n[1], n[2], n2h, nh,
+                     wt);
   hipLaunchKernelGGL(twiddle_kernel<T>, dim3(1), dim3(128), 0, s, n[2], dz);
   hipLaunchKernelGGL(twiddle_kernel<T>, dim3(1), dim3(128), 0, s, n[1], dy);
   hipLaunchKernelGGL(twiddle_kernel<T>, dim3(1), dim3(128), 0, s, n[0], dx);
   THC_HIP(hipGetLastError());
   p.twz = dz; p.twy = dy; p.twx = dx;
   T* o = reinterpret_cast<T*>(out);
+  p.vec_out = (reinterpret_cast<uintptr_t>(out) % (2 * sizeof(T))) == 0;
   if (ipa == 16) THC_TRY((launch_a<T, 16, X>(dim3(n[0], k), la, x, tmid, p, s)));
   else if (ipa == 24) THC_TRY((launch_a<T, 24, X>(dim3(n[0], k), la, x, tmid, p, s)));
   else THC_TRY((launch_a<T, 0, X>(dim3(n[0], k), la, x, tmid, p, s)));
-  if (ipb == 16) THC_TRY((launch_b<T, 16>(dim3(n[1], k), lb, tmid, coulG, o, p, s)));
-  else if (ipb == 24) THC_TRY((launch_b<T, 24>(dim3(n[1], k), lb, tmid, coulG, o, p, s)));
-  else THC_TRY((launch_b<T, 0>(dim3(n[1], k), lb, tmid, coulG, o, p, s)));
+  if (ipb == 16) THC_TRY((launch_b<T, 16>(dim3(n[1], k), lb, tmid, wt, o, p, s)));
+  else if (ipb == 24) THC_TRY((launch_b<T, 24>(dim3(n[1], k), lb, tmid, wt, o, p, s)));
+  else THC_TRY((launch_b<T, 0>(dim3(n[1], k), lb, tmid, wt, o, p, s)));
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
@@ -446,7 +469,8 @@ int run(const X* x, int k, const int n[3], const double* coulG, void* out, void*
 size_t fft_work_bytes(int k, const int n[3], int prec) {
   const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
   const size_t tb = align_up(es * 2 * MAXN, 256);
-  return 3 * tb + align_up(2 * es * (size_t)k * n[0] * n[1] * (n[2] / 2 + 1), 256) + 256;
+  const size_t nh = (size_t)n[0] * n[1] * (n[2] / 2 + 1);
+  return 3 * tb + align_up(es * nh, 256) + align_up(2 * es * (size_t)k * nh, 256) + 256;
 }
 
 int fft_coulomb(const void* x, int x_f32, int k, const int n[3], const double* coulG, void* out, int prec,
