@@ -254,14 +254,15 @@ def test_isdf_f32_x_store_bitwise(ctx, prec, n, m, k):
     assert torch.equal(with_xi["V"].cpu(), without["V"].cpu())
 
 
-@pytest.mark.parametrize("k", [256, 272])
-def test_isdf_solve_large_tiles(ctx, k):
+@pytest.mark.parametrize("n,k", [((40, 36, 30), 256), ((40, 36, 30), 272), ((50, 40, 36), 272)])
+def test_isdf_solve_large_tiles(ctx, n, k):
     """The solve at sizes that take the direct-to-LDS 128 x 128 x 16 FP64 tile (A
     and B contiguous along k, k a multiple of 16, at least 512 tiles): k = 272
-    leaves a 16-row last tile row, N = 43200 a 64-column last tile column.  xi
-    against scipy's solve of CC^T X = ZC^T, within the conditioning of CC^T."""
+    leaves a 16-row last tile row, N = 43200 a 64-column last tile column, and
+    N = 72000 (>= 65536) also takes the one-workgroup-per-CU Zt GEMM.  xi against
+    scipy's solve of CC^T X = ZC^T, within the conditioning of CC^T."""
     from thcpp_amd import synthetic
-    n, m = (40, 36, 30), 64
+    m = 64
     N = int(np.prod(n))
     rng = np.random.default_rng(k)
     a = np.linalg.qr(rng.standard_normal((N, m)))[0]
