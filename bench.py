@@ -30,7 +30,7 @@ PEAK = {"hbm_gbs": 8000.0, "fp64_tflops": 78.6, "fp32_tflops": 157.3, "bf16_tflo
 # WRITE_SIZE counters scaled by the same counters on a 1 GiB device copy (fetch x 2.0, write x 1.0); the
 # summary with its commands is profiles/isdf_diamond64_n64_k2048_hbm_traffic.txt.  Other workloads
 # report null.
-SOLVE_TRAFFIC_BYTES = {"diamond64_n64_k2048": 38.56e9}
+SOLVE_TRAFFIC_BYTES = {"diamond64_n64_k2048": 38.42e9}
 METRIC = "ISDF factorizations/s + % roofline; mixed vs FP64 GPU at <=1e-6 rel V error"
 
 
