@@ -6,8 +6,17 @@
  * device unless marked "host".  No C++ types, exceptions or allocations cross
  * the boundary; every call returns an int status (THC_OK = 0).  A context owns
  * one HIP stream and a device workspace; it is not internally synchronised
- * (one context per host thread).  Calls are stream-ordered on the context's
- * stream and return after the stream has drained unless stated otherwise.
+ * (one context per host thread).
+ *
+ * Ordering: every call runs on the context's stream and returns after that
+ * stream has drained, so outputs are complete when a call returns.  Inputs
+ * must be complete before the call starts on that stream: either pass the
+ * stream the caller produces them on to thc_ctx_create, or create the context
+ * with stream = NULL, in which case the library creates a BLOCKING stream
+ * (hipStreamDefault), which HIP orders after earlier work on the legacy
+ * default (NULL) stream -- the stream torch and most hosts use by default.
+ * Inputs produced on any other non-blocking stream must be synchronised by
+ * the caller first.
  *
  * Reference interfaces replaced (paths relative to JoonhoLee-Group/thcpp):
  *   KMeans::KMeansSolver::kernel            src/kmeans.cpp:187-264
@@ -33,20 +42,33 @@ enum thc_status {
   THC_ERR_EMPTY_CLUSTER = 3,    /* a k-means cluster weight < 1e-8 (kmeans.cpp:132,137-143, the reference exits) */
   THC_ERR_NOT_POSDEF = 4,       /* FP64 Cholesky of CC^T failed (the reference's QR solve would return garbage) */
   THC_ERR_NO_MEMORY = 5,        /* device workspace allocation failed */
-  THC_ERR_UNSUPPORTED = 6,      /* FFT length with a prime factor > 5, or > 4096 clusters, etc. */
+  THC_ERR_UNSUPPORTED = 6,      /* an FFT edge with a prime factor > 5 or above THC_MAX_FFT_EDGE, more than
+                                   THC_MAX_CLUSTERS k-means clusters, or an option combination listed as
+                                   refused on the entry point */
   THC_WARN_DUPLICATE_INDICES = 100, /* map_to_grid found repeated indices (kmeans.cpp:151-155) */
   THC_WARN_LUV_NOT_POSDEF = 101,    /* chol(Muv) failed; Luv still written (interpolating_vectors.cpp:296-304) */
-  THC_WARN_FFT_GRID_MISMATCH = 102  /* prod(fft_grid) != N (interpolating_vectors.cpp:368-370) */
+  THC_WARN_RANK_DEFICIENT = 102     /* CC^T has numerical rank < N_mu under the reference's rule
+                                       (interpolating_vectors.cpp:192-208, matrix_operations.h:713-735);
+                                       outputs still written */
 };
+/* An FFT grid with prod(fft_grid) != N is refused with THC_ERR_ARG.  The
+ * reference prints a warning there and goes on (interpolating_vectors.cpp:
+ * 368-370): it transforms the first prod(fft_grid) entries of each column and
+ * leaves the rest untransformed when the product is smaller, and reads and
+ * writes past the column when it is larger, so it has no result to match. */
 
-/* Arithmetic used by the K^2 N stages (solve application, G contraction). */
+/* Size limits. */
+#define THC_MAX_FFT_EDGE 128   /* per-axis FFT length (fft_grid[i]) */
+#define THC_MAX_CLUSTERS 4096  /* k-means clusters (N_mu) */
+
+/* Arithmetic of the ISDF stages.  ZC^T, CC^T and the Cholesky factor and
+ * inverse of CC^T are FP64 in every mode (CC^T is ill-conditioned). */
 enum thc_precision {
-  THC_PREC_FP64 = 0,  /* FP64 MFMA throughout */
-  THC_PREC_FP32 = 1,  /* FP32 MFMA for ZC^T and the G contraction, FP32 FFT; FP64 Cholesky/solve/V sum */
-  THC_PREC_BF16 = 2,  /* reserved */
-  THC_PREC_MIXED = 3  /* FP64 ZC^T, Cholesky and solve; FP32 FFT; V from exact int8 MFMA products of
-                         5 signed 7-bit digits per value (per row and 8192-column chunk exponents),
-                         summed in FP64 */
+  THC_PREC_FP64 = 0,  /* FP64 MFMA throughout: solve, FFT, G contraction */
+  THC_PREC_FP32 = 1,  /* FP64 solve; FP32 FFT; G contraction on FP32 MFMA with FP32 accumulation
+                         over each split-K chunk and an FP64 sum of the chunks */
+  THC_PREC_MIXED = 3  /* FP64 solve; FP32 FFT; V from exact int8 MFMA products of 5 signed 7-bit
+                         digits per value (per row and 8192-column chunk exponents), summed in FP64 */
 };
 
 typedef struct thc_ctx thc_ctx;
@@ -67,11 +89,15 @@ typedef struct thc_isdf_opts {
   int precision;           /* enum thc_precision */
   int refine_steps;        /* FP64 iterative-refinement steps of the least-squares solve (0 = none) */
   int want_luv;            /* 1: also factor V = Luv Luv^T */
+  int check_rank;          /* 1: numerical rank of CC^T as IVecs::check_rank computes it (thc_cct_rank),
+                              beside the ZC^T GEMM; THC_WARN_RANK_DEFICIENT when rank < N_mu */
 } thc_isdf_opts;
 
 typedef struct thc_isdf_stats {
   int cct_chol_info;       /* 0, or the 1-based order of the first non-positive leading minor of CC^T */
   int luv_chol_info;       /* same for V */
+  int cct_rank;            /* numerical rank of CC^T (opts->check_rank), else -1 */
+  double cct_smax;         /* largest singular value of CC^T (opts->check_rank), else 0 */
   double stage_ms[8];      /* device time: 0 ZC^T, 1 the part of CC^T + Cholesky + inverse not hidden behind ZC^T
                               (they run on a second stream beside it), 2 solve, 3 FFT, 4 V, 5 Luv, 6 total,
                               7 refinement */
@@ -80,7 +106,8 @@ typedef struct thc_isdf_stats {
 /* ---------------------------------------------------------------- context */
 
 /* Create a context on HIP device 'device'.  'stream' may be NULL (the context
- * creates a default-priority stream of its own) or a hipStream_t to use. */
+ * creates a blocking, default-priority stream of its own; see "Ordering"
+ * above) or a hipStream_t to use. */
 int thc_ctx_create(int device, void *stream, thc_ctx **out);
 int thc_ctx_destroy(thc_ctx *ctx);
 /* Use HIP graphs for the k-means iteration loop (default 1). */
@@ -147,6 +174,16 @@ int thc_czt(thc_ctx *ctx, const double *aoR, long n, int m, const int *idx, int 
 
 /* CC^T = ZC^T[:, idx], k x k (IVecs::setup_CCt, src/interpolating_vectors.cpp:163-190). */
 int thc_cct(thc_ctx *ctx, const double *czt, long n, const int *idx, int k, double *cct);
+
+/* Numerical rank of the symmetric k x k matrix cct (k <= THC_MAX_CLUSTERS):
+ * the number of singular values >= k * s0 * 1e-16, s0 the largest, the rule
+ * of MatrixOperations::rank (src/matrix_operations.h:713-735) that
+ * IVecs::check_rank applies to CC^T (src/interpolating_vectors.cpp:192-208).
+ * The singular values of a symmetric matrix are its |eigenvalues|; they are
+ * counted on the GPU's Householder tridiagonal form by Sturm sequences, with
+ * no eigenvector work.  rank (host) out; smax (host, nullable) gets s0.
+ * Returns THC_WARN_RANK_DEFICIENT when the rank is below k. */
+int thc_cct_rank(thc_ctx *ctx, const double *cct, int k, int *rank, double *smax);
 
 /* In-place lower Cholesky of a k x k row-major SPD matrix (strict upper part
  * zeroed).  *info = 0 or the 1-based failing leading minor (pzpotrf 'L',

@@ -20,6 +20,9 @@ are real here, so every imaginary part is zero, but the steps are the same):
   fft_vectors_backward src/interpolating_vectors.cpp:384-391 unnormalised FFTW_BACKWARD of X^H
   construct_muv_half src/interpolating_vectors.cpp:258-273 Muv = IVMG^T IVG, both times sqrt(coulG)
   isdf_half          src/main.cpp:63-72 + IVecs(rotate=true): HalfTransformedMuv, no Cholesky
+  xi_rows            rows of least_squares' X at full size without forming ZC^T: the
+                     QR of CC^T applied as pzgels applies it (Q^H, then R^-1) to ZC^T
+                     built block by block of grid columns
 
 Parity pins (tests/test_oracle_isdf.py): numpy/scipy closed forms on small
 inputs, brute-force DFT (both signs), plane-wave Coulomb closed form, the G=0
@@ -67,6 +70,34 @@ def least_squares(c: np.ndarray, z: np.ndarray) -> np.ndarray:
     (matrix_operations.h:184-202, called at interpolating_vectors.cpp:416)."""
     q, r = scipy.linalg.qr(c, mode="economic")
     return scipy.linalg.solve_triangular(r, q.conj().T @ z, lower=False)
+
+
+def xi_rows(aoR: np.ndarray, idx: np.ndarray, rows: np.ndarray, block: int = 32768) -> np.ndarray:
+    """Rows 'rows' of the least-squares X = CC^T^-1 ZC^T (K x N) for systems
+    whose ZC^T does not fit in memory: CC^T = (Phi_mu Phi_mu^H) * conj(.)
+    (setup_CCt, interpolating_vectors.cpp:163-190), its Householder QR, then
+    for one block of grid columns at a time ZC^T (czt on that block), Q^H
+    applied to it and the triangular solve with R, as pzgels('N') treats every
+    right-hand side (matrix_operations.h:184-202).  The columns of ZC^T are
+    independent right-hand sides, so the rows kept equal those of
+    least_squares(cct(Z), Z)."""
+    aoR = np.asarray(aoR)
+    idx = np.asarray(idx)
+    mu = aoR[idx]
+    pc = mu.conj() @ mu.T
+    c = (pc * pc.conj())
+    c = c.real if np.isrealobj(aoR) else c
+    q, r = scipy.linalg.qr(c, mode="economic")
+    qh = np.ascontiguousarray(q.conj().T)
+    rows = np.asarray(rows)
+    n = aoR.shape[0]
+    out = np.empty((rows.shape[0], n), dtype=c.dtype)
+    for b0 in range(0, n, block):
+        p = mu.conj() @ aoR[b0:b0 + block].T
+        z = p * p.conj()
+        z = z.real if np.isrealobj(aoR) else z
+        out[:, b0:b0 + block] = scipy.linalg.solve_triangular(r, qh @ z, lower=False)[rows]
+    return out
 
 
 def fft_vectors(x: np.ndarray, fft_grid) -> np.ndarray:

@@ -15,9 +15,9 @@
 //   oracle_map_to_grid        src/kmeans.cpp:146-157
 //   oracle_kmeans             src/kmeans.cpp:187-264 (KMeansSolver::kernel, 1 rank)
 //
-// Build: g++ -O2 -ffp-contract=off (no -march: baseline x86-64 has no FMA, so the
-// distance sums round after every multiply and add, exactly as the reference
-// compiled without -march does).  Parity pins: tests/test_oracle_kmeans.py.
+// Build: g++ -O2 -ffp-contract=off -fopenmp (no -march: baseline x86-64 has no
+// FMA, so the distance sums round after every multiply and add, exactly as the
+// reference compiled without -march does).  Parity pins: tests/test_oracle_kmeans.py.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -29,19 +29,25 @@
 namespace {
 
 // Nearest centroid per point; ties resolved to the first index, as
-// std::min_element over the distance vector does (kmeans.cpp:102-104).
-void classify(const double* pts, long npts, const double* cen, long ncen,
-              int* out, std::vector<double>& deltas) {
-  deltas.resize(ncen);
-  for (long i = 0; i < npts; ++i) {
-    for (long j = 0; j < ncen; ++j) {
-      // centroid minus point, C order xyz (kmeans.cpp:96-99)
-      double dx = cen[j * 3 + 0] - pts[i * 3 + 0];
-      double dy = cen[j * 3 + 1] - pts[i * 3 + 1];
-      double dz = cen[j * 3 + 2] - pts[i * 3 + 2];
-      deltas[j] = std::sqrt(dx * dx + dy * dy + dz * dz);
+// std::min_element over the distance vector does (kmeans.cpp:102-104).  Points
+// are independent, so they are split over OpenMP threads; each point's scan
+// runs in the reference's order in one thread, so the map does not depend on
+// the thread count.
+void classify(const double* pts, long npts, const double* cen, long ncen, int* out) {
+#pragma omp parallel
+  {
+    std::vector<double> deltas(ncen);
+#pragma omp for schedule(static)
+    for (long i = 0; i < npts; ++i) {
+      for (long j = 0; j < ncen; ++j) {
+        // centroid minus point, C order xyz (kmeans.cpp:96-99)
+        double dx = cen[j * 3 + 0] - pts[i * 3 + 0];
+        double dy = cen[j * 3 + 1] - pts[i * 3 + 1];
+        double dz = cen[j * 3 + 2] - pts[i * 3 + 2];
+        deltas[j] = std::sqrt(dx * dx + dy * dy + dz * dz);
+      }
+      out[i] = (int)(std::min_element(deltas.begin(), deltas.end()) - deltas.begin());
     }
-    out[i] = (int)(std::min_element(deltas.begin(), deltas.end()) - deltas.begin());
   }
 }
 
@@ -70,8 +76,7 @@ void oracle_mt19937(unsigned int seed, long count, unsigned int* out) {
 
 void oracle_classify(const double* grid, long n, const double* centroids, long k,
                      int* grid_map) {
-  std::vector<double> deltas;
-  classify(grid, n, centroids, k, grid_map, deltas);
+  classify(grid, n, centroids, k, grid_map);
 }
 
 // Weighted centroid update.  Sums run in ascending grid index with the product
@@ -126,8 +131,7 @@ double oracle_percentage_change(const int* a, const int* b, long n) {
 // duplicates (the reference only prints them, kmeans.cpp:151-155).
 int oracle_map_to_grid(const double* grid, long n, const double* centroids, long k,
                        int* out_idx) {
-  std::vector<double> deltas;
-  classify(centroids, k, grid, n, out_idx, deltas);
+  classify(centroids, k, grid, n, out_idx);
   std::sort(out_idx, out_idx + k);
   int dups = 0;
   for (long i = 0; i + 1 < k; ++i) dups += (out_idx[i] == out_idx[i + 1]);
@@ -159,15 +163,14 @@ int oracle_kmeans(const double* grid, const double* rho, long n, long k, int max
     for (int d = 0; d < 3; ++d) cur[j * 3 + d] = grid[(long)idx0[j] * 3 + d];
   for (long j = 0; j < k; ++j) interp_idx[j] = 0;
   std::vector<int> map(n), old_map(n);
-  std::vector<double> deltas;
-  classify(grid, n, cur.data(), k, map.data(), deltas);
+  classify(grid, n, cur.data(), k, map.data());
   old_map = map;
   double diff = 0.0;
   int status = 0;
   int it = 0;
   bool broke = false;
   for (int i = 0; i < max_it; ++i) {
-    classify(grid, n, cur.data(), k, map.data(), deltas);
+    classify(grid, n, cur.data(), k, map.data());
     if (maps) std::memcpy(maps + (long)i * n, map.data(), sizeof(int) * n);
     if (oracle_update_centroids(rho, grid, n, map.data(), k, nxt.data())) {
       *iters_run = i + 1;

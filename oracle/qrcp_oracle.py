@@ -5,6 +5,13 @@ checker for the HIP path; only tests/ may import it.
 
   z_matrix       src/qrcp.cpp:33-48   ZT[(i k), a] = conj(phi_i(r_a)) phi_k(r_a)   (M^2 x N)
   z_half_matrix  src/qrcp.cpp:49-66   ZT[(i k), a] = conj(phi_i^occ(r_a)) phi_k(r_a) (M_occ M x N)
+                 -- the matrix that function's comment defines, with row i*M + k.
+                 The reference's loop writes row i*nelec + k instead (qrcp.cpp:62),
+                 so with M > nelec its rows overlap and the tail stays zero; that
+                 indexing is not reproduced (tests/test_oracle_qrcp.py shows both
+                 matrices and that their pivots differ), so the half-rotated
+                 pivots are pinned to the stated definition, not to the
+                 reference's output.
   qrcp_points    src/qrcp.cpp:100-173 QR with column pivoting of ZT (pzgeqpf,
                  src/matrix_operations.h:756-788, every column free), the first
                  K pivots sorted ascending; |R_jj| in pivot order.
@@ -28,7 +35,9 @@ def z_matrix(aoR: np.ndarray) -> np.ndarray:
 
 
 def z_half_matrix(aoR: np.ndarray, aoR_half: np.ndarray) -> np.ndarray:
-    """(M_occ M, N): row i*M + k holds conj(phi_i^occ) phi_k (src/qrcp.cpp:49-66)."""
+    """(M_occ M, N): row i*M + k holds conj(phi_i^occ) phi_k, the Khatri-Rao
+    rows src/qrcp.cpp:53-56 states (its loop at :58-64 indexes the rows with
+    stride nelec instead of M; see the module docstring)."""
     a, h = np.asarray(aoR), np.asarray(aoR_half)
     n = a.shape[0]
     return (h.conj()[:, :, None] * a[:, None, :]).reshape(n, -1).T

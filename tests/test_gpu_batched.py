@@ -50,6 +50,27 @@ def test_batched_kmeans_matches_oracle(ctx, batch):
         assert np.array_equal(o["interp_idx"].cpu().numpy(), ref["interp_idx"]), s.n
 
 
+@pytest.mark.parametrize("prec", [L.PREC_FP64, L.PREC_MIXED])
+def test_batched_v_matches_oracle(ctx, batch, prec):
+    """V of three cells of the batch (smallest, middle, largest grid) against
+    the oracle's FP64 ISDF on the oracle's k-means points: FP64 within the
+    conditioning floor, mixed within 1e-6 beyond it."""
+    from oracle import isdf_oracle as IO
+    st, outs = ctx.factorize_batched([as_dict(s) for s in batch], precision=prec)
+    assert st == L.THC_OK
+    order = np.argsort([s.N for s in batch])
+    for i in (order[0], order[len(order) // 2], order[-1]):
+        s, o = batch[i], outs[i]
+        idx = KO.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, s.seed)["interp_idx"]
+        v_ref = IO.isdf(s.aoR, idx, s.n, s.coulG)["V"]
+        fp64 = ctx.isdf(s.aoR, idx, s.n, s.coulG)
+        e64 = np.linalg.norm(fp64["V"].cpu().numpy() - v_ref) / np.linalg.norm(v_ref)
+        e = np.linalg.norm(o["V"].cpu().numpy() - v_ref) / np.linalg.norm(v_ref)
+        print("cell %d (grid %s, K %d) prec %d: rel V vs oracle %.3e (FP64 path %.3e)" % (i, s.n, s.K, prec, e, e64))
+        assert e64 < 1e-6
+        assert e < e64 + (1e-6 if prec == L.PREC_MIXED else 1e-12)
+
+
 def test_batched_error_isolated(ctx, batch):
     """A system with thc_cfac = 0 (no interpolation points) fails with
     THC_ERR_ARG; the rest are still factorised and the call returns that

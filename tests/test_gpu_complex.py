@@ -72,4 +72,6 @@ def test_isdf_complex_vs_oracle(ctx):
     l = out["Luv"].cpu().numpy()
     assert rel(l @ l.T, out["V"].cpu()) < 1e-13
     mixed = ctx.isdf_complex(a, idx, n, cg, precision=L.PREC_MIXED)
-    assert rel(mixed["V"].cpu(), out["V"].cpu()) < 1e-6
+    e_mx = rel(mixed["V"].cpu(), ref["V"].real)
+    print("complex orbitals mixed rel V vs oracle %.3e" % e_mx)
+    assert e_mx < ev + 1e-6

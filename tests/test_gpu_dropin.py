@@ -38,7 +38,10 @@ def read(h, path, name, dtype=np.float64, cap=1 << 24):
     return out[:n]
 
 
-def test_dropin_driver(tmp_path):
+@pytest.mark.parametrize("precision,tol", [(None, 1e-9), ("mixed", 1e-6)])
+def test_dropin_driver(tmp_path, precision, tol):
+    """The driver's IVecs::kernel is one thc_isdf call; "thc_precision" picks
+    the path (absent: FP64).  CC^T's rank is checked as check_rank does."""
     h = host()
     s = synthetic.make_system(reps=1, n=16, M=8, thc_cfac=3, max_it=7)
     orb = str(tmp_path / "orbitals.h5")
@@ -51,6 +54,8 @@ def test_dropin_driver(tmp_path):
                                          P(hc), P(ne), -2.5, None, 0, None) == 0
     inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac,
            "interpolating_points": {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}}}
+    if precision:
+        inp["thc_precision"] = precision
     (tmp_path / "input.json").write_text(json.dumps(inp))
     r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,
                        text=True, timeout=120)
@@ -66,7 +71,12 @@ def test_dropin_driver(tmp_path):
     assert not np.triu(luv, 1).any()
     v_ref = IO.isdf(s.aoR, idx, s.n, s.coulG)["V"]
     err = np.linalg.norm(luv @ luv.T - v_ref) / np.linalg.norm(v_ref)
-    assert err < 1e-9, err
+    assert err < tol, err
+    c = IO.cct(IO.czt(s.aoR, idx), idx)
+    rank = IO.rank(c)
+    assert (" * CCt matrix has full rank : %d" % rank if rank == s.K else
+            " * WARNING: CCt matrix is rank deficient. Rank = %d" % rank) in r.stdout
+    assert ("(mixed)" if precision == "mixed" else "(fp64)") in r.stdout
     np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/dims", np.int32), [-1, 0, 0, s.M, 4, 4, 0, 0])
     np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/THC/dims", np.int32), [s.M, s.K, 0])
     np.testing.assert_array_equal(read(h, thc, "/Hamiltonian/Energies"), [-2.5])

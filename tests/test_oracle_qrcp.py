@@ -5,6 +5,7 @@ density matrix (the identity the GPU path is built on), and pivoted Cholesky of
 that Gram matrix reproduces the QRCP pivots and |R_jj|."""
 import numpy as np
 import pytest
+import scipy.linalg
 
 from oracle import qrcp_oracle as QO
 
@@ -77,3 +78,44 @@ def test_half_rotated_z_and_pivots():
     gpiv, _ = _greedy_gram_schmidt(z, 12)
     np.testing.assert_array_equal(piv, gpiv)
     np.testing.assert_array_equal(_pivoted_cholesky_gram(a, 12, occ), piv)
+
+
+def _reference_literal_half_z(aoR, aoR_half):
+    """ZT as src/qrcp.cpp:49-66 literally fills it: row i*nelec + k (stride
+    nelec, not nbasis) for i < nelec, k < nbasis in a matrix of nelec*nbasis
+    rows, later writes overwriting earlier ones."""
+    n, m = aoR.shape
+    mo = aoR_half.shape[1]
+    z = np.zeros((mo * m, n))
+    for i in range(mo):
+        for k in range(m):
+            z[i * mo + k] = aoR_half[:, i].conj() * aoR[:, k]
+    return z
+
+
+def test_half_rotated_z_departs_from_reference_indexing():
+    """The reference writes the half-rotated ZT with row stride nelec where the
+    (i, k) pair index needs nbasis (src/qrcp.cpp:62).  With nbasis > nelec the
+    rows overlap: every product with k >= nelec is overwritten except for the
+    last occupied orbital, and the last nelec*nbasis - (nelec-1)*nelec - nbasis
+    rows stay zero.  The oracle (and the GPU path) use the Khatri-Rao rows the
+    comment there states, [ZT]_{(ik)a} = phi_i^occ*(r_a) phi_k(r_a): both are
+    shown here, and the pivots they give differ."""
+    a = _orbitals(600, 8, 21)
+    occ = np.ascontiguousarray(a[:, :3])
+    lit = _reference_literal_half_z(a, occ)
+    kr = QO.z_half_matrix(a, occ)
+    assert kr.shape == lit.shape == (24, 600)
+    # rows of the literal matrix: products (i, k < nelec) for i < nelec - 1, then
+    # all of (nelec - 1, k), then zeros
+    mo, m = 3, 8
+    last = (mo - 1) * mo + m
+    assert not lit[last:].any()
+    np.testing.assert_array_equal(lit[:last], np.concatenate(
+        [kr[i * m:i * m + mo] for i in range(mo - 1)] + [kr[(mo - 1) * m:mo * m]]))
+    k = 12
+    piv_lit = np.sort(scipy.linalg.qr(lit, mode="r", pivoting=True)[1][:k])
+    piv_kr, _, _ = QO.qrcp_points(a, k, aoR_half=occ)
+    assert not np.array_equal(piv_lit, piv_kr)
+    # the literal matrix has rank (nelec - 1) nelec + nbasis = 14 at most
+    assert np.linalg.matrix_rank(lit) <= (mo - 1) * mo + m

@@ -20,9 +20,9 @@ THC_ERR_NO_MEMORY = 5
 THC_ERR_UNSUPPORTED = 6
 THC_WARN_DUPLICATE_INDICES = 100
 THC_WARN_LUV_NOT_POSDEF = 101
-THC_WARN_FFT_GRID_MISMATCH = 102
+THC_WARN_RANK_DEFICIENT = 102
 
-PREC_FP64, PREC_FP32, PREC_BF16, PREC_MIXED = 0, 1, 2, 3
+PREC_FP64, PREC_FP32, PREC_MIXED = 0, 1, 3
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -38,11 +38,12 @@ class KmeansStats(ctypes.Structure):
 
 
 class IsdfOpts(ctypes.Structure):
-    _fields_ = [("precision", _i), ("refine_steps", _i), ("want_luv", _i)]
+    _fields_ = [("precision", _i), ("refine_steps", _i), ("want_luv", _i), ("check_rank", _i)]
 
 
 class IsdfStats(ctypes.Structure):
-    _fields_ = [("cct_chol_info", _i), ("luv_chol_info", _i), ("stage_ms", _d * 8)]
+    _fields_ = [("cct_chol_info", _i), ("luv_chol_info", _i), ("cct_rank", _i), ("cct_smax", _d),
+                ("stage_ms", _d * 8)]
 
 
 class System(ctypes.Structure):
@@ -68,6 +69,7 @@ SIGNATURES = {
     "thc_czt": (_i, [_vp, _vp, _l, _i, _vp, _i, _vp]),
     "thc_cct": (_i, [_vp, _vp, _l, _vp, _i, _vp]),
     "thc_cholesky": (_i, [_vp, _vp, _i, ctypes.POINTER(_i)]),
+    "thc_cct_rank": (_i, [_vp, _vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_d)]),
     "thc_least_squares": (_i, [_vp, _vp, _vp, _l, _i, _i, _vp, ctypes.POINTER(_i)]),
     "thc_fft_vectors": (_i, [_vp, _vp, _i, ctypes.POINTER(_i), _vp, _vp]),
     "thc_construct_muv": (_i, [_vp, _vp, _i, _l, _vp]),

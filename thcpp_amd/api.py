@@ -28,11 +28,31 @@ class Context:
     """One thc_ctx on one GPU (device index 'device')."""
 
     def __init__(self, device: int = 0, graphs: bool = True, stream=None):
+        """stream: a torch.cuda.Stream or raw hipStream_t handle for the library
+        to run on.  By default it is torch's current stream on the device; when
+        that is the legacy default stream (handle 0) the library creates a
+        blocking stream of its own, which HIP orders after torch's work there."""
         self.device = torch.device("cuda", device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+        elif isinstance(stream, torch.cuda.Stream):
+            stream = stream.cuda_stream
+        self._stream = int(stream or 0)
         h = ctypes.c_void_p()
-        L.check(L.lib().thc_ctx_create(device, stream, ctypes.byref(h)), "thc_ctx_create")
+        L.check(L.lib().thc_ctx_create(device, ctypes.c_void_p(self._stream or None), ctypes.byref(h)),
+                "thc_ctx_create")
         self.h = h
         L.check(L.lib().thc_ctx_set_graphs(self.h, 1 if graphs else 0), "thc_ctx_set_graphs")
+
+    def _lib(self):
+        """The library, after making torch's pending work on its current stream
+        visible to the context's stream: nothing to do when they are the same
+        stream or torch's is the legacy default stream (the context's own
+        stream is blocking); otherwise wait for torch's stream."""
+        cur = torch.cuda.current_stream(self.device).cuda_stream
+        if cur != 0 and cur != self._stream:
+            torch.cuda.current_stream(self.device).synchronize()
+        return L.lib()
 
     def close(self):
         if self.h:
@@ -62,20 +82,20 @@ class Context:
     def classify_grid_points(self, grid, centroids):
         g, c = self.f64(grid), self.f64(centroids)
         out = torch.empty(g.shape[0], dtype=torch.int32, device=self.device)
-        L.check(L.lib().thc_classify_grid_points(self.h, _p(g), g.shape[0], _p(c), c.shape[0], _p(out)),
+        L.check(self._lib().thc_classify_grid_points(self.h, _p(g), g.shape[0], _p(c), c.shape[0], _p(out)),
                 "thc_classify_grid_points")
         return out
 
     def update_centroids(self, rho, grid, grid_map, k):
         r, g, m = self.f64(rho).reshape(-1), self.f64(grid), self.i32(grid_map)
         out = torch.empty((k, 3), dtype=torch.float64, device=self.device)
-        st = L.lib().thc_update_centroids(self.h, _p(r), _p(g), g.shape[0], _p(m), k, _p(out))
+        st = self._lib().thc_update_centroids(self.h, _p(r), _p(g), g.shape[0], _p(m), k, _p(out))
         return out, st
 
     def map_to_grid(self, grid, centroids):
         g, c = self.f64(grid), self.f64(centroids)
         out = torch.empty(c.shape[0], dtype=torch.int32, device=self.device)
-        st = L.lib().thc_map_to_grid(self.h, _p(g), g.shape[0], _p(c), c.shape[0], _p(out))
+        st = self._lib().thc_map_to_grid(self.h, _p(g), g.shape[0], _p(c), c.shape[0], _p(out))
         L.check(st, "thc_map_to_grid", allow=(L.THC_WARN_DUPLICATE_INDICES,))
         return out, st
 
@@ -94,7 +114,7 @@ class Context:
         init = None
         if init_idx is not None:
             init = np.ascontiguousarray(init_idx, dtype=np.int32)
-        st = L.lib().thc_kmeans(self.h, _p(g), _p(r), n, k, max_it, threshold, seed,
+        st = self._lib().thc_kmeans(self.h, _p(g), _p(r), n, k, max_it, threshold, seed,
                                 init.ctypes.data_as(ctypes.c_void_p) if init is not None else None,
                                 _p(idx), _p(gm), _p(cen), ctypes.byref(stats))
         it = stats.iterations
@@ -111,7 +131,7 @@ class Context:
         n, m = a.shape
         idx = torch.empty(k, dtype=torch.int32, device=self.device)
         rd = np.zeros(k)
-        st = L.lib().thc_qrcp_points(self.h, _p(a), _p(ah), n, m, ah.shape[1] if ah is not None else 0, k, _p(idx),
+        st = self._lib().thc_qrcp_points(self.h, _p(a), _p(ah), n, m, ah.shape[1] if ah is not None else 0, k, _p(idx),
                                      rd.ctypes.data_as(ctypes.c_void_p))
         return st, idx, rd
 
@@ -119,7 +139,7 @@ class Context:
     def czt(self, aoR, idx):
         a, i = self.f64(aoR), self.i32(idx)
         out = torch.empty((i.shape[0], a.shape[0]), dtype=torch.float64, device=self.device)
-        L.check(L.lib().thc_czt(self.h, _p(a), a.shape[0], a.shape[1], _p(i), i.shape[0], _p(out)), "thc_czt")
+        L.check(self._lib().thc_czt(self.h, _p(a), a.shape[0], a.shape[1], _p(i), i.shape[0], _p(out)), "thc_czt")
         return out
 
     @staticmethod
@@ -132,7 +152,7 @@ class Context:
         a, i = self.f64(self._interleave(aoR)), self.i32(idx)
         n, m = a.shape[0], a.shape[1] // 2
         out = torch.empty((i.shape[0], n), dtype=torch.float64, device=self.device)
-        L.check(L.lib().thc_czt_complex(self.h, _p(a), n, m, _p(i), i.shape[0], _p(out)), "thc_czt_complex")
+        L.check(self._lib().thc_czt_complex(self.h, _p(a), n, m, _p(i), i.shape[0], _p(out)), "thc_czt_complex")
         return out
 
     def isdf_complex(self, aoR, idx, fft_grid, coulG, precision=L.PREC_FP64, want_luv=True, want_xi=False):
@@ -145,34 +165,43 @@ class Context:
         opts = L.IsdfOpts(precision, 0, 1 if want_luv else 0)
         stats = L.IsdfStats()
         fg = (ctypes.c_int * 3)(*[int(q) for q in fft_grid])
-        st = L.lib().thc_isdf_complex(self.h, _p(a), n, m, _p(i), k, fg, _p(cg), ctypes.byref(opts), _p(v), _p(luv),
+        st = self._lib().thc_isdf_complex(self.h, _p(a), n, m, _p(i), k, fg, _p(cg), ctypes.byref(opts), _p(v), _p(luv),
                                       _p(xi), ctypes.byref(stats))
         return {"status": st, "V": v, "Luv": luv, "xi": xi, "stage_ms": list(stats.stage_ms)}
 
     def czt_half(self, aoR, aoR_half, idx):
         a, ah, i = self.f64(aoR), self.f64(aoR_half), self.i32(idx)
         out = torch.empty((i.shape[0], a.shape[0]), dtype=torch.float64, device=self.device)
-        L.check(L.lib().thc_czt_half(self.h, _p(a), _p(ah), a.shape[0], a.shape[1], ah.shape[1], _p(i),
+        L.check(self._lib().thc_czt_half(self.h, _p(a), _p(ah), a.shape[0], a.shape[1], ah.shape[1], _p(i),
                                      i.shape[0], _p(out)), "thc_czt_half")
         return out
 
     def cct(self, czt, idx):
         z, i = self.f64(czt), self.i32(idx)
         out = torch.empty((i.shape[0], i.shape[0]), dtype=torch.float64, device=self.device)
-        L.check(L.lib().thc_cct(self.h, _p(z), z.shape[1], _p(i), i.shape[0], _p(out)), "thc_cct")
+        L.check(self._lib().thc_cct(self.h, _p(z), z.shape[1], _p(i), i.shape[0], _p(out)), "thc_cct")
         return out
+
+    def cct_rank(self, cct):
+        """Numerical rank of CC^T under MatrixOperations::rank's rule
+        (src/matrix_operations.h:713-735) and its largest singular value."""
+        c = self.f64(cct)
+        rank, smax = ctypes.c_int(0), ctypes.c_double(0.0)
+        st = self._lib().thc_cct_rank(self.h, _p(c), c.shape[0], ctypes.byref(rank), ctypes.byref(smax))
+        L.check(st, "thc_cct_rank", allow=(L.THC_WARN_RANK_DEFICIENT,))
+        return rank.value, smax.value, st
 
     def cholesky(self, a):
         m = self.f64(a).clone()
         info = ctypes.c_int(0)
-        L.check(L.lib().thc_cholesky(self.h, _p(m), m.shape[0], ctypes.byref(info)), "thc_cholesky")
+        L.check(self._lib().thc_cholesky(self.h, _p(m), m.shape[0], ctypes.byref(info)), "thc_cholesky")
         return m, info.value
 
     def least_squares(self, cct, czt, refine=0):
         c, z = self.f64(cct), self.f64(czt)
         out = torch.empty_like(z)
         info = ctypes.c_int(0)
-        st = L.lib().thc_least_squares(self.h, _p(c), _p(z), z.shape[1], z.shape[0], refine, _p(out),
+        st = self._lib().thc_least_squares(self.h, _p(c), _p(z), z.shape[1], z.shape[0], refine, _p(out),
                                        ctypes.byref(info))
         L.check(st, "thc_least_squares")
         return out
@@ -182,29 +211,31 @@ class Context:
         n = (ctypes.c_int * 3)(*[int(v) for v in fft_grid])
         g2 = 2 * int(fft_grid[0]) * int(fft_grid[1]) * (int(fft_grid[2]) // 2 + 1)
         out = torch.empty((xx.shape[0], g2), dtype=torch.float64, device=self.device)
-        L.check(L.lib().thc_fft_vectors(self.h, _p(xx), xx.shape[0], n, _p(cg), _p(out)), "thc_fft_vectors")
+        L.check(self._lib().thc_fft_vectors(self.h, _p(xx), xx.shape[0], n, _p(cg), _p(out)), "thc_fft_vectors")
         return out
 
     def construct_muv(self, b):
         bb = self.f64(b)
         out = torch.empty((bb.shape[0], bb.shape[0]), dtype=torch.float64, device=self.device)
-        L.check(L.lib().thc_construct_muv(self.h, _p(bb), bb.shape[0], bb.shape[1], _p(out)), "thc_construct_muv")
+        L.check(self._lib().thc_construct_muv(self.h, _p(bb), bb.shape[0], bb.shape[1], _p(out)), "thc_construct_muv")
         return out
 
-    def isdf(self, aoR, idx, fft_grid, coulG, precision=L.PREC_FP64, refine=0, want_luv=True, want_xi=False):
+    def isdf(self, aoR, idx, fft_grid, coulG, precision=L.PREC_FP64, refine=0, want_luv=True, want_xi=False,
+             check_rank=False):
         a, i, cg = self.f64(aoR), self.i32(idx), self.f64(coulG).reshape(-1)
         n, m = a.shape
         k = i.shape[0]
         v = torch.empty((k, k), dtype=torch.float64, device=self.device)
         luv = torch.empty((k, k), dtype=torch.float64, device=self.device) if want_luv else None
         xi = torch.empty((k, n), dtype=torch.float64, device=self.device) if want_xi else None
-        opts = L.IsdfOpts(precision, refine, 1 if want_luv else 0)
+        opts = L.IsdfOpts(precision, refine, 1 if want_luv else 0, 1 if check_rank else 0)
         stats = L.IsdfStats()
         fg = (ctypes.c_int * 3)(*[int(q) for q in fft_grid])
-        st = L.lib().thc_isdf(self.h, _p(a), n, m, _p(i), k, fg, _p(cg), ctypes.byref(opts), _p(v), _p(luv),
+        st = self._lib().thc_isdf(self.h, _p(a), n, m, _p(i), k, fg, _p(cg), ctypes.byref(opts), _p(v), _p(luv),
                               _p(xi), ctypes.byref(stats))
         return {"status": st, "V": v, "Luv": luv, "xi": xi, "cct_info": stats.cct_chol_info,
-                "luv_info": stats.luv_chol_info, "stage_ms": list(stats.stage_ms)}
+                "luv_info": stats.luv_chol_info, "cct_rank": stats.cct_rank, "cct_smax": stats.cct_smax,
+                "stage_ms": list(stats.stage_ms)}
 
     def isdf_half(self, aoR, aoR_half, idx, fft_grid, coulG, precision=L.PREC_FP64, refine=0, want_xi=False):
         a, ah, i, cg = self.f64(aoR), self.f64(aoR_half), self.i32(idx), self.f64(coulG).reshape(-1)
@@ -215,7 +246,7 @@ class Context:
         opts = L.IsdfOpts(precision, refine, 0)
         stats = L.IsdfStats()
         fg = (ctypes.c_int * 3)(*[int(q) for q in fft_grid])
-        st = L.lib().thc_isdf_half(self.h, _p(a), _p(ah), n, m, ah.shape[1], _p(i), k, fg, _p(cg),
+        st = self._lib().thc_isdf_half(self.h, _p(a), _p(ah), n, m, ah.shape[1], _p(i), k, fg, _p(cg),
                                    ctypes.byref(opts), _p(muv), _p(xi), ctypes.byref(stats))
         return {"status": st, "Muv": muv, "xi": xi, "cct_info": stats.cct_chol_info,
                 "stage_ms": list(stats.stage_ms)}
@@ -231,7 +262,7 @@ class Context:
         opts = L.IsdfOpts(precision, refine, 1 if want_luv else 0)
         ks, ist = L.KmeansStats(), L.IsdfStats()
         fg = (ctypes.c_int * 3)(*[int(q) for q in fft_grid])
-        st = L.lib().thc_factorize(self.h, _p(g), _p(r), _p(a), n, m, thc_cfac, max_it, threshold, seed, fg,
+        st = self._lib().thc_factorize(self.h, _p(g), _p(r), _p(a), n, m, thc_cfac, max_it, threshold, seed, fg,
                                    _p(cg), ctypes.byref(opts), _p(idx), _p(v), _p(luv), ctypes.byref(ks),
                                    ctypes.byref(ist))
         return {"status": st, "interp_idx": idx, "V": v, "Luv": luv, "kmeans_ms": ks.elapsed_ms,
@@ -261,7 +292,7 @@ class Context:
             q.luv = luv.data_ptr() if luv is not None else None
             outs.append({"interp_idx": idx, "V": v, "Luv": luv})
         opts = L.IsdfOpts(precision, refine, 1 if want_luv else 0)
-        st = L.lib().thc_factorize_batched(self.h, len(systems), arr, ctypes.byref(opts), max_streams)
+        st = self._lib().thc_factorize_batched(self.h, len(systems), arr, ctypes.byref(opts), max_streams)
         for i, o in enumerate(outs):
             o["status"] = arr[i].status
             o["iters"] = arr[i].kstats.iterations

@@ -104,11 +104,14 @@ struct IsdfBufs {
   double* w;            // i Phi for complex orbitals (n x 2m), else null
   void* b;      // weighted half spectrum (f64 or f32)
   void *chol_w, *inv_w, *fft_w, *syrk_w;
+  void* solve_w;        // digit planes of the int8 solve (mixed precision)
+  void* rank_w;         // tridiagonal reduction of CC^T (rank diagnostic)
   int* info;
 };
 
 // m is the real row width of Phi (2 x orbitals for interleaved complex ones)
-size_t isdf_bytes(long n, int k, int m, int m_occ, bool cplx, const int fft[3], int prec, int refine, bool own_x) {
+size_t isdf_bytes(long n, int k, int m, int m_occ, bool cplx, const int fft[3], int prec, int refine, bool own_x,
+                  bool rank) {
   const long g2 = half_spectrum_len(fft);
   const size_t kk = (size_t)k * k;
   size_t b = carve_bytes<double>((size_t)k * m) + carve_bytes<double>((size_t)k * std::max(m_occ, 1));
@@ -121,12 +124,14 @@ size_t isdf_bytes(long n, int k, int m, int m_occ, bool cplx, const int fft[3], 
   const int fprec = prec == 0 ? 0 : 1;  // FFT / spectrum precision
   b += chol_work_bytes(k) + 256 + spd_inverse_work_bytes(k) + 256 + fft_work_bytes(k, fft, fprec) + 256 +
        (prec == THC_PREC_MIXED ? oz_syrk_work_bytes(k, g2) : syrk_work_bytes(k, g2, fprec)) + 256;
+  if (prec == THC_PREC_MIXED) b += oz_solve_work_bytes(k, n) + 256;
+  if (rank) b += tridiag_work_bytes(k) + 256;
   b += carve_bytes<int>(16);
   return b;
 }
 
 bool isdf_carve(Arena& a, long n, int k, int m, int m_occ, bool cplx, const int fft[3], int prec, int refine,
-                double* xi, IsdfBufs& B) {
+                double* xi, bool rank, IsdfBufs& B) {
   const long g2 = half_spectrum_len(fft);
   const size_t kk = (size_t)k * k;
   B = IsdfBufs{};
@@ -148,9 +153,12 @@ bool isdf_carve(Arena& a, long n, int k, int m, int m_occ, bool cplx, const int 
   const int fprec = prec == 0 ? 0 : 1;
   B.fft_w = a.take<char>(fft_work_bytes(k, fft, fprec));
   B.syrk_w = a.take<char>(prec == THC_PREC_MIXED ? oz_syrk_work_bytes(k, g2) : syrk_work_bytes(k, g2, fprec));
+  B.solve_w = prec == THC_PREC_MIXED ? a.take<char>(oz_solve_work_bytes(k, n)) : nullptr;
+  B.rank_w = rank ? a.take<char>(tridiag_work_bytes(k)) : nullptr;
   B.info = a.take<int>(16);
   return B.phim && B.phio && B.z && B.x && B.c && B.l && B.ci && B.v && (refine <= 0 || B.r) && B.b &&
-         B.chol_w && B.inv_w && B.fft_w && B.syrk_w && B.info;
+         B.chol_w && B.inv_w && B.fft_w && B.syrk_w && (prec != THC_PREC_MIXED || B.solve_w) &&
+         (!rank || B.rank_w) && B.info;
 }
 
 __global__ void zero_ints(int* p, int count) {
@@ -174,18 +182,21 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   const int prec = opts ? opts->precision : THC_PREC_FP64;
   const int refine = opts ? std::max(0, opts->refine_steps) : 0;
   const int want_luv = opts ? opts->want_luv : 1;
+  const bool want_rank = opts && opts->check_rank;
   if (prec != THC_PREC_FP64 && prec != THC_PREC_FP32 && prec != THC_PREC_MIXED) return THC_ERR_UNSUPPORTED;
   const long nfft = (long)fft[0] * fft[1] * fft[2];
   if (fft[0] < 1 || fft[1] < 1 || fft[2] < 1) return THC_ERR_ARG;
   // The reference only warns when prod(fft_grid) != N (interpolating_vectors.cpp:368-370) and then
-  // transforms N-point columns with the wrong shape; here that case is refused.
+  // transforms a prefix of each column (or runs past it); here that case is refused.
   if (nfft != n) return THC_ERR_ARG;
   THC_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  THC_TRY(ctx->ws.reserve(isdf_bytes(n, k, m, m_occ, cplx, fft, prec, refine, xi_out == nullptr)));
+  if (want_rank && k > THC_MAX_CLUSTERS) return THC_ERR_UNSUPPORTED;
+  THC_TRY(ctx->ws.reserve(isdf_bytes(n, k, m, m_occ, cplx, fft, prec, refine, xi_out == nullptr, want_rank)));
   ctx->ws.reset();
   IsdfBufs B;
-  if (!isdf_carve(ctx->ws, n, k, m, m_occ, cplx, fft, prec, refine, xi_out, B)) return THC_ERR_NO_MEMORY;
+  if (!isdf_carve(ctx->ws, n, k, m, m_occ, cplx, fft, prec, refine, xi_out, want_rank, B)) return THC_ERR_NO_MEMORY;
+  if (want_rank) THC_TRY(ensure_pinned(ctx, sizeof(double) * 2 * (size_t)k));
   const long g2 = half_spectrum_len(fft);
   hipEvent_t* ev = ctx->ev;
 
@@ -237,6 +248,10 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   THC_TRY(cholesky_f64(B.l, k, k, B.info, B.chol_w, sa));
   THC_TRY(spd_inverse_from_chol(B.l, k, k, B.ci, B.inv_w, sa));
   THC_HIP(hipEventRecord(ctx->join, sa));
+  // rank diagnostic (IVecs::check_rank) after the join: nothing on the main
+  // stream waits for it; its tridiagonal form lands in pinned host memory
+  double* rk_d = reinterpret_cast<double*>(ctx->pinned);
+  if (want_rank) THC_TRY(sym_tridiag(B.c, k, k, rk_d, rk_d + k, B.rank_w, sa));
   // 1. Zt = (Phi Phi_mu^T)^2, N x K row-major (Zt[r][mu] = ZC^T[mu][r]): B rows
   //    gathered by idx, epilogue squares; this layout lets the solve read both
   //    operands along k and keeps the 4 GB store row-contiguous
@@ -273,7 +288,10 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   // so when nothing else reads X (no xi output, no refinement) the solve
   // stores it as FP32 directly: the same values, half the bytes.
   const int x_f32 = prec != THC_PREC_FP64 && xi_out == nullptr && refine == 0;
-  {
+  if (prec == THC_PREC_MIXED) {
+    // exact int8 digit products of the scaled operands (ozaki.hip)
+    THC_TRY(oz_solve(B.ci, k, B.z, k, n, idx, aoR, m, aoR_half, aoR_half ? m_occ : 0, B.x, x_f32, n, B.solve_w, s));
+  } else {
     GemmArgs g = {};
     g.m = k; g.n = n; g.k = k;
     g.a = B.ci; g.lda = k; g.a_trans = 0;
@@ -323,9 +341,17 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   int info[4] = {0, 0, 0, 0};
   THC_HIP(hipMemcpyAsync(info, B.info, sizeof(info), hipMemcpyDeviceToHost, s));
   THC_HIP(hipStreamSynchronize(s));
+  int cct_rank = -1;
+  double cct_smax = 0.0;
+  if (want_rank) {
+    THC_HIP(hipStreamSynchronize(sa));
+    tridiag_rank(rk_d, rk_d + k, k, &cct_rank, &cct_smax);
+  }
   if (stats) {
     stats->cct_chol_info = info[0];
     stats->luv_chol_info = info[1];
+    stats->cct_rank = cct_rank;
+    stats->cct_smax = cct_smax;
     float t;
     const int map[6][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {5, 6}, {6, 7}};
     for (int q = 0; q < 6; ++q) {
@@ -340,6 +366,7 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   if (info[2]) return THC_ERR_ARG;
   if (info[0]) return THC_ERR_NOT_POSDEF;
   if (info[1] && want_luv && luv_out) return THC_WARN_LUV_NOT_POSDEF;
+  if (want_rank && cct_rank < k) return THC_WARN_RANK_DEFICIENT;
   return THC_OK;
 }
 
@@ -366,7 +393,7 @@ const char* thc_status_string(int st) {
     case THC_ERR_UNSUPPORTED: return "unsupported size";
     case THC_WARN_DUPLICATE_INDICES: return "WARNING: Found repeated interpolation indices";
     case THC_WARN_LUV_NOT_POSDEF: return "WARNING: Cholesky decomposition of Muv failed";
-    case THC_WARN_FFT_GRID_MISMATCH: return "WARNING: FFT grid not consistent with number of real space grid points";
+    case THC_WARN_RANK_DEFICIENT: return "WARNING: CCt matrix is rank deficient";
     default: return "unknown status";
   }
 }
@@ -383,7 +410,9 @@ int thc_ctx_create(int device, void* stream, thc_ctx** out) {
   if (stream) {
     c->stream = reinterpret_cast<hipStream_t>(stream);
   } else {
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    // blocking: ordered after earlier work on the legacy default stream, where
+    // hosts that pass no stream (torch among them) produce the inputs
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault);
     if (e != hipSuccess) {
       delete c;
       set_last_error(hipGetErrorString(e), __FILE__, __LINE__);
@@ -577,6 +606,24 @@ int thc_cholesky(thc_ctx* ctx, double* a, int k, int* info) {
   THC_HIP(hipStreamSynchronize(ctx->stream));
   if (info) *info = h;
   return THC_OK;
+}
+
+int thc_cct_rank(thc_ctx* ctx, const double* cct, int k, int* rank, double* smax) {
+  if (!ctx || !cct || !rank || k < 1) return THC_ERR_ARG;
+  if (k > THC_MAX_CLUSTERS) return THC_ERR_UNSUPPORTED;
+  THC_HIP(hipSetDevice(ctx->device));
+  THC_TRY(ctx->ws.reserve(tridiag_work_bytes(k) + 256));
+  ctx->ws.reset();
+  void* w = ctx->ws.take<char>(tridiag_work_bytes(k));
+  if (!w) return THC_ERR_NO_MEMORY;
+  THC_TRY(ensure_pinned(ctx, sizeof(double) * 2 * (size_t)k));
+  double* d = reinterpret_cast<double*>(ctx->pinned);
+  THC_TRY(sym_tridiag(cct, k, k, d, d + k, w, ctx->stream));
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  double s0 = 0.0;
+  tridiag_rank(d, d + k, k, rank, &s0);
+  if (smax) *smax = s0;
+  return *rank < k ? THC_WARN_RANK_DEFICIENT : THC_OK;
 }
 
 int thc_least_squares(thc_ctx* ctx, const double* cct, const double* czt, long n, int k, int refine_steps,

@@ -135,6 +135,15 @@ int cholesky_f64(double* a, int k, long lda, int* d_info, void* work, hipStream_
 size_t spd_inverse_work_bytes(int k);
 // ainv = A^-1 from its lower Cholesky factor l (k x k, lda).
 int spd_inverse_from_chol(const double* l, int k, long lda, double* ainv, void* work, hipStream_t s);
+// Householder reduction of a symmetric k x k matrix (k <= 4096) to tridiagonal
+// form; the diagonal (k) and off-diagonal (k - 1) are copied to host arrays
+// asynchronously on s (valid once s has drained).
+size_t tridiag_work_bytes(int k);
+int sym_tridiag(const double* a, int k, long lda, double* d_host, double* e_host, void* work, hipStream_t s);
+// Rank by the rule of MatrixOperations::rank (src/matrix_operations.h:713-735):
+// singular values below k * s0 * 1e-16 count as zero; s0 = largest.  Host code
+// on the tridiagonal form (Sturm counts and bisection).
+void tridiag_rank(const double* d, const double* e, int k, int* rank, double* smax);
 int zero_upper(double* a, int k, long lda, hipStream_t s);
 int symmetrize_from_lower(double* a, int k, long lda, hipStream_t s);
 
@@ -143,6 +152,13 @@ int symmetrize_from_lower(double* a, int k, long lda, hipStream_t s);
 // exact int8 digit products, written full symmetric into v.
 size_t oz_syrk_work_bytes(int k, long g2);
 int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s);
+// X = ainv * Z (k x n) from Zt (n x k row-major, ZC^T transposed) by exact
+// int8 digit products of the scaled operands ainv diag(s(idx)) and
+// Zt / (s(r) s(idx)), s(r) = sqrt(sum phi_a(r)^2 sum phi_b(r)^2) (phi_b null:
+// sum phi_a(r)^2).  X is float (x_f32) or double with row stride ldx.
+size_t oz_solve_work_bytes(int k, long n);
+int oz_solve(const double* ainv, long lda, const double* zt, int k, long n, const int* idx, const double* phi_a,
+             int ma, const double* phi_b, int mb, void* x, int x_f32, long ldx, void* work, hipStream_t s);
 
 // ---- QR with column pivoting (qrcp.hip) ---------------------------------------
 // First k pivots of QRCP of the orbital-product matrix (occ = nullptr: all

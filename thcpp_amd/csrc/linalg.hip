@@ -11,6 +11,7 @@
 // forms Linv21 = -Linv22 (L21 Linv11) for every pair of blocks of one size in
 // two batched GEMMs per level; finally A^-1 = Linv^T Linv.
 #include <algorithm>
+#include <cmath>
 #include <utility>
 
 #include "common.h"
@@ -403,6 +404,122 @@ __global__ __launch_bounds__(256) void fill_zero(double* a, long count) {
 
 inline int grid_for(long count) { return (int)std::min<long>(4096, std::max<long>(1, (count + 255) / 256)); }
 
+// ---- symmetric tridiagonal reduction (rank of CC^T) -------------------------
+// Householder reduction A = Q T Q^T of a symmetric k x k matrix, one launch
+// per column j.  Launch j applies the rank-2 update of reflection j - 1
+// (A22 -= v w^T + w v^T, w = p - (beta/2)(p.v) v) to the rows it owns, forms
+// the updated row j in every workgroup (row j is read, never written, in this
+// launch), takes reflection j from it (v = x - alpha e1, beta = 1/(alpha
+// (alpha - x0)), alpha = -sign(x0) |x|), and multiplies its updated rows by v
+// (p = beta A22 v).  The diagonal d and off-diagonal e of T go to d, e.
+constexpr int TD_MAXK = 4096;
+
+__device__ inline double td_block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void tridiag_step(double* __restrict__ W, int k, int j,
+                                                    const double* __restrict__ vp, const double* __restrict__ pp,
+                                                    const double* __restrict__ bp, double* __restrict__ vn,
+                                                    double* __restrict__ pn, double* __restrict__ bn,
+                                                    double* __restrict__ d, double* __restrict__ e) {
+  __shared__ double sv[TD_MAXK], sw[TD_MAXK], sx[TD_MAXK];
+  __shared__ double red[4];
+  const int t = threadIdx.x;
+  const bool upd = j > 0;
+  if (upd) {
+    double g = 0.0;
+    for (int c = j + t; c < k; c += 256) g += pp[c] * vp[c];
+    g = td_block_sum(g, red);
+    const double hb = 0.5 * bp[0] * g;
+    for (int c = j + t; c < k; c += 256) {
+      sv[c] = vp[c];
+      sw[c] = pp[c] - hb * vp[c];
+    }
+    __syncthreads();
+  }
+  for (int c = j + t; c < k; c += 256) {
+    double x = W[(long)j * k + c];
+    if (upd) x -= sv[j] * sw[c] + sw[j] * sv[c];
+    sx[c] = x;
+  }
+  __syncthreads();
+  double nrm2 = 0.0;
+  for (int c = j + 1 + t; c < k; c += 256) nrm2 += sx[c] * sx[c];
+  nrm2 = td_block_sum(nrm2, red);
+  double alpha = 0.0, beta = 0.0;
+  if (j + 1 < k && nrm2 > 0.0) {
+    const double x0 = sx[j + 1];
+    alpha = x0 >= 0.0 ? -sqrt(nrm2) : sqrt(nrm2);
+    beta = 1.0 / (alpha * (alpha - x0));
+  }
+  __syncthreads();
+  if (t == 0 && j + 1 < k) sx[j + 1] -= alpha;  // v = x - alpha e1
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    if (t == 0) {
+      d[j] = sx[j];
+      if (j + 1 < k) e[j] = alpha;
+      bn[0] = beta;
+    }
+    for (int c = j + 1 + t; c < k; c += 256) vn[c] = sx[c];
+  }
+  for (int i = j + 1 + blockIdx.x; i < k; i += gridDim.x) {
+    double acc = 0.0;
+    double* row = W + (long)i * k;
+    for (int c = j + 1 + t; c < k; c += 256) {
+      double x = row[c];
+      if (upd) x -= sv[i] * sw[c] + sw[i] * sv[c];
+      row[c] = x;
+      acc += x * sx[c];
+    }
+    acc = td_block_sum(acc, red);
+    if (t == 0) pn[i] = beta * acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void copy_square(const double* a, long lda, int k, double* w) {
+  const long total = (long)k * k;
+  for (long q = blockIdx.x * 256L + threadIdx.x; q < total; q += 256L * gridDim.x) w[q] = a[(q / k) * lda + q % k];
+}
+
+// number of eigenvalues of the tridiagonal (d, e) below x (Sturm count of the
+// LDL^T pivots of T - x I)
+int sturm_count(const double* d, const double* e, int k, double x) {
+  int neg = 0;
+  double q = 1.0;
+  for (int i = 0; i < k; ++i) {
+    const double off = i > 0 ? e[i - 1] * e[i - 1] : 0.0;
+    q = d[i] - x - (i > 0 ? off / q : 0.0);
+    if (q == 0.0) q = -1e-300;  // an exact zero pivot counts as negative
+    if (q < 0.0) ++neg;
+  }
+  return neg;
+}
+
+// eigenvalue of rank 'idx' (0 = smallest) by bisection within the Gershgorin
+// interval, to the resolution of double
+double tridiag_eig(const double* d, const double* e, int k, int idx) {
+  double lo = 0.0, hi = 0.0;
+  for (int i = 0; i < k; ++i) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(e[i]) : 0.0);
+    lo = i == 0 ? d[i] - r : std::min(lo, d[i] - r);
+    hi = i == 0 ? d[i] + r : std::max(hi, d[i] + r);
+  }
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_count(d, e, k, mid) > idx) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
 }  // namespace
 
 size_t chol_work_bytes(int) { return 2 * align_up(sizeof(double) * CB * CB, 256); }
@@ -486,6 +603,57 @@ int spd_inverse_from_chol(const double* l, int k, long lda, double* ainv, void* 
   return THC_OK;
 }
 
+size_t tridiag_work_bytes(int k) {
+  return align_up(sizeof(double) * (size_t)k * k, 256) + 6 * align_up(sizeof(double) * (size_t)k, 256) +
+         2 * align_up(sizeof(double) * 4, 256) + 256;
+}
+
+int sym_tridiag(const double* a, int k, long lda, double* d_host, double* e_host, void* work, hipStream_t s) {
+  if (k < 1) return THC_OK;
+  if (k > TD_MAXK) return THC_ERR_UNSUPPORTED;
+  char* w = reinterpret_cast<char*>(work);
+  auto take = [&](size_t count) {
+    double* p = reinterpret_cast<double*>(w);
+    w += align_up(sizeof(double) * count, 256);
+    return p;
+  };
+  double* W = take((size_t)k * k);
+  double* vb[2] = {take(k), take(k)};
+  double* pb[2] = {take(k), take(k)};
+  double* bb[2] = {take(4), take(4)};
+  double* dd = take(k);
+  double* ee = take(k);
+  hipLaunchKernelGGL(copy_square, dim3(grid_for((long)k * k)), dim3(256), 0, s, a, lda, k, W);
+  THC_HIP(hipGetLastError());
+  for (int j = 0; j < k; ++j) {
+    const int cur = j & 1, prev = cur ^ 1;
+    const int rows = std::max(1, std::min(128, k - j - 1));
+    hipLaunchKernelGGL(tridiag_step, dim3(rows), dim3(256), 0, s, W, k, j, vb[prev], pb[prev], bb[prev], vb[cur],
+                       pb[cur], bb[cur], dd, ee);
+    THC_HIP(hipGetLastError());
+  }
+  THC_HIP(hipMemcpyAsync(d_host, dd, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+  if (k > 1) THC_HIP(hipMemcpyAsync(e_host, ee, sizeof(double) * (k - 1), hipMemcpyDeviceToHost, s));
+  return THC_OK;
+}
+
+void tridiag_rank(const double* d, const double* e, int k, int* rank, double* smax) {
+  if (k < 1) {
+    *rank = 0;
+    *smax = 0.0;
+    return;
+  }
+  // singular values of a symmetric matrix are |eigenvalues|; the largest is
+  // the larger magnitude of the two extreme eigenvalues
+  const double lmin = tridiag_eig(d, e, k, 0), lmax = tridiag_eig(d, e, k, k - 1);
+  const double s0 = std::max(std::fabs(lmin), std::fabs(lmax));
+  // MatrixOperations::rank: sigma_i < max(m, n) s0 1e-16 counts as zero
+  const double rcond = (double)k * s0 * 1e-16;
+  const int nul = sturm_count(d, e, k, rcond) - sturm_count(d, e, k, -rcond);
+  *rank = k - nul;
+  *smax = s0;
+}
+
 int zero_upper(double* a, int k, long lda, hipStream_t s) {
   hipLaunchKernelGGL(zero_upper_kernel, dim3(grid_for((long)k * k)), dim3(256), 0, s, a, k, lda);
   THC_HIP(hipGetLastError());
@@ -499,3 +667,4 @@ int symmetrize_from_lower(double* a, int k, long lda, hipStream_t s) {
 }
 
 }  // namespace thc
+

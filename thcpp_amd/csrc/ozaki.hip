@@ -22,6 +22,8 @@
 // in while the current step's MFMAs run.  Rows are 64 bytes with no padding;
 // 16-byte segment s of row r is stored at segment (s + r / 4) mod 4, which
 // keeps the 16-row fragment reads free of bank conflicts.
+#include <algorithm>
+
 #include "common.h"
 
 namespace thc {
@@ -334,6 +336,371 @@ int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ld
   THC_HIP(hipGetLastError());
   hipLaunchKernelGGL(oz_reduce, dim3((unsigned)(ntiles * (OZ_T / RB) * (OZ_T / RB))), dim3(256), 0, s, part, nchunk,
                      ntiles, k, v, ldv);
+  THC_HIP(hipGetLastError());
+  return THC_OK;
+}
+
+}  // namespace thc
+
+namespace thc {
+namespace {
+
+// ============================================================================
+// Least-squares application X = CC^T^-1 ZC^T in exact int8 digit products.
+//
+// The product is scaled so that both operands have a known range:
+//   A'[mu][nu] = (CC^T)^-1[mu][nu] s[nu]          (s = s(r_nu) at the points)
+//   Zs[r][nu]  = ZC^T[nu][r] / (s(r_nu) s(r))     |Zs| <= 1 by Cauchy-Schwarz
+//   X[mu][r]   = s(r) sum_nu A'[mu][nu] Zs[r][nu]
+// with s(r) = sqrt(rho_a(r) rho_b(r)), rho = sum of squared orbitals of the two
+// orbital sets whose pair products form ZC^T (rho itself when they are the
+// same set).  Each row of A' gets one power-of-two exponent and SV_D signed
+// 7-bit digits; Zs/2 is cut into SV_D digits with a fixed exponent (no row
+// maximum is needed).  The digit products with p + q < SV_D are accumulated
+// exactly in int32 by v_mfma_i32_32x32x32_i8, grouped by p + q, and combined
+// in FP64 in the epilogue; the dropped products are below 2^-7(SV_D+1) of
+// |A'| |Zs| per term.  Seven digits hold the interpolating vectors of the
+// 64-atom, 2048-point system to 1.4e-7 and V to 5e-8 of the FP64 solve (six
+// digits give 5.6e-6 in V).
+//
+// Digit planes are k-blocked: byte (row, k) of plane p lives at
+//   p * rows * kp + (k / 32) * rows * 32 + row * 32 + k % 32,
+// so the 32 contraction steps of 32 consecutive rows that one stage of the
+// GEMM needs are one contiguous KB.  Row counts are padded (A to 128, Z to 64
+// rows) with zero digits, so no load needs a bounds check.
+//
+// Tile: 128 (mu) x 64 (r) per 512-thread workgroup, 8 waves in a 4 x 2 grid of
+// 32 x 32 outputs (one MFMA block, 7 accumulator groups = 112 registers).
+// Stages of 32 contraction steps (7 planes x (128 + 64) rows x 32 bytes =
+// 42 KB) circulate through three LDS buffers filled by direct global-to-LDS
+// loads, two stages ahead of the MFMAs; the barrier between stages waits only
+// for this wave's loads of the stage about to be read.
+// ============================================================================
+
+constexpr int SV_D = 7;
+constexpr int SV_TM = 128, SV_TN = 64, SV_KS = 32;
+constexpr int SV_ABLK = SV_D * SV_TM * SV_KS / 1024;  // 1 KB blocks of 32 rows x 32 bytes
+constexpr int SV_BBLK = SV_D * SV_TN * SV_KS / 1024;
+constexpr int SV_NBLK = SV_ABLK + SV_BBLK;           // 42 per stage
+constexpr int SV_LPW = (SV_NBLK + 7) / 8;            // loads a wave issues per stage (6)
+static_assert((long)SV_D * 4096 * 127 * 127 < 2147483647L, "int32 group sums must not overflow for K <= 4096");
+
+typedef int i16v __attribute__((ext_vector_type(16)));
+
+// physical 16-byte segment of logical segment ls in row r of a 32-row block:
+// rows 4 apart in the same bank group alternate segments, so an 8- or 16-lane
+// phase of a fragment read touches every bank once
+__device__ inline int sv_seg(int r, int ls) { return ls ^ ((r >> 2) & 1) ^ ((r >> 3) & 1); }
+
+// byte offset of (row, k) in a k-blocked plane of 'rows' rows
+__device__ inline long sv_off(long row, long k, long rows) { return (k >> 5) * rows * 32 + row * 32 + (k & 31); }
+
+// s at an interpolation point; indices outside the grid (refused by the caller
+// afterwards) read as 0
+__device__ inline double sv_at(const double* s, const int* idx, int j, long n) {
+  const int i = idx[j];
+  return (i >= 0 && i < n) ? s[i] : 0.0;
+}
+
+// A'[mu][nu] = a[mu][nu] s[idx[nu]] -> digits SA (k-blocked, kr rows, kp
+// columns, zero padded) and exponent EA[mu].  One workgroup per row mu < kr;
+// rows >= k are zero.  Thread t takes columns 4 t .. 4 t + 3 of every 1024.
+__global__ __launch_bounds__(256) void sv_split_rows(const double* __restrict__ a, long lda, int k, int kr, int kp,
+                                                     const double* __restrict__ s, long n,
+                                                     const int* __restrict__ idx, signed char* __restrict__ SA,
+                                                     int* __restrict__ EA) {
+  __shared__ double red[4];
+  const int row = blockIdx.x;
+  const bool live = row < k;
+  double mx = 0.0;
+  if (live)
+    for (int j = threadIdx.x; j < k; j += 256) mx = fmax(mx, fabs(a[(long)row * lda + j] * sv_at(s, idx, j, n)));
+  mx = block_max(mx, red);
+  int ex = 0;
+  if (mx > 0.0) frexp(mx, &ex);
+  if (threadIdx.x == 0) EA[row] = ex;
+  const long plane = (long)kr * kp;
+  for (int j0 = 4 * threadIdx.x; j0 < kp; j0 += 1024) {
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = j0 + u;
+      x[u] = (live && j < k) ? ldexp(a[(long)row * lda + j] * sv_at(s, idx, j, n), -ex) : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < SV_D; ++p) {
+      unsigned w = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        x[u] *= 128.0;
+        const double d = trunc(x[u]);
+        x[u] -= d;
+        w |= ((unsigned)(int)d & 0xffu) << (8 * u);
+      }
+      *reinterpret_cast<unsigned*>(SA + p * plane + sv_off(row, j0, kr)) = w;
+    }
+  }
+}
+
+// inv2[j] = 1 / (2 s[idx[j]]) for j < k (0 where s = 0 or j >= k), j < kp
+__global__ __launch_bounds__(256) void sv_point_scale(const double* __restrict__ s, long n,
+                                                      const int* __restrict__ idx, int k, int kp,
+                                                      double* __restrict__ inv2) {
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < kp; j += 256 * gridDim.x) {
+    const double v = j < k ? sv_at(s, idx, j, n) : 0.0;
+    inv2[j] = v > 0.0 ? 0.5 / v : 0.0;
+  }
+}
+
+// Zs[r][nu] / 2 = zt[r][nu] inv2[nu] / s[r] -> digits SZ (k-blocked, nr rows).
+// Block: 64 rows x one 32-column chunk; thread t takes row t / 4, columns
+// 8 (t % 4) .. + 7 of the chunk (two 32-byte reads, one 8-byte store per
+// plane), so a wave stores 16 rows x 32 bytes contiguous per plane.  Rows
+// with s[r] = 0 (their ZC^T column is zero too) and padding are zero.
+__global__ __launch_bounds__(256) void sv_split_zt(const double* __restrict__ zt, long n, int k, long nr, int kp,
+                                                   const double* __restrict__ s, const double* __restrict__ inv2,
+                                                   signed char* __restrict__ SZ) {
+  const long r = (long)blockIdx.x * 64 + (threadIdx.x >> 2);
+  const int j0 = blockIdx.y * 32 + 8 * (threadIdx.x & 3);
+  const double sr = r < n ? s[r] : 0.0;
+  const double isr = sr > 0.0 ? 1.0 / sr : 0.0;
+  double x[8];
+  if (r < n && j0 + 8 <= k && (k & 1) == 0) {
+    const double2* src = reinterpret_cast<const double2*>(zt + r * k + j0);
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const double2 v = src[h];
+      x[2 * h] = v.x * inv2[j0 + 2 * h] * isr;
+      x[2 * h + 1] = v.y * inv2[j0 + 2 * h + 1] * isr;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = (r < n && j0 + u < k) ? zt[r * k + j0 + u] * inv2[j0 + u] * isr : 0.0;
+  }
+  const long plane = nr * kp;
+  const long off = sv_off(r, j0, nr);
+#pragma unroll
+  for (int q = 0; q < SV_D; ++q) {
+    unsigned w0 = 0, w1 = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      x[u] *= 128.0;
+      const double d = trunc(x[u]);
+      x[u] -= d;
+      if (u < 4) w0 |= ((unsigned)(int)d & 0xffu) << (8 * u);
+      else w1 |= ((unsigned)(int)d & 0xffu) << (8 * (u - 4));
+    }
+    *reinterpret_cast<uint2*>(SZ + q * plane + off) = make_uint2(w0, w1);
+  }
+}
+
+// s[r] = sqrt(sum_i a[r][i]^2 * sum_i b[r][i]^2) (b = null: s[r] = sum_i a[r][i]^2)
+__global__ __launch_bounds__(256) void sv_scale(const double* __restrict__ a, int ma, const double* __restrict__ b,
+                                                int mb, long n, double* __restrict__ s) {
+  for (long r = blockIdx.x * 256L + threadIdx.x; r < n; r += 256L * gridDim.x) {
+    double ra = 0.0;
+    for (int i = 0; i < ma; ++i) ra += a[r * ma + i] * a[r * ma + i];
+    if (b) {
+      double rb = 0.0;
+      for (int i = 0; i < mb; ++i) rb += b[r * mb + i] * b[r * mb + i];
+      s[r] = sqrt(ra * rb);
+    } else {
+      s[r] = ra;
+    }
+  }
+}
+
+__device__ inline void sv_wait_vm(int ahead) {
+  // s_waitcnt vmcnt(ahead * SV_LPW) with expcnt and lgkmcnt left free (gfx9
+  // encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+  static_assert(SV_LPW <= 15, "vmcnt fits the low field");
+  if (ahead == 0) __builtin_amdgcn_s_waitcnt(0x0F70);
+  else __builtin_amdgcn_s_waitcnt(0x0F70 | SV_LPW);
+}
+
+// The k loop of sv_gemm.  The three stage buffers are separate LDS objects, so
+// the compiler can tell that the direct loads into one never touch the
+// fragments read from another and adds no wait for them; the waits on this
+// wave's own loads are the explicit ones.
+__device__ __forceinline__ void sv_mainloop(signed char* __restrict__ b0, signed char* __restrict__ b1,
+                                            signed char* __restrict__ b2, const signed char* const (&src)[SV_LPW],
+                                            const long (&inc)[SV_LPW], const int (&dst)[SV_LPW], int nst,
+                                            int aoff, int boff, i16v (&acc)[SV_D]) {
+  // stages past the last reload the last one into a buffer nobody reads again
+  auto issue = [&](int st, signed char* buf) {
+    const long sc = min(st, nst - 1);
+#pragma unroll
+    for (int j = 0; j < SV_LPW; ++j)
+      __builtin_amdgcn_global_load_lds(src[j] + sc * inc[j], (lds_void*)(buf + dst[j]), 16, 0, 0);
+  };
+  auto compute = [&](const signed char* buf) {
+    i4 bq[SV_D];
+#pragma unroll
+    for (int q = 0; q < SV_D; ++q) bq[q] = *reinterpret_cast<const i4*>(buf + boff + q * (SV_TN / 32) * 1024);
+#pragma unroll
+    for (int p = 0; p < SV_D; ++p) {
+      const i4 ap = *reinterpret_cast<const i4*>(buf + aoff + p * (SV_TM / 32) * 1024);
+#pragma unroll
+      for (int q = 0; q < SV_D - p; ++q)
+        acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ap, bq[q], acc[p + q], 0, 0, 0);
+    }
+  };
+  // stage st lives in buffer st % 3.  Before reading stage st every wave
+  // waits for its own loads of st (the SV_LPW loads of st + 1 may still fly),
+  // then the barrier makes all waves' loads visible and frees buffer
+  // (st + 2) % 3, whose stage st - 1 everyone has read.
+  auto step = [&](int st, const signed char* cur, signed char* ahead) {
+    sv_wait_vm(1);
+    asm volatile("s_barrier" ::: "memory");
+    issue(st + 2, ahead);
+    compute(cur);
+  };
+  issue(0, b0);
+  issue(1, b1);
+  // whole triples first: a conditional step inside the loop body makes the
+  // compiler lose track of which buffer each load in flight fills, and it then
+  // waits for all of them before every fragment read
+  int st = 0;
+  for (; st + 3 <= nst; st += 3) {
+    step(st, b0, b2);
+    step(st + 1, b1, b0);
+    step(st + 2, b2, b1);
+  }
+  if (st < nst) step(st, b0, b2);
+  if (st + 1 < nst) step(st + 1, b1, b0);
+  sv_wait_vm(0);
+}
+
+template <class TX>
+__global__ __launch_bounds__(512) void sv_gemm(const signed char* __restrict__ SA, const int* __restrict__ EA,
+                                              const signed char* __restrict__ SZ, const double* __restrict__ s,
+                                              int k, int kr, int kp, long n, long nr, TX* __restrict__ x, long ldx,
+                                              int mt, long nt, long ngroups) {
+  __shared__ __attribute__((aligned(16))) signed char L0[SV_NBLK * 1024], L1[SV_NBLK * 1024],
+      L2[SV_NBLK * 1024];
+  // XCD-aware order: groups of 4 mu-tiles x 8 r-tiles (32 workgroups, one per CU
+  // of an XCD) are dealt to the XCDs in turn; workgroup i runs on XCD i % 8
+  const long b = blockIdx.x;
+  const long g = (b / 256) * 8 + (b % 8);  // group
+  const int w = (int)((b / 8) % 32);       // member of the group
+  if (g >= ngroups) return;
+  const long ngm = (mt + 3) / 4;
+  const long tm = (g % ngm) * 4 + (w % 4);
+  const long tn = (g / ngm) * 8 + (w / 4);
+  if (tm >= mt || tn >= nt) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 3, wn = wave >> 2;
+  const int l32 = lane & 31, lh = lane >> 5;
+
+  // loads: block bl = wave + 8 j, one contiguous KB of a k-blocked plane; lane
+  // L fills row L / 2, physical segment L % 2 of the LDS block from the
+  // logical segment that belongs there.  Every wave issues SV_LPW loads per
+  // stage (waves whose last block index is past the stage reload their
+  // previous block: the same bytes to the same place), so the count of loads
+  // in flight is the same at every barrier.
+  const int lrow = lane >> 1;
+  const int lseg = (lane & 1) ^ ((lrow >> 2) & 1) ^ ((lrow >> 3) & 1);
+  const long planeA = (long)kr * kp, planeZ = nr * kp;
+  const signed char* src[SV_LPW];
+  long inc[SV_LPW];
+  int dst[SV_LPW];
+#pragma unroll
+  for (int j = 0; j < SV_LPW; ++j) {
+    const int bl = wave + 8 * j < SV_NBLK ? wave + 8 * j : wave + 8 * (j - 1);
+    dst[j] = bl * 1024;
+    if (bl < SV_ABLK) {
+      const int p = bl / (SV_TM / 32), rb = bl % (SV_TM / 32);
+      src[j] = SA + p * planeA + (tm * SV_TM + rb * 32 + lrow) * 32 + lseg * 16;
+      inc[j] = (long)kr * 32;
+    } else {
+      const int q = (bl - SV_ABLK) / (SV_TN / 32), rb = (bl - SV_ABLK) % (SV_TN / 32);
+      src[j] = SZ + q * planeZ + (tn * SV_TN + rb * 32 + lrow) * 32 + lseg * 16;
+      inc[j] = nr * 32;
+    }
+  }
+  const int nst = kp / SV_KS;
+  i16v acc[SV_D];
+#pragma unroll
+  for (int d = 0; d < SV_D; ++d)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[d][v] = 0;
+  // fragment reads: row l32 of the wave's 32-row block, logical segment lh
+  const int aoff = wm * 1024 + l32 * 32 + sv_seg(l32, lh) * 16;
+  const int boff = SV_ABLK * 1024 + wn * 1024 + l32 * 32 + sv_seg(l32, lh) * 16;
+  sv_mainloop(L0, L1, L2, src, inc, dst, nst, aoff, boff, acc);
+  // epilogue: acc[d][v] is row 8 (v / 4) + 4 lh + v % 4, column l32 of the block
+  const long c = tn * SV_TN + wn * 32 + l32;
+  if (c >= n) return;
+  const double sc = s[c];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const long r = tm * SV_TM + wm * 32 + 8 * (v / 4) + 4 * lh + (v % 4);
+    if (r < k) {
+      double sum = 0.0;
+#pragma unroll
+      for (int d = SV_D - 1; d >= 0; --d) sum += ldexp((double)acc[d][v], -7 * (d + 2));
+      x[r * ldx + c] = (TX)(ldexp(sum, EA[r] + 1) * sc);
+    }
+  }
+}
+
+struct SvPlan {
+  int kp, kr, mt;
+  long nr, nt, ngroups;
+};
+
+SvPlan sv_plan(int k, long n) {
+  SvPlan p;
+  p.kp = (k + SV_KS - 1) / SV_KS * SV_KS;
+  p.mt = (k + SV_TM - 1) / SV_TM;
+  p.kr = p.mt * SV_TM;
+  p.nt = (n + SV_TN - 1) / SV_TN;
+  p.nr = p.nt * SV_TN;
+  p.ngroups = (long)((p.mt + 3) / 4) * ((p.nt + 7) / 8);
+  return p;
+}
+
+}  // namespace
+
+size_t oz_solve_work_bytes(int k, long n) {
+  const SvPlan P = sv_plan(k, n);
+  return align_up((size_t)SV_D * P.kr * P.kp, 256) + align_up((size_t)SV_D * P.nr * P.kp, 256) +
+         align_up(sizeof(int) * (size_t)P.kr, 256) + align_up(sizeof(double) * (size_t)n, 256) +
+         align_up(sizeof(double) * (size_t)P.kp, 256) + 1024;
+}
+
+int oz_solve(const double* ainv, long lda, const double* zt, int k, long n, const int* idx, const double* phi_a,
+             int ma, const double* phi_b, int mb, void* x, int x_f32, long ldx, void* work, hipStream_t s) {
+  if (k <= 0 || n <= 0) return THC_OK;
+  const SvPlan P = sv_plan(k, n);
+  char* w = reinterpret_cast<char*>(work);
+  signed char* SA = reinterpret_cast<signed char*>(w);
+  w += align_up((size_t)SV_D * P.kr * P.kp, 256);
+  signed char* SZ = reinterpret_cast<signed char*>(w);
+  w += align_up((size_t)SV_D * P.nr * P.kp, 256);
+  int* EA = reinterpret_cast<int*>(w);
+  w += align_up(sizeof(int) * (size_t)P.kr, 256);
+  double* sc = reinterpret_cast<double*>(w);
+  w += align_up(sizeof(double) * (size_t)n, 256);
+  double* inv2 = reinterpret_cast<double*>(w);
+  const int gs = (int)std::min<long>(8192, (n + 255) / 256);
+  hipLaunchKernelGGL(sv_scale, dim3(gs), dim3(256), 0, s, phi_a, ma, phi_b, mb, n, sc);
+  THC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sv_split_rows, dim3(P.kr), dim3(256), 0, s, ainv, lda, k, P.kr, P.kp, sc, n, idx, SA, EA);
+  THC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sv_point_scale, dim3((P.kp + 255) / 256), dim3(256), 0, s, sc, n, idx, k, P.kp, inv2);
+  THC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(sv_split_zt, dim3((unsigned)(P.nr / 64), (unsigned)(P.kp / 32)), dim3(256), 0, s, zt, n, k, P.nr,
+                     P.kp, sc, inv2, SZ);
+  THC_HIP(hipGetLastError());
+  const long blocks = (P.ngroups + 7) / 8 * 256;  // every group gets its 32 workgroups on one XCD
+  if (blocks > 0x7fffffffL) return THC_ERR_ARG;
+  if (x_f32)
+    hipLaunchKernelGGL(sv_gemm<float>, dim3((unsigned)blocks), dim3(512), 0, s, SA, EA, SZ, sc, k, P.kr, P.kp, n,
+                       P.nr, (float*)x, ldx, P.mt, P.nt, P.ngroups);
+  else
+    hipLaunchKernelGGL(sv_gemm<double>, dim3((unsigned)blocks), dim3(512), 0, s, SA, EA, SZ, sc, k, P.kr, P.kp, n,
+                       P.nr, (double*)x, ldx, P.mt, P.nt, P.ngroups);
   THC_HIP(hipGetLastError());
   return THC_OK;
 }

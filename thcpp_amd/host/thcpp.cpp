@@ -79,6 +79,15 @@ std::vector<double> real_part_checked(const std::vector<double>& z, const std::s
   return r;
 }
 
+// rows idx of a row-major (rows x width) host matrix
+std::vector<double> gather_rows(const std::vector<double>& a, int width, const std::vector<int>& idx) {
+  std::vector<double> out(idx.size() * (size_t)width);
+  for (size_t mu = 0; mu < idx.size(); ++mu)
+    std::copy(a.begin() + (size_t)idx[mu] * width, a.begin() + (size_t)(idx[mu] + 1) * width,
+              out.begin() + mu * width);
+  return out;
+}
+
 }  // namespace
 
 namespace ContextHandler {
@@ -290,6 +299,14 @@ IVecs::IVecs(thc::Json& input, ContextHandler::BlacsHandler& BH, std::vector<int
              bool append) {
   input_file = input.at("orbital_file").as_string();
   output_file = input.at("output_file").as_string();
+  if (input.contains("thc_precision")) {
+    const std::string p = input.at("thc_precision").as_string();
+    if (p == "fp64") precision = THC_PREC_FP64;
+    else if (p == "fp32") precision = THC_PREC_FP32;
+    else if (p == "mixed") precision = THC_PREC_MIXED;
+    else throw std::runtime_error("thc_precision must be \"fp64\", \"fp32\" or \"mixed\", not \"" + p + "\"");
+  }
+  if (input.contains("thc_refine_steps")) refine_steps = input.at("thc_refine_steps").as_int();
   std::cout << "#################################################" << std::endl;
   std::cout << "##   Setting up interpolative vector solver.   ##" << std::endl;
   std::cout << "#################################################" << std::endl;
@@ -303,6 +320,7 @@ IVecs::IVecs(thc::Json& input, ContextHandler::BlacsHandler& BH, std::vector<int
   }
   if (append) thc::h5::File(output_file, thc::h5::File::Truncate);
   nmu = (int)interp_indxs.size();
+  d_idx = to_dev(interp_indxs);
   if (half_rotated) {
     // to distinguish Luv between the half rotated and rotated case
     prefix = "HalfTransformed";
@@ -340,87 +358,94 @@ void IVecs::write_orbitals_at_points(const std::vector<double>& ao, int m, const
   fo.write_int("/Hamiltonian/THC/" + prfx + "InterpIndx", interp_indxs.data(), {(uint64_t)k});
 }
 
-// Half-rotated ZC^T = Pocc o P (src/interpolating_vectors.cpp:143-161), with the
-// full-set and occupied-set orbitals at the points written under the
-// HalfTransformedFull / HalfTransformedOcc prefixes.
+// Half-rotated set-up (src/interpolating_vectors.cpp:143-161): the full-set and
+// occupied-set orbitals at the points are written under the
+// HalfTransformedFull / HalfTransformedOcc prefixes, and both orbital sets go
+// to the device; ZC^T = Pocc o P itself is formed inside kernel().
 void IVecs::setup_CZt_half(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH) {
+  (void)BH;
   std::vector<double> aoR, aoR_half;
-  int mocc = 0;
   {
     thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
     aoR = real_part_checked(fin.read_complex("aoR"), "aoR");
     const auto d = fin.dims("aoR_half");
     if (d.size() < 2 || (long)d[0] != ngrid) throw std::runtime_error("aoR_half must be (N, M_occ)");
-    mocc = (int)d[1];
+    nocc = (int)d[1];
     aoR_half = real_part_checked(fin.read_complex("aoR_half"), "aoR_half");
   }
-  const int k = nmu, m = nbasis;
-  const long n = ngrid;
-  write_orbitals_at_points(aoR, m, interp_indxs, "HalfTransformedFull");
-  write_orbitals_at_points(aoR_half, mocc, interp_indxs, "HalfTransformedOcc");
+  write_orbitals_at_points(aoR, nbasis, interp_indxs, "HalfTransformedFull");
+  write_orbitals_at_points(aoR_half, nocc, interp_indxs, "HalfTransformedOcc");
+  ao_mu = gather_rows(aoR, nbasis, interp_indxs);
+  aoh_mu = gather_rows(aoR_half, nocc, interp_indxs);
   std::cout << " * Constructing CZt matrix" << std::endl;
-  std::cout << " * Matrix Shape: (" << k << ", " << n << ")" << std::endl;
-  double* d_ao = to_dev(aoR);
-  DevFree fa(d_ao);
-  double* d_aoh = to_dev(aoR_half);
-  DevFree fh(d_aoh);
-  d_idx = to_dev(interp_indxs);
-  d_czt = dev_alloc<double>((size_t)k * n);
-  status_ok(thc_czt_half(BH.ctx, d_ao, d_aoh, n, m, mocc, d_idx, k, d_czt), "thc_czt_half");
+  std::cout << " * Matrix Shape: (" << nmu << ", " << ngrid << ")" << std::endl;
+  d_ao = to_dev(aoR);
+  d_aoh = to_dev(aoR_half);
 }
 
 IVecs::~IVecs() {
-  if (d_czt) (void)hipFree(d_czt);
+  if (d_ao) (void)hipFree(d_ao);
+  if (d_aoh) (void)hipFree(d_aoh);
   if (d_cct) (void)hipFree(d_cct);
   if (d_idx) (void)hipFree(d_idx);
 }
 
-// ZC^T = P o conj(P) (src/interpolating_vectors.cpp:127-141); complex orbitals
-// (any non-zero imaginary part) take the complex product, which is still real.
+// ZC^T = P o conj(P) (src/interpolating_vectors.cpp:127-141): the orbitals go
+// to the device (complex ones, any non-zero imaginary part, interleaved);
+// ZC^T itself is formed inside kernel().
 void IVecs::setup_CZt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH) {
+  (void)BH;
   std::vector<double> z;
   {
     thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
     z = fin.read_complex("aoR");
   }
-  const bool cplx = has_imag(z);
-  const int k = nmu, m = nbasis;
-  const long n = ngrid;
+  cplx = has_imag(z);
   std::vector<double> aoR;
   if (!cplx) aoR = real_part_checked(z, "aoR");
-  write_orbitals_at_points(cplx ? z : aoR, m, interp_indxs, prefix, cplx);
+  write_orbitals_at_points(cplx ? z : aoR, nbasis, interp_indxs, prefix, cplx);
+  ao_mu = gather_rows(cplx ? z : aoR, cplx ? 2 * nbasis : nbasis, interp_indxs);
   std::cout << " * Constructing CZt matrix" << (cplx ? " (complex orbitals)" : "") << std::endl;
-  std::cout << " * Matrix Shape: (" << k << ", " << n << ")" << std::endl;
-  double* d_ao = to_dev(cplx ? z : aoR);
-  DevFree fa(d_ao);
-  d_idx = to_dev(interp_indxs);
-  d_czt = dev_alloc<double>((size_t)k * n);
-  if (cplx) status_ok(thc_czt_complex(BH.ctx, d_ao, n, m, d_idx, k, d_czt), "thc_czt_complex");
-  else status_ok(thc_czt(BH.ctx, d_ao, n, m, d_idx, k, d_czt), "thc_czt");
+  std::cout << " * Matrix Shape: (" << nmu << ", " << ngrid << ")" << std::endl;
+  d_ao = to_dev(cplx ? z : aoR);
 }
 
+// CC^T = ZC^T[:, idx] (src/interpolating_vectors.cpp:163-190), formed directly
+// from the orbitals at the points: the ZC^T product of the K gathered rows.
 void IVecs::setup_CCt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH) {
   (void)interp_indxs;
-  d_cct = dev_alloc<double>((size_t)nmu * nmu);
-  status_ok(thc_cct(BH.ctx, d_czt, ngrid, d_idx, nmu, d_cct), "thc_cct");
+  const int k = nmu;
+  std::vector<int> iota(k);
+  for (int mu = 0; mu < k; ++mu) iota[mu] = mu;
+  double* d_rows = to_dev(ao_mu);
+  DevFree fr(d_rows);
+  int* d_iota = to_dev(iota);
+  DevFree fi(d_iota);
+  d_cct = dev_alloc<double>((size_t)k * k);
+  if (half_rotate) {
+    double* d_rh = to_dev(aoh_mu);
+    DevFree fh(d_rh);
+    status_ok(thc_czt_half(BH.ctx, d_rows, d_rh, k, nbasis, nocc, d_iota, k, d_cct), "thc_czt_half");
+  } else if (cplx) {
+    status_ok(thc_czt_complex(BH.ctx, d_rows, k, nbasis, d_iota, k, d_cct), "thc_czt_complex");
+  } else {
+    status_ok(thc_czt(BH.ctx, d_rows, k, nbasis, d_iota, k, d_cct), "thc_czt");
+  }
 }
 
-// The reference counts singular values of CC^T (src/interpolating_vectors.cpp:192-208); here the
-// Cholesky factorisation the solve needs reports whether CC^T is numerically positive definite.
+// IVecs::check_rank (src/interpolating_vectors.cpp:192-208): singular values of
+// CC^T below K s0 1e-16 count as zero (MatrixOperations::rank).
 void IVecs::check_rank(ContextHandler::BlacsHandler& BH) {
   std::cout << " * Checking rank of CCt matrix." << std::endl;
-  double* tmp = dev_alloc<double>((size_t)nmu * nmu);
-  DevFree ft(tmp);
-  hip_ok(hipMemcpy(tmp, d_cct, sizeof(double) * nmu * nmu, hipMemcpyDeviceToDevice), "hipMemcpy");
-  int info = 0;
-  status_ok(thc_cholesky(BH.ctx, tmp, nmu, &info), "thc_cholesky");
-  if (info != 0) {
-    std::cout << " * WARNING: CCt matrix is rank deficient. Leading minor " << info
-              << " is not positive definite." << std::endl;
+  int rank = 0;
+  status_ok(thc_cct_rank(BH.ctx, d_cct, nmu, &rank, nullptr), "thc_cct_rank");
+  if (rank < nmu) {
+    std::cout << " * WARNING: CCt matrix is rank deficient. Rank = " << rank << std::endl;
+    std::cout << std::endl;
   } else {
-    std::cout << " * CCt matrix has full rank : " << nmu << std::endl;
+    std::cout << " * CCt matrix has full rank : " << rank << std::endl;
+    std::cout << std::endl;
   }
-  std::cout << std::endl;
 }
 
 void IVecs::kernel(ContextHandler::BlacsHandler& BH) {
@@ -429,30 +454,6 @@ void IVecs::kernel(ContextHandler::BlacsHandler& BH) {
   std::cout << "#################################################" << std::endl;
   std::cout << "##        Finding interpolating vectors.       ##" << std::endl;
   std::cout << "#################################################" << std::endl;
-  std::cout << std::endl;
-  std::cout << " * Performing least squares solve." << std::endl;
-  auto t0 = std::chrono::steady_clock::now();
-  double* d_x = dev_alloc<double>((size_t)k * n);
-  DevFree fx(d_x);
-  int info = 0;
-  const int s = thc_least_squares(BH.ctx, d_cct, d_czt, n, k, 0, d_x, &info);
-  std::cout << " * Time for least squares solve : "
-            << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds"
-            << std::endl;
-  std::cout << std::endl;
-  if (s != THC_OK) {
-    std::cout << " * Parallel least squares failed." << std::endl;
-    std::cout << " * Error code: " << info << std::endl;
-    status_ok(s, "thc_least_squares");
-  }
-  // release ZC^T before the spectrum is formed
-  (void)hipFree(d_czt);
-  d_czt = nullptr;
-  std::cout << "#################################################" << std::endl;
-  std::cout << "##        Constructing Muv matrix.             ##" << std::endl;
-  std::cout << "#################################################" << std::endl;
-  std::cout << std::endl;
-  std::cout << " * Performing FFT on interpolating vectors." << std::endl;
   std::cout << std::endl;
   if ((long)fft_grid[0] * fft_grid[1] * fft_grid[2] != n)
     std::cout << " * WARNING: FFT grid not consitent with number of real space grid points." << std::endl;
@@ -463,27 +464,49 @@ void IVecs::kernel(ContextHandler::BlacsHandler& BH) {
   }
   double* d_cg = to_dev(coulG);
   DevFree fc(d_cg);
-  const long g2 = 2L * fft_grid[0] * fft_grid[1] * (fft_grid[2] / 2 + 1);
-  double* d_b = dev_alloc<double>((size_t)k * g2);
-  DevFree fb(d_b);
-  t0 = std::chrono::steady_clock::now();
-  status_ok(thc_fft_vectors(BH.ctx, d_x, k, fft_grid.data(), d_cg, d_b), "thc_fft_vectors");
-  std::cout << " * Time to FFT interpolating vectors : "
-            << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds"
-            << std::endl;
-  std::cout << std::endl;
-  std::cout << " * Constructing Muv." << std::endl;
   double* d_v = dev_alloc<double>((size_t)k * k);
   DevFree fv(d_v);
-  t0 = std::chrono::steady_clock::now();
-  status_ok(thc_construct_muv(BH.ctx, d_b, k, g2, d_v), "thc_construct_muv");
-  std::cout << "  * Time to construct Muv: "
-            << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds."
-            << std::endl;
+  double* d_l = half_rotate ? nullptr : dev_alloc<double>((size_t)k * k);
+  DevFree fl(d_l);
+  thc_isdf_opts opts = {precision, refine_steps, half_rotate ? 0 : 1, 0};
+  thc_isdf_stats st = {};
+  const auto t0 = std::chrono::steady_clock::now();
+  int s;
+  if (half_rotate)
+    s = thc_isdf_half(BH.ctx, d_ao, d_aoh, n, nbasis, nocc, d_idx, k, fft_grid.data(), d_cg, &opts, d_v, nullptr,
+                      &st);
+  else if (cplx)
+    s = thc_isdf_complex(BH.ctx, d_ao, n, nbasis, d_idx, k, fft_grid.data(), d_cg, &opts, d_v, d_l, nullptr, &st);
+  else
+    s = thc_isdf(BH.ctx, d_ao, n, nbasis, d_idx, k, fft_grid.data(), d_cg, &opts, d_v, d_l, nullptr, &st);
+  const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (s == THC_ERR_NOT_POSDEF) {
+    std::cout << " * Parallel least squares failed." << std::endl;
+    std::cout << " * Error code: " << st.cct_chol_info << std::endl;
+  }
+  status_ok(s, half_rotate ? "thc_isdf_half" : "thc_isdf");
+  // the reference's stage lines, from the library's device times
+  std::cout << " * Performing least squares solve." << std::endl;
+  std::cout << " * Time for least squares solve : " << (st.stage_ms[0] + st.stage_ms[1] + st.stage_ms[2]) * 1e-3
+            << " seconds" << std::endl;
+  std::cout << std::endl;
+  std::cout << "#################################################" << std::endl;
+  std::cout << "##        Constructing Muv matrix.             ##" << std::endl;
+  std::cout << "#################################################" << std::endl;
+  std::cout << std::endl;
+  std::cout << " * Performing FFT on interpolating vectors." << std::endl;
+  std::cout << std::endl;
+  std::cout << " * Time to FFT interpolating vectors : " << st.stage_ms[3] * 1e-3 << " seconds" << std::endl;
+  std::cout << std::endl;
+  std::cout << " * Constructing Muv." << std::endl;
+  std::cout << "  * Time to construct Muv: " << st.stage_ms[4] * 1e-3 << " seconds." << std::endl;
+  std::cout << " * Factorisation (" << (precision == THC_PREC_MIXED ? "mixed" : precision == THC_PREC_FP32 ? "fp32"
+                                                                                                       : "fp64")
+            << ") in " << wall << " seconds." << std::endl;
   // dump_thc_data (src/interpolating_vectors.cpp:283-328)
   if (half_rotate) {
-    // Muv = IVMG^T IVG; for real interpolating vectors IVMG = conj(IVG), so it is
-    // the matrix just formed, written full with no Cholesky
+    // Muv = IVMG^T IVG; for real interpolating vectors IVMG = conj(IVG): the
+    // matrix returned, written full with no Cholesky
     std::vector<double> mh = to_host(d_v, (size_t)k * k);
     std::cout << " * Dumping half transformed Muv data to: " << output_file << "." << std::endl;
     std::cout << std::endl;
@@ -495,18 +518,15 @@ void IVecs::kernel(ContextHandler::BlacsHandler& BH) {
     return;
   }
   std::cout << " * Performing Cholesky decomposition on Muv." << std::endl;
-  t0 = std::chrono::steady_clock::now();
-  int cinfo = 0;
-  status_ok(thc_cholesky(BH.ctx, d_v, k, &cinfo), "thc_cholesky");
-  std::vector<double> l = to_host(d_v, (size_t)k * k);
-  std::cout << " * Time to perform cholesky decomposition on Muv: "
-            << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " seconds."
+  std::cout << " * Time to perform cholesky decomposition on Muv: " << st.stage_ms[5] * 1e-3 << " seconds."
             << std::endl;
+  std::vector<double> l = to_host(d_l, (size_t)k * k);
   std::cout << " * Dumping Luv to: " << output_file << "." << std::endl;
-  if (cinfo != 0) {
+  if (s == THC_WARN_LUV_NOT_POSDEF) {
     std::cout << "  * WARNING: Cholesky decomposition failed!" << std::endl;
-    std::cout << "  * SCALAPACK Error code: " << cinfo << std::endl;
-    std::cout << "   * The leading minor of order " << cinfo << " is not positive definite." << std::endl;
+    std::cout << "  * SCALAPACK Error code: " << st.luv_chol_info << std::endl;
+    std::cout << "   * The leading minor of order " << st.luv_chol_info << " is not positive definite."
+              << std::endl;
   }
   std::cout << std::endl;
   std::vector<double> luv(2 * (size_t)k * k, 0.0);

@@ -73,6 +73,11 @@ class IPoints {
 }  // namespace InterpolatingPoints
 
 namespace InterpolatingVectors {
+// The constructor reads the orbitals, writes Orbitals / InterpIndx, forms CC^T
+// and checks its rank as the reference's does; kernel() runs the whole
+// factorisation (ZC^T, solve, FFT, Muv, Cholesky) as one thc_isdf call.
+// Input keys beyond the reference's: "thc_precision" ("fp64" default, "fp32",
+// "mixed") and "thc_refine_steps" (FP64 refinement steps, default 0).
 class IVecs {
  public:
   IVecs(thc::Json& input, ContextHandler::BlacsHandler& BH, std::vector<int>& interp_indxs, bool half_rotated,
@@ -92,11 +97,17 @@ class IVecs {
   int nbasis = 0;
   long ngrid = 0;
   int nmu = 0;
+  int nocc = 0;
+  bool cplx = false;
   std::vector<int> fft_grid;
   std::string prefix;
   bool half_rotate = false;
+  int precision = THC_PREC_FP64;
+  int refine_steps = 0;
+  std::vector<double> ao_mu, aoh_mu;  // orbital rows at the points (host), for CC^T
   // device buffers
-  double* d_czt = nullptr;
+  double* d_ao = nullptr;   // aoR (n x m real, or n x 2m interleaved complex)
+  double* d_aoh = nullptr;  // aoR_half (n x nocc), half-rotated only
   double* d_cct = nullptr;
   int* d_idx = nullptr;
 };
