@@ -190,6 +190,16 @@ int thc_cct_rank(thc_ctx *ctx, const double *cct, int k, int *rank, double *smax
  * src/matrix_operations.h:259-267). */
 int thc_cholesky(thc_ctx *ctx, double *a, int k, int *info);
 
+/* thc_cholesky without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The Cholesky's scratch
+ * (two 64 x 64 FP64 panels and the info word, 65 792 bytes whatever k is) is
+ * a static device array each GPU holds for the life of the process, shared
+ * by every call on that GPU: calls that overlap on one GPU race.  The status
+ * word goes through the context's pinned buffer, made by thc_ctx_create.
+ * Same result as thc_cholesky, bit for bit. */
+int thc_cholesky_noalloc(thc_ctx *ctx, double *a, int k, int *info);
+
 /* X = CC^T^-1 ZC^T, k x n: FP64 Cholesky of CC^T, explicit FP64 inverse from
  * the factor, one MFMA GEMM, plus 'refine_steps' FP64 residual corrections.
  * Replaces MatrixOperations::least_squares / pzgels (src/matrix_operations.h:184-202,

@@ -428,6 +428,14 @@ int thc_ctx_create(int device, void* stream, thc_ctx** out) {
       return THC_ERR_HIP;
     }
   }
+  // set-up that thc_cholesky_noalloc relies on: the device address of its
+  // static scratch on this device, and a small pinned buffer for its status
+  int st = noalloc_scratch_init(device);
+  if (st == THC_OK) st = ensure_pinned(c, 4096);
+  if (st != THC_OK) {
+    thc_ctx_destroy(c);
+    return st;
+  }
   hipError_t r = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
   if (r == hipSuccess) r = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
   if (r == hipSuccess) r = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);

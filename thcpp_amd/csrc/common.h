@@ -130,6 +130,9 @@ int syrk_splitk(const void* a, long m, long k, long lda, int prec, double* v, lo
 
 // ---- dense FP64 factorisations (linalg.hip) --------------------------------
 size_t chol_work_bytes(int k);
+// Device address of the no-allocation Cholesky's static scratch, taken once
+// per device (that device current); thc_ctx_create calls it.
+int noalloc_scratch_init(int device);
 // In-place lower Cholesky (row-major, lda); info written to device int.
 int cholesky_f64(double* a, int k, long lda, int* d_info, void* work, hipStream_t s);
 size_t spd_inverse_work_bytes(int k);

@@ -12,6 +12,7 @@
 // two batched GEMMs per level; finally A^-1 = Linv^T Linv.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <utility>
 
 #include "common.h"
@@ -668,3 +669,75 @@ int symmetrize_from_lower(double* a, int k, long lda, hipStream_t s) {
 
 }  // namespace thc
 
+// ---------------------------------------------------------------- no-allocation Cholesky
+//
+// thc_cholesky_noalloc's scratch: the two panel-inverse buffers cholesky_f64
+// works in, then its info word.  A file-scope device array: each GPU the
+// library runs on holds one copy, for the life of the process.  In host code
+// the name 'scratch' is a host-side stand-in, never a device address; the
+// device address of each GPU's copy comes from hipGetSymbolAddress, taken once
+// per device (from thc_ctx_create, with that device current) and kept below.
+namespace thc {
+namespace {
+constexpr size_t kCholScratchWork = 2 * 64 * 64 * sizeof(double);  // chol_work_bytes(): two 64 x 64 panels
+constexpr size_t kCholScratchBytes = kCholScratchWork + 256;        // + the info word (256-byte slot)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+}  // namespace thc
+
+static __device__ __attribute__((aligned(256))) unsigned char scratch[thc::kCholScratchBytes];
+
+namespace thc {
+
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return THC_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return THC_OK;
+  void* p = nullptr;
+  THC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = p;
+  return THC_OK;
+}
+
+namespace {
+__global__ void zero_info(int* p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+}  // namespace
+
+}  // namespace thc
+
+extern "C" int thc_cholesky_noalloc(thc_ctx* ctx, double* a, int k, int* info) {
+  using namespace thc;
+  if (!ctx || !a || !info || k < 1) return THC_ERR_ARG;
+  if (chol_work_bytes(k) > kCholScratchWork) {
+    set_last_error("thc_cholesky_noalloc: the Cholesky panels need more than the static scratch holds", __FILE__,
+                   __LINE__);
+    return THC_ERR_UNSUPPORTED;
+  }
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return THC_ERR_UNSUPPORTED;
+  if (!ctx->pinned || ctx->pinned_bytes < sizeof(int)) return THC_ERR_ARG;  // thc_ctx_create provides it
+  THC_HIP(hipSetDevice(ctx->device));
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[ctx->device];
+  }
+  if (!base) THC_TRY(noalloc_scratch_init(ctx->device));  // a device the library was not set up on
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[ctx->device];
+  }
+  double* work = reinterpret_cast<double*>(base);
+  int* d_info = reinterpret_cast<int*>(reinterpret_cast<char*>(base) + kCholScratchWork);
+  hipLaunchKernelGGL(zero_info, dim3(1), dim3(64), 0, ctx->stream, d_info);
+  THC_HIP(hipGetLastError());
+  THC_TRY(cholesky_f64(a, k, k, d_info, work, ctx->stream));
+  int* h = reinterpret_cast<int*>(ctx->pinned);
+  THC_HIP(hipMemcpyAsync(h, d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  *info = *h;
+  return THC_OK;
+}
