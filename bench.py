@@ -6,10 +6,14 @@ Cholesky + inverse, the least-squares solve, the Coulomb-weighted FFT, V and
 Luv = chol(V).  The workload is the 2x2x2 diamond supercell (64 atoms,
 64^3 grid, 128 orbitals, N_mu = 2048, 30 k-means iterations) with synthetic
 orbitals; the headline value is the mixed-precision path, the plain FP64 path
-is timed beside it on the same inputs, and the relative V error between the
-two is reported.
+is timed beside it on the same inputs.
 
-    python bench.py [--steps K] [--warmup W] [--config NAME] [--no-cpu]
+The CPU leg runs the oracle (the FP64 CPU restatement of the reference) over
+one complete factorisation of the same system and is both the CPU baseline
+and the reference the GPU results are checked against: k-means points
+(identical), and the relative error of xi and V of every GPU path.
+
+    python bench.py [--steps K] [--warmup W] [--config NAME] [--no-cpu] [--no-batch]
     python bench.py --copy-cal     # 1 GiB device copy x3, for HBM counter calibration
 """
 from __future__ import annotations
@@ -25,13 +29,17 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"hbm_gbs": 8000.0, "fp64_tflops": 78.6, "fp32_tflops": 157.3, "bf16_tflops": 2500.0}
-# HBM bytes per launch of the mixed-path solve GEMM on diamond64_n64_k2048, from rocprofv3 FETCH_SIZE /
-# WRITE_SIZE counters scaled by the same counters on a 1 GiB device copy (fetch x 2.0, write x 1.0); the
-# summary with its commands is profiles/isdf_diamond64_n64_k2048_hbm_traffic.txt.  Other workloads
-# report null.
-SOLVE_TRAFFIC_BYTES = {"diamond64_n64_k2048": 38.42e9}
+PEAK = {"hbm_gbs": 8000.0, "fp64_tflops": 78.6, "fp32_tflops": 157.3, "bf16_tflops": 2500.0, "int8_tops": 5000.0}
+SV_DIGITS = 7                        # digits of the int8 solve; products with p + q < 7
+SV_PRODUCTS = SV_DIGITS * (SV_DIGITS + 1) // 2
+# HBM bytes per launch of the mixed path's solve GEMM (sv_gemm) on diamond64_n64_k2048, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE counters scaled by the same counters on a 1 GiB device copy (fetch x 2.0,
+# write x 1.0); the summary with its commands is profiles/sv_gemm_diamond64_n64_k2048_hbm_traffic.txt.
+# Other workloads report null.
+SOLVE_TRAFFIC_BYTES = {}
 METRIC = "ISDF factorizations/s + % roofline; mixed vs FP64 GPU at <=1e-6 rel V error"
+STAGES = ["zct", "cct_chol_inv", "solve", "fft", "V", "luv", "isdf_total", "refine", "solve_gemm_kernel",
+          "V_kernel"]
 
 
 def parse():
@@ -40,9 +48,9 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--config", default="diamond64_n64_k2048")
-    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg (and the errors against it)")
     p.add_argument("--no-batch", action="store_true", help="skip the 32-cell batched leg")
-    p.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU sample work")
+    p.add_argument("--no-variants", action="store_true", help="skip the FP32 and refinement legs")
     p.add_argument("--copy-cal", action="store_true",
                    help="only run a 1 GiB float64 device copy three times (the known-size kernel that "
                         "FETCH_SIZE / WRITE_SIZE counters are scaled against) and exit")
@@ -61,54 +69,58 @@ def copy_calibration():
     return 0
 
 
-def cpu_baseline(s, budget):
-    """The oracle (the CPU FP64 restatement of the reference) on a bounded
-    sample of the same workload, scaled to one factorisation:
-    k-means: one sequential classify + update pass over the full grid, times T+1
-    (plus one map_to_grid pass, same cost); ISDF: ZC^T, the QR solve, the FFT and
-    Muv on a column slice of the grid / a subset of mu, scaled linearly."""
-    import scipy.linalg
+def cpu_oracle(s):
+    """One complete factorisation by the oracle on the host's cores: k-means
+    (classify on OpenMP threads, the rest sequential, as the reference at one
+    rank), ZC^T, CC^T, the QR least-squares solve, the forward FFTs, Muv and
+    its Cholesky, with the multithreaded BLAS/FFT numpy and scipy bring."""
     from oracle import isdf_oracle as IO
     from oracle import kmeans_oracle as KO
     cores = len(os.sched_getaffinity(0))
-    idx0 = KO.initial_indices(s.N, s.K, 0)
-    cen = s.grid[idx0]
-    t = time.perf_counter()
-    gmap = KO.classify(s.grid, cen)
-    KO.update_centroids(s.rho, s.grid, gmap, s.K)
-    t_iter = time.perf_counter() - t
-    t_kmeans = t_iter * (s.max_it + 2)
-    # ISDF pieces on a fraction f of the grid columns / of the mu rows
-    f = 1.0 / 16
-    idx = np.sort(idx0)
-    ncols = int(s.N * f)
-    t = time.perf_counter()
-    z_full_idx = IO.czt(s.aoR, idx)  # needed whole for CC^T (Z[:, idx])
-    t_z = time.perf_counter() - t
-    c = IO.cct(z_full_idx, idx)
-    t = time.perf_counter()
-    q, r = scipy.linalg.qr(c, mode="economic")
-    t_qr = time.perf_counter() - t
-    t = time.perf_counter()
-    x_part = scipy.linalg.solve_triangular(r, q.T @ z_full_idx[:, :ncols], lower=False)
-    t_apply = (time.perf_counter() - t) / f
-    del x_part
-    kmu = max(1, int(s.K * f))
-    x = z_full_idx[:kmu]
-    t = time.perf_counter()
-    ivg = IO.fft_vectors(x, s.n)
-    t_fft = (time.perf_counter() - t) * s.K / kmu
-    a = ivg[:ncols]
-    t = time.perf_counter()
-    _ = a.conj().T @ a
-    t_muv = (time.perf_counter() - t) / f * (s.K / kmu) ** 2
-    total = t_kmeans + t_z + t_qr + t_apply + t_fft + t_muv
-    return {"value": 1.0 / total, "unit": "factorizations/s", "cores": cores, "kind": "port",
-            "sample": ("oracle FP64 restatement on the same system: 1 k-means pass x (T+2), ZC^T in full, "
-                       "QR of CC^T, QR solve on 1/16 of the grid columns, FFT of %d of %d vectors, Muv on "
-                       "1/16 of G for those vectors, each scaled to a full factorisation; stage seconds "
-                       "kmeans %.1f zct %.1f qr %.1f solve %.1f fft %.1f muv %.1f"
-                       % (kmu, s.K, t_kmeans, t_z, t_qr, t_apply, t_fft, t_muv))}
+    t = {}
+    t0 = time.perf_counter()
+    km = KO.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, 0)
+    t["kmeans"] = time.perf_counter() - t0
+    idx = km["interp_idx"]
+    t0 = time.perf_counter()
+    z = IO.czt(s.aoR, idx)
+    c = IO.cct(z, idx)
+    t["zct_cct"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    x = IO.least_squares(c, z)
+    t["solve"] = time.perf_counter() - t0
+    del z
+    t0 = time.perf_counter()
+    ivg = IO.fft_vectors(x, s.n, workers=cores)
+    t["fft"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    v = IO.construct_muv(ivg, s.coulG, real_part=True)
+    t["muv"] = time.perf_counter() - t0
+    del ivg
+    t0 = time.perf_counter()
+    IO.cholesky_luv(v)
+    t["luv"] = time.perf_counter() - t0
+    total = sum(t.values())
+    base = {"value": round(1.0 / total, 5), "unit": "factorizations/s", "cores": cores, "kind": "port",
+            "sample": "one complete factorisation of the same system, no extrapolation; stage seconds " +
+                      " ".join("%s %.1f" % kv for kv in t.items())}
+    return base, {"idx": idx, "X": x, "V": v}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def rel_rows(dev, host):
+    """||dev - host|| / ||host|| over a K x N array kept on the device, row block by row block."""
+    import torch
+    num = den = 0.0
+    for r0 in range(0, host.shape[0], 256):
+        h = torch.from_numpy(np.ascontiguousarray(host[r0:r0 + 256]))
+        d = dev[r0:r0 + 256].double().cpu()
+        num += float(torch.sum((d - h) ** 2))
+        den += float(torch.sum(h ** 2))
+    return (num / den) ** 0.5
 
 
 def batched_leg(ctx, steps, warmup):
@@ -160,35 +172,37 @@ def main():
     ctx = Context(0)
     g, r, ao, cg = ctx.f64(s.grid), ctx.f64(s.rho), ctx.f64(s.aoR), ctx.f64(s.coulG)
 
-    def run(prec):
-        return ctx.factorize(g, r, ao, s.thc_cfac, s.max_it, s.n, cg, precision=prec)
+    def run(prec, refine=0):
+        return ctx.factorize(g, r, ao, s.thc_cfac, s.max_it, s.n, cg, precision=prec, refine=refine)
 
-    results = {}
-    for name, prec in (("fp64", L.PREC_FP64), ("mixed", L.PREC_MIXED)):
-        for _ in range(a.warmup):
-            run(prec)
+    def timed(prec, refine=0, steps=a.steps, warmup=a.warmup):
+        for _ in range(warmup):
+            run(prec, refine)
         torch.cuda.synchronize()
         stages = []
         t0 = time.perf_counter()
-        for _ in range(a.steps):
-            out = run(prec)
+        for _ in range(steps):
+            out = run(prec, refine)
             stages.append(out["stage_ms"] + [out["kmeans_ms"]])
         torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / a.steps
-        if out["status"] not in (L.THC_OK,):
-            print("factorisation status %d" % out["status"], file=sys.stderr)
-            return 1
-        results[name] = {"ms": dt * 1e3, "stages": np.mean(np.array(stages), axis=0), "V": out["V"].double().cpu()}
-        print("bench: %s path %.3f ms per factorisation" % (name, dt * 1e3), file=sys.stderr, flush=True)
-    v64, vmx = results["fp64"]["V"], results["mixed"]["V"]
-    relv = float(torch.linalg.norm(vmx - v64) / torch.linalg.norm(v64))
-    mx = results["mixed"]
+        dt = (time.perf_counter() - t0) / steps
+        if out["status"] != L.THC_OK:
+            raise RuntimeError("factorisation status %d" % out["status"])
+        print("bench: precision %d refine %d: %.3f ms per factorisation" % (prec, refine, dt * 1e3), file=sys.stderr,
+              flush=True)
+        return {"ms": dt * 1e3, "stages": np.mean(np.array(stages), axis=0), "V": out["V"].double().cpu().numpy(),
+                "idx": out["interp_idx"].cpu().numpy()}
+
+    res = {"fp64": timed(L.PREC_FP64), "mixed": timed(L.PREC_MIXED)}
+    mx = res["mixed"]
     ms = mx["ms"]
-    st = mx["stages"]  # 0 zct,1 cct+chol+inv,2 solve,3 fft,4 V,5 luv,6 isdf total,7 refine,8 kmeans
+    st = mx["stages"]  # STAGES order, then kmeans
     K, N = s.K, s.N
-    solve_ms = float(st[2])
-    solve_flops = 2.0 * K * K * N
-    achieved = solve_flops / (solve_ms * 1e-3) / 1e12
+    gemm_ms = float(st[8])
+    sv_ops = SV_PRODUCTS * 2.0 * K * K * N
+    achieved = sv_ops / (gemm_ms * 1e-3) / 1e12
+    kp = -(-K // 32) * 32
+    kr, nr = -(-K // 128) * 128, -(-N // 64) * 64
     line = {
         "metric": METRIC,
         "value": round(1000.0 / ms, 3),
@@ -199,26 +213,49 @@ def main():
         "dtype": "f64+f32+i8",
         "data": "synthetic",
         "config": {"workload": a.config, "atoms": int(s.atoms.shape[0]), "grid": list(s.n), "N": N,
-                   "M": s.M, "N_mu": K, "kmeans_iters": s.max_it, "precision": "mixed: FP64 k-means, ZC^T, "
-                   "Cholesky, solve; FP32 FFT; V from exact int8 MFMA products of 5 digits, FP64 sum",
+                   "M": s.M, "N_mu": K, "kmeans_iters": s.max_it,
+                   "precision": "mixed: FP64 k-means, ZC^T and Cholesky/inverse of CC^T; solve from exact int8 "
+                                "MFMA products of 7-digit splits (28 products); FP32 FFT; V from exact int8 MFMA "
+                                "products of 5 digits; FP64 sums",
                    "parallelism": "single GPU"},
-        "fp64_path": {"value": round(1000.0 / results["fp64"]["ms"], 3), "ms_per_step": round(results["fp64"]["ms"], 3)},
-        "mixed_over_fp64_speedup": round(results["fp64"]["ms"] / ms, 3),
-        "rel_V_error_mixed_vs_fp64": relv,
-        "stage_ms": {k: round(float(v), 3) for k, v in zip(
-            ["zct", "cct_chol_inv", "solve", "fft", "V", "luv", "isdf_total", "refine", "kmeans"], st)},
-        "roofline": {"kernel": "solve GEMM X = CC^T^-1 ZC^T (gemm_dtl_kernel, FP64 MFMA)", "bound": "mfma",
-                     "achieved": round(achieved, 2), "peak": PEAK["fp64_tflops"], "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK["fp64_tflops"], 4),
+        "fp64_path": {"value": round(1000.0 / res["fp64"]["ms"], 3), "ms_per_step": round(res["fp64"]["ms"], 3),
+                      "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES + ["kmeans"], res["fp64"]["stages"])}},
+        "mixed_over_fp64_speedup": round(res["fp64"]["ms"] / ms, 3),
+        "rel_V_error_mixed_vs_fp64": rel(mx["V"], res["fp64"]["V"]),
+        "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES + ["kmeans"], st)},
+        "roofline": {"kernel": "solve GEMM X = CC^T^-1 ZC^T as exact int8 digit products (sv_gemm, "
+                               "v_mfma_i32_32x32x32_i8); int8 multiply-adds counted as 2 ops",
+                     "bound": "mfma",
+                     "achieved": round(achieved, 1), "peak": PEAK["int8_tops"], "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK["int8_tops"], 4),
                      "traffic": SOLVE_TRAFFIC_BYTES.get(a.config),
                      "traffic_unit": "bytes per launch",
-                     # Zt and (CC^T)^-1 read in FP64, X written in FP32 (nothing else reads X)
-                     "algorithmic_bytes": 8.0 * (K * N + K * K) + 4.0 * K * N},
+                     # digit planes read once (7 planes of the padded K x K and N x K operands), X written in FP32
+                     "algorithmic_bytes": float(SV_DIGITS * (kr * kp + nr * kp) + 4.0 * K * N),
+                     "algorithmic_ops": sv_ops, "kernel_ms": round(gemm_ms, 3)},
     }
+    if not a.no_variants:
+        f32 = timed(L.PREC_FP32)
+        ref1 = timed(L.PREC_MIXED, refine=1, steps=2, warmup=1)
+        line["variants"] = {
+            "fp32": {"ms_per_step": round(f32["ms"], 3), "rel_V_vs_fp64_path": rel(f32["V"], res["fp64"]["V"])},
+            "mixed_refine1": {"ms_per_step": round(ref1["ms"], 3),
+                              "rel_V_vs_fp64_path": rel(ref1["V"], res["fp64"]["V"])}}
     if not a.no_batch:
         line["batched"] = batched_leg(ctx, max(1, min(a.steps, 3)), 1)
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(s, a.cpu_budget)
+        base, ref = cpu_oracle(s)
+        line["cpu_baseline"] = base
+        acc = {"kmeans_points_identical": bool(all(np.array_equal(res[p]["idx"], ref["idx"]) for p in res))}
+        for name, prec in (("fp64", L.PREC_FP64), ("mixed", L.PREC_MIXED)):
+            out = ctx.isdf(ao, ref["idx"], s.n, cg, precision=prec, want_xi=True)
+            acc["rel_xi_vs_oracle_" + name] = rel_rows(out["xi"], ref["X"])
+            acc["rel_V_vs_oracle_" + name] = rel(out["V"].double().cpu().numpy(), ref["V"])
+            del out
+        if "variants" in line:
+            out = ctx.isdf(ao, ref["idx"], s.n, cg, precision=L.PREC_FP32)
+            acc["rel_V_vs_oracle_fp32"] = rel(out["V"].double().cpu().numpy(), ref["V"])
+        line["accuracy_vs_oracle"] = acc
     print(json.dumps(line))
     return 0
 

@@ -100,25 +100,37 @@ def xi_rows(aoR: np.ndarray, idx: np.ndarray, rows: np.ndarray, block: int = 327
     return out
 
 
-def fft_vectors(x: np.ndarray, fft_grid) -> np.ndarray:
+def fft_vectors(x: np.ndarray, fft_grid, workers=None) -> np.ndarray:
     """IVG[G, mu] = DFT_forward(conj(X[mu, :]))  (interpolating_vectors.cpp:342-383).
 
     X (K, N) is the least-squares solution; the reference Hermitian-transposes it
     (pzgeadd 'C') and runs an unnormalised FFTW_FORWARD 3-D transform over the
     row-major fft_grid on each column.  Returns (N, K) complex, k-space C order.
+    workers: threads for the transforms (scipy.fft, the same pocketfft
+    transform numpy runs on one thread).
     """
     n0, n1, n2 = (int(v) for v in fft_grid)
     k, n = x.shape
     assert n0 * n1 * n2 == n
     cols = np.conj(x).reshape(k, n0, n1, n2)
+    if workers:
+        import scipy.fft
+        return scipy.fft.fftn(cols, axes=(1, 2, 3), workers=workers).reshape(k, n).T
     return np.fft.fftn(cols, axes=(1, 2, 3)).reshape(k, n).T
 
 
-def construct_muv(ivg: np.ndarray, coulg: np.ndarray) -> np.ndarray:
+def construct_muv(ivg: np.ndarray, coulg: np.ndarray, real_part: bool = False) -> np.ndarray:
     """Muv = A^H A with A[G, mu] = IVG[G, mu] sqrt(coulG[G])
     (interpolating_vectors.cpp:258-278, pzherk 'L','C').  Returns the full
-    Hermitian matrix (the reference keeps only the lower triangle)."""
+    Hermitian matrix (the reference keeps only the lower triangle); with
+    real_part only Re Muv = Re(A)^T Re(A) + Im(A)^T Im(A), all that real
+    orbitals keep (their Muv is real), in two real products."""
     a = ivg * np.sqrt(np.asarray(coulg, dtype=np.float64).reshape(-1))[:, None]
+    if real_part:
+        re = np.ascontiguousarray(a.real)
+        im = np.ascontiguousarray(a.imag)
+        del a
+        return re.T @ re + im.T @ im
     return a.conj().T @ a
 
 

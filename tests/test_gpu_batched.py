@@ -30,7 +30,7 @@ def as_dict(s):
 def test_batched_equals_single(ctx, batch, prec, streams):
     print("batched vs single: precision %d, %d streams, %d systems" % (prec, streams, len(batch)), flush=True)
     st, outs = ctx.factorize_batched([as_dict(s) for s in batch], precision=prec, max_streams=streams)
-    assert st == L.THC_OK
+    assert st == L.THC_OK, (st, [o["status"] for o in outs], L.lib().thc_last_error())
     for s, o in zip(batch, outs):
         assert o["status"] == L.THC_OK
         assert o["iters"] == s.max_it

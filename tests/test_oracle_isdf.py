@@ -229,3 +229,16 @@ def test_xi_rows_cfg0_within_conditioning_floor():
     assert rel(xr, ref["X"][rows]) < 1e-12
     v = IO.construct_muv(IO.fft_vectors(xr, s.n), s.coulG).real
     assert rel(v, ref["V"][np.ix_(rows, rows)]) < 1e-12
+
+
+def test_fft_and_muv_fast_options_agree():
+    """The threaded transform and the real-part Muv (the CPU baseline's options)
+    give the same numbers as the defaults."""
+    n = (8, 6, 10)
+    x = np.random.default_rng(4).standard_normal((5, int(np.prod(n))))
+    cg = synthetic.coulomb_kernel(5.0, n)
+    a = IO.fft_vectors(x, n)
+    b = IO.fft_vectors(x, n, workers=2)
+    assert rel(b, a) < 1e-15
+    m = IO.construct_muv(a, cg)
+    assert rel(IO.construct_muv(a, cg, real_part=True), m.real) < 1e-14

@@ -188,7 +188,10 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   if (fft[0] < 1 || fft[1] < 1 || fft[2] < 1) return THC_ERR_ARG;
   // The reference only warns when prod(fft_grid) != N (interpolating_vectors.cpp:368-370) and then
   // transforms a prefix of each column (or runs past it); here that case is refused.
-  if (nfft != n) return THC_ERR_ARG;
+  if (nfft != n) {
+    set_last_error("prod(fft_grid) differs from the number of grid points", __FILE__, __LINE__);
+    return THC_ERR_ARG;
+  }
   THC_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   if (want_rank && k > THC_MAX_CLUSTERS) return THC_ERR_UNSUPPORTED;
@@ -290,7 +293,8 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   const int x_f32 = prec != THC_PREC_FP64 && xi_out == nullptr && refine == 0;
   if (prec == THC_PREC_MIXED) {
     // exact int8 digit products of the scaled operands (ozaki.hip)
-    THC_TRY(oz_solve(B.ci, k, B.z, k, n, idx, aoR, m, aoR_half, aoR_half ? m_occ : 0, B.x, x_f32, n, B.solve_w, s));
+    THC_TRY(oz_solve(B.ci, k, B.z, k, n, idx, aoR, m, aoR_half, aoR_half ? m_occ : 0, B.x, x_f32, n, B.solve_w, s,
+                     ev[8], ev[9]));
   } else {
     GemmArgs g = {};
     g.m = k; g.n = n; g.k = k;
@@ -299,7 +303,9 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
     g.c = B.x; g.ldc = n;
     g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
     g.c_f32 = x_f32;
+    THC_HIP(hipEventRecord(ev[8], s));
     THC_TRY(gemm_f64(g, s));
+    THC_HIP(hipEventRecord(ev[9], s));
   }
   THC_HIP(hipEventRecord(ev[3], s));
   // 3b. FP64 iterative refinement in the transposed layout:
@@ -327,9 +333,9 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   THC_HIP(hipEventRecord(ev[5], s));
   // 5. V = B B^T, FP64 sum of the split-K partials (exact int8 digit products on the mixed path)
   if (prec == THC_PREC_MIXED) {
-    THC_TRY(oz_syrk(B.b, k, g2, g2, 1, v_out, k, B.syrk_w, s));
+    THC_TRY(oz_syrk(B.b, k, g2, g2, 1, v_out, k, B.syrk_w, s, ev[10], ev[11]));
   } else {
-    THC_TRY(syrk_splitk(B.b, k, g2, g2, prec == THC_PREC_FP64 ? 0 : 1, v_out, k, B.syrk_w, s));
+    THC_TRY(syrk_splitk(B.b, k, g2, g2, prec == THC_PREC_FP64 ? 0 : 1, v_out, k, B.syrk_w, s, ev[10], ev[11]));
   }
   THC_HIP(hipEventRecord(ev[6], s));
   // 6. Luv = chol(V)
@@ -362,8 +368,15 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
     stats->stage_ms[6] = t;
     THC_HIP(hipEventElapsedTime(&t, ev[3], ev[4]));
     stats->stage_ms[7] = t;
+    THC_HIP(hipEventElapsedTime(&t, ev[8], ev[9]));
+    stats->stage_ms[8] = t;
+    THC_HIP(hipEventElapsedTime(&t, ev[10], ev[11]));
+    stats->stage_ms[9] = t;
   }
-  if (info[2]) return THC_ERR_ARG;
+  if (info[2]) {
+    set_last_error("an interpolating-point index lies outside [0, n)", __FILE__, __LINE__);
+    return THC_ERR_ARG;
+  }
   if (info[0]) return THC_ERR_NOT_POSDEF;
   if (info[1] && want_luv && luv_out) return THC_WARN_LUV_NOT_POSDEF;
   if (want_rank && cct_rank < k) return THC_WARN_RANK_DEFICIENT;

@@ -60,7 +60,7 @@ struct thc_ctx {
   thc::Arena ws;       // per-call scratch
   thc::Arena kmws;     // k-means state (kept across calls for graph replay)
   KmGraphCache* kmgraph = nullptr;
-  hipEvent_t ev[10] = {};
+  hipEvent_t ev[12] = {};  // isdf stage boundaries (0-7), solve GEMM (8, 9), V kernel (10, 11)
   hipStream_t aux = nullptr;      // second stream: CC^T factorisation beside the ZC^T GEMM
   hipEvent_t fork = nullptr, join = nullptr;
   hipEvent_t kev[3] = {};  // k-means: centroid sums done, loop control done (by iteration parity)
@@ -125,8 +125,9 @@ int gemm_f32(const GemmArgs& g, hipStream_t s);
 // V = A A^T for A (m x k row-major, lda), written full symmetric into v (ldv).
 // A is double (prec 0) or float (prec 1); partials live in 'work'.
 size_t syrk_work_bytes(long m, long k, int prec);
+// (e0, e1 non-null: recorded on s around the partial-tile kernel alone)
 int syrk_splitk(const void* a, long m, long k, long lda, int prec, double* v, long ldv,
-                void* work, hipStream_t s);
+                void* work, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
 // ---- dense FP64 factorisations (linalg.hip) --------------------------------
 size_t chol_work_bytes(int k);
@@ -154,14 +155,16 @@ int symmetrize_from_lower(double* a, int k, long lda, hipStream_t s);
 // V = B B^T (B: k x g2, f64 if prec 0, f32 if prec 1, row stride ld) from
 // exact int8 digit products, written full symmetric into v.
 size_t oz_syrk_work_bytes(int k, long g2);
-int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s);
+int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s,
+            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // X = ainv * Z (k x n) from Zt (n x k row-major, ZC^T transposed) by exact
 // int8 digit products of the scaled operands ainv diag(s(idx)) and
 // Zt / (s(r) s(idx)), s(r) = sqrt(sum phi_a(r)^2 sum phi_b(r)^2) (phi_b null:
 // sum phi_a(r)^2).  X is float (x_f32) or double with row stride ldx.
 size_t oz_solve_work_bytes(int k, long n);
 int oz_solve(const double* ainv, long lda, const double* zt, int k, long n, const int* idx, const double* phi_a,
-             int ma, const double* phi_b, int mb, void* x, int x_f32, long ldx, void* work, hipStream_t s);
+             int ma, const double* phi_b, int mb, void* x, int x_f32, long ldx, void* work, hipStream_t s,
+             hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
 // ---- QR with column pivoting (qrcp.hip) ---------------------------------------
 // First k pivots of QRCP of the orbital-product matrix (occ = nullptr: all

@@ -708,7 +708,7 @@ size_t syrk_work_bytes(long m, long k, int prec) {
 }
 
 int syrk_splitk(const void* a, long m, long k, long lda, int prec, double* v, long ldv, void* work,
-                hipStream_t s) {
+                hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   if (m <= 0) return THC_OK;
   long ntiles, kchunk;
   int nsplit;
@@ -725,6 +725,7 @@ int syrk_splitk(const void* a, long m, long k, long lda, int prec, double* v, lo
   const dim3 rgrid((unsigned)(ntiles * (SY_BM / 32) * (SY_BN / 32)));  // 32 x 32 blocks of the tiles
   const int vw = prec == 0 ? 2 : 4;
   const bool vec = aligned16(a) && lda % vw == 0;
+  if (e0) THC_HIP(hipEventRecord(e0, s));
   if (prec == 0) {
     // direct-to-LDS tiles when every split covers whole 16-deep k-steps
     static_assert(SY_BM == DTL_BM && SY_BN == DTL_BN && SY_BK == DTL_BK, "same tile as the direct-to-LDS kernel");
@@ -733,11 +734,13 @@ int syrk_splitk(const void* a, long m, long k, long lda, int prec, double* v, lo
     else
       launch_vec<double, CfgLarge, EPI_PARTIAL, true, true, false, false>(p, vec, grid, s);
     THC_HIP(hipGetLastError());
+    if (e1) THC_HIP(hipEventRecord(e1, s));
     hipLaunchKernelGGL((syrk_reduce_kernel<double, SY_BM, SY_BN>), rgrid, dim3(256), 0, s,
                        (const double*)work, nsplit, ntiles, m, v, ldv);
   } else {
     launch_vec<float, CfgLarge, EPI_PARTIAL, true, true, false, false>(p, vec, grid, s);
     THC_HIP(hipGetLastError());
+    if (e1) THC_HIP(hipEventRecord(e1, s));
     hipLaunchKernelGGL((syrk_reduce_kernel<float, SY_BM, SY_BN>), rgrid, dim3(256), 0, s,
                        (const float*)work, nsplit, ntiles, m, v, ldv);
   }

@@ -832,7 +832,10 @@ int kmeans_run(thc_ctx* ctx, const double* grid, const double* rho, long n, int 
   if (n < 1 || k < 1 || k > n || max_it < 1 || n > 0x7fffffffL) return THC_ERR_ARG;
   if (k > MAXB) return THC_ERR_UNSUPPORTED;
   for (int j = 0; j < k; ++j)
-    if (host_init_idx[j] < 0 || host_init_idx[j] >= n) return THC_ERR_ARG;
+    if (host_init_idx[j] < 0 || host_init_idx[j] >= n) {
+      set_last_error("a starting index lies outside [0, n)", __FILE__, __LINE__);
+      return THC_ERR_ARG;
+    }
   THC_HIP(hipSetDevice(ctx->device));
   THC_TRY(km_set_attrs());
   const Layout L = plan(n, k, max_it);
@@ -943,7 +946,10 @@ int kmeans_update(thc_ctx* ctx, const double* rho, const double* grid, long n, c
   State hst;
   THC_HIP(hipMemcpyAsync(&hst, b.st, sizeof(State), hipMemcpyDeviceToHost, s));
   THC_HIP(hipStreamSynchronize(s));
-  if (hst.error & 2) return THC_ERR_ARG;
+  if (hst.error & 2) {
+    set_last_error("a grid_map entry lies outside [0, k)", __FILE__, __LINE__);
+    return THC_ERR_ARG;
+  }
   if (hst.serr[0]) return THC_ERR_EMPTY_CLUSTER;
   return THC_OK;
 }

@@ -20,8 +20,7 @@
 // global-to-LDS loads (global_load_lds_dwordx4: lane L of a wave writes bytes
 // 16 L .. 16 L + 15 of the wave's 1 KB block), so the next step's slabs stream
 // in while the current step's MFMAs run.  Rows are 64 bytes with no padding;
-// 16-byte segment s of row r is stored at segment (s + r / 4) mod 4, which
-// keeps the 16-row fragment reads free of bank conflicts.
+// 16-byte segment s of row r is stored at segment (s + r / 4) mod 4.
 #include <algorithm>
 
 #include "common.h"
@@ -312,7 +311,8 @@ size_t oz_syrk_work_bytes(int k, long g2) {
          align_up(sizeof(double) * (size_t)nchunk * ntiles * OZ_T * OZ_T, 256) + 256;
 }
 
-int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s) {
+int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s,
+            hipEvent_t e0, hipEvent_t e1) {
   if (k <= 0) return THC_OK;
   int nchunk;
   long g2p, ntiles;
@@ -332,8 +332,10 @@ int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ld
   THC_HIP(hipGetLastError());
   const long slots = (ntiles * nchunk + 7) / 8 * 8;  // every w < ntiles * nchunk gets one workgroup
   if (slots > 0x7fffffffL) return THC_ERR_ARG;
+  if (e0) THC_HIP(hipEventRecord(e0, s));
   hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, S, g2, g2p, k, E, nchunk, part, ntiles);
   THC_HIP(hipGetLastError());
+  if (e1) THC_HIP(hipEventRecord(e1, s));
   hipLaunchKernelGGL(oz_reduce, dim3((unsigned)(ntiles * (OZ_T / RB) * (OZ_T / RB))), dim3(256), 0, s, part, nchunk,
                      ntiles, k, v, ldv);
   THC_HIP(hipGetLastError());
@@ -670,7 +672,8 @@ size_t oz_solve_work_bytes(int k, long n) {
 }
 
 int oz_solve(const double* ainv, long lda, const double* zt, int k, long n, const int* idx, const double* phi_a,
-             int ma, const double* phi_b, int mb, void* x, int x_f32, long ldx, void* work, hipStream_t s) {
+             int ma, const double* phi_b, int mb, void* x, int x_f32, long ldx, void* work, hipStream_t s,
+             hipEvent_t e0, hipEvent_t e1) {
   if (k <= 0 || n <= 0) return THC_OK;
   const SvPlan P = sv_plan(k, n);
   char* w = reinterpret_cast<char*>(work);
@@ -695,6 +698,7 @@ int oz_solve(const double* ainv, long lda, const double* zt, int k, long n, cons
   THC_HIP(hipGetLastError());
   const long blocks = (P.ngroups + 7) / 8 * 256;  // every group gets its 32 workgroups on one XCD
   if (blocks > 0x7fffffffL) return THC_ERR_ARG;
+  if (e0) THC_HIP(hipEventRecord(e0, s));
   if (x_f32)
     hipLaunchKernelGGL(sv_gemm<float>, dim3((unsigned)blocks), dim3(512), 0, s, SA, EA, SZ, sc, k, P.kr, P.kp, n,
                        P.nr, (float*)x, ldx, P.mt, P.nt, P.ngroups);
@@ -702,6 +706,7 @@ int oz_solve(const double* ainv, long lda, const double* zt, int k, long n, cons
     hipLaunchKernelGGL(sv_gemm<double>, dim3((unsigned)blocks), dim3(512), 0, s, SA, EA, SZ, sc, k, P.kr, P.kp, n,
                        P.nr, (double*)x, ldx, P.mt, P.nt, P.ngroups);
   THC_HIP(hipGetLastError());
+  if (e1) THC_HIP(hipEventRecord(e1, s));
   return THC_OK;
 }
 
