@@ -39,7 +39,7 @@ SV_PRODUCTS = SV_DIGITS * (SV_DIGITS + 1) // 2
 SOLVE_TRAFFIC_BYTES = {"diamond64_n64_k2048": 17739351 * 1024 * 2.0 + 2097152 * 1024 * 1.0}
 METRIC = "ISDF factorizations/s + % roofline; mixed vs FP64 GPU at <=1e-6 rel V error"
 STAGES = ["zct", "cct_chol_inv", "solve", "fft", "V", "luv", "isdf_total", "refine", "solve_gemm_kernel",
-          "V_kernel"]
+          "V_kernel", "fft_pass_a", "fft_pass_b"]
 
 
 def parse():
@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--config", default="diamond64_n64_k2048")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg (and the errors against it)")
-    p.add_argument("--no-batch", action="store_true", help="skip the 32-cell batched leg")
+    p.add_argument("--no-batch", action="store_true", help="skip the 32-cell batched leg and the 8-atom cell leg")
     p.add_argument("--no-variants", action="store_true", help="skip the FP32 and refinement legs")
     p.add_argument("--copy-cal", action="store_true",
                    help="only run a 1 GiB float64 device copy three times (the known-size kernel that "
@@ -126,7 +126,8 @@ def rel_rows(dev, host):
 def batched_leg(ctx, steps, warmup):
     """The uneven 32-cell batch (side 6-14 bohr, 8-64 atoms, grids 24^3-48^3,
     N_mu = 8-10 M) through thc_factorize_batched, mixed precision, on 4
-    streams and, for comparison, on 1."""
+    streams and, for comparison, on 1; then the plain FP64 path one system
+    at a time (one stream), the baseline of the batched speed-up."""
     import torch
     from thcpp_amd import _lib as L
     from thcpp_amd import synthetic
@@ -136,19 +137,77 @@ def batched_leg(ctx, steps, warmup):
     out = {"workload": "batch of 32 uneven diamond cells, 20 k-means iterations each",
            "systems": len(batch), "sum_N": int(sum(s.N for s in batch)),
            "sum_N_mu": int(sum(s.K for s in batch))}
-    for streams in (4, 1):
+    for name, prec, streams in (("streams_4", L.PREC_MIXED, 4), ("streams_1", L.PREC_MIXED, 1),
+                                ("fp64_streams_1", L.PREC_FP64, 1)):
         for _ in range(warmup):
-            ctx.factorize_batched(systems, precision=L.PREC_MIXED, max_streams=streams)
+            ctx.factorize_batched(systems, precision=prec, max_streams=streams)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            st, _res = ctx.factorize_batched(systems, precision=L.PREC_MIXED, max_streams=streams)
+            st, _res = ctx.factorize_batched(systems, precision=prec, max_streams=streams)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         if st != L.THC_OK:
             raise RuntimeError("batched factorisation status %d" % st)
-        out["streams_%d" % streams] = {"ms_per_batch": round(dt * 1e3, 3),
-                                       "systems_per_s": round(len(batch) / dt, 2)}
+        out[name] = {"ms_per_batch": round(dt * 1e3, 3), "systems_per_s": round(len(batch) / dt, 2)}
+    out["mixed_streams_4_over_fp64_streams_1"] = round(out["fp64_streams_1"]["ms_per_batch"] /
+                                                       out["streams_4"]["ms_per_batch"], 3)
+    return out
+
+
+def small_cell_leg(ctx, steps, with_cpu):
+    """The 8-atom cell (32^3 grid, M = 16, N_mu = 128, 20 k-means iterations),
+    the case the reference runs on a CPU: GPU FP64 and mixed factorisation
+    rates and, with with_cpu, the CPU oracle's complete factorisation and the
+    GPU results against it."""
+    import torch
+    from thcpp_amd import _lib as L
+    from thcpp_amd import synthetic
+    s = synthetic.make_config("diamond8_n32")
+    g, r, ao, cg = ctx.f64(s.grid), ctx.f64(s.rho), ctx.f64(s.aoR), ctx.f64(s.coulG)
+    out = {"workload": "diamond8_n32_k128", "N": s.N, "M": s.M, "N_mu": s.K, "kmeans_iters": s.max_it}
+    res = {}
+    for name, prec in (("fp64", L.PREC_FP64), ("mixed", L.PREC_MIXED)):
+        for _ in range(3):
+            o = ctx.factorize(g, r, ao, s.thc_cfac, s.max_it, s.n, cg, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o = ctx.factorize(g, r, ao, s.thc_cfac, s.max_it, s.n, cg, precision=prec)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        if o["status"] != L.THC_OK:
+            raise RuntimeError("factorisation status %d" % o["status"])
+        res[name] = o
+        out[name] = {"ms_per_step": round(dt * 1e3, 3), "value": round(1.0 / dt, 2)}
+    if with_cpu:
+        base, ref = cpu_oracle(s)
+        out["cpu_baseline"] = base
+        out["kmeans_points_identical"] = bool(all(np.array_equal(res[p]["interp_idx"].cpu().numpy(), ref["idx"])
+                                                  for p in res))
+        for name in res:
+            out["rel_V_vs_oracle_" + name] = rel(res[name]["V"].double().cpu().numpy(), ref["V"])
+    return out
+
+
+def bandwidth(stages, s, x_bytes, es):
+    """Achieved GB/s of the HBM-bound stages from their device times: each FFT
+    pass reads and writes the k-row block once (pass A: x in, the complex
+    z/y-transformed half spectrum out; pass B: that spectrum and the weights
+    in, the weighted spectrum out); k-means reads the coordinates (24 B) and
+    the density (8 B) and writes the assignment (4 B) of every point once per
+    iteration."""
+    K, N = s.K, s.N
+    nh = s.n[0] * s.n[1] * (s.n[2] // 2 + 1)
+    a_bytes = K * N * x_bytes + K * nh * 2 * es
+    b_bytes = 2 * K * nh * 2 * es + nh * es
+    km_bytes = 36.0 * N * s.max_it
+    out = {}
+    for name, nbytes, ms in (("fft_pass_a", a_bytes, stages[10]), ("fft_pass_b", b_bytes, stages[11]),
+                             ("kmeans", km_bytes, stages[12])):
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        out[name] = {"bytes": float(nbytes), "ms": round(float(ms), 3), "GB_s": round(gbs, 1),
+                     "frac_hbm": round(gbs / PEAK["hbm_gbs"], 3)}
     return out
 
 
@@ -221,6 +280,7 @@ def main():
         "fp64_path": {"value": round(1000.0 / res["fp64"]["ms"], 3), "ms_per_step": round(res["fp64"]["ms"], 3),
                       "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES + ["kmeans"], res["fp64"]["stages"])}},
         "mixed_over_fp64_speedup": round(res["fp64"]["ms"] / ms, 3),
+        "bandwidth": {"mixed": bandwidth(st, s, 4, 4), "fp64": bandwidth(res["fp64"]["stages"], s, 8, 8)},
         "rel_V_error_mixed_vs_fp64": rel(mx["V"], res["fp64"]["V"]),
         "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES + ["kmeans"], st)},
         "roofline": {"kernel": "solve GEMM X = CC^T^-1 ZC^T as exact int8 digit products (sv_gemm, "
@@ -243,6 +303,7 @@ def main():
                               "rel_V_vs_fp64_path": rel(ref1["V"], res["fp64"]["V"])}}
     if not a.no_batch:
         line["batched"] = batched_leg(ctx, max(1, min(a.steps, 3)), 1)
+        line["small_cell"] = small_cell_leg(ctx, 20, not a.no_cpu)
     if not a.no_cpu:
         base, ref = cpu_oracle(s)
         line["cpu_baseline"] = base

@@ -98,10 +98,11 @@ typedef struct thc_isdf_stats {
   int luv_chol_info;       /* same for V */
   int cct_rank;            /* numerical rank of CC^T (opts->check_rank), else -1 */
   double cct_smax;         /* largest singular value of CC^T (opts->check_rank), else 0 */
-  double stage_ms[10];     /* device time: 0 ZC^T, 1 the part of CC^T + Cholesky + inverse not hidden behind ZC^T
+  double stage_ms[12];     /* device time: 0 ZC^T, 1 the part of CC^T + Cholesky + inverse not hidden behind ZC^T
                               (they run on a second stream beside it), 2 solve, 3 FFT, 4 V, 5 Luv, 6 total,
                               7 refinement, 8 the solve's GEMM kernel alone (inside 2), 9 the V contraction's
-                              main kernel alone (inside 4) */
+                              main kernel alone (inside 4), 10 and 11 the FFT's first (z and y) and second
+                              (x, weighting) pass kernels (inside 3) */
 } thc_isdf_stats;
 
 /* ---------------------------------------------------------------- context */

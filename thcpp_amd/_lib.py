@@ -43,7 +43,7 @@ class IsdfOpts(ctypes.Structure):
 
 class IsdfStats(ctypes.Structure):
     _fields_ = [("cct_chol_info", _i), ("luv_chol_info", _i), ("cct_rank", _i), ("cct_smax", _d),
-                ("stage_ms", _d * 10)]
+                ("stage_ms", _d * 12)]
 
 
 class System(ctypes.Structure):

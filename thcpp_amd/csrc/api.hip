@@ -329,7 +329,7 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   }
   THC_HIP(hipEventRecord(ev[4], s));
   // 4. weighted half spectrum
-  THC_TRY(fft_coulomb(B.x, x_f32, k, fft, coulG, B.b, prec == THC_PREC_FP64 ? 0 : 1, B.fft_w, s));
+  THC_TRY(fft_coulomb(B.x, x_f32, k, fft, coulG, B.b, prec == THC_PREC_FP64 ? 0 : 1, B.fft_w, s, ev + 12));
   THC_HIP(hipEventRecord(ev[5], s));
   // 5. V = B B^T, FP64 sum of the split-K partials (exact int8 digit products on the mixed path)
   if (prec == THC_PREC_MIXED) {
@@ -372,6 +372,10 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
     stats->stage_ms[8] = t;
     THC_HIP(hipEventElapsedTime(&t, ev[10], ev[11]));
     stats->stage_ms[9] = t;
+    THC_HIP(hipEventElapsedTime(&t, ev[12], ev[13]));
+    stats->stage_ms[10] = t;
+    THC_HIP(hipEventElapsedTime(&t, ev[13], ev[14]));
+    stats->stage_ms[11] = t;
   }
   if (info[2]) {
     set_last_error("an interpolating-point index lies outside [0, n)", __FILE__, __LINE__);

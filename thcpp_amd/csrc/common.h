@@ -60,7 +60,7 @@ struct thc_ctx {
   thc::Arena ws;       // per-call scratch
   thc::Arena kmws;     // k-means state (kept across calls for graph replay)
   KmGraphCache* kmgraph = nullptr;
-  hipEvent_t ev[12] = {};  // isdf stage boundaries (0-7), solve GEMM (8, 9), V kernel (10, 11)
+  hipEvent_t ev[15] = {};  // isdf stage boundaries (0-7), solve GEMM (8, 9), V kernel (10, 11), FFT passes (12-14)
   hipStream_t aux = nullptr;      // second stream: CC^T factorisation beside the ZC^T GEMM
   hipEvent_t fork = nullptr, join = nullptr;
   hipEvent_t kev[3] = {};  // k-means: centroid sums done, loop control done (by iteration parity)
@@ -178,8 +178,10 @@ int qrcp_points(const double* phi, const double* occ, long n, int m, int mo, int
 size_t fft_work_bytes(int k, const int n[3], int prec);
 // x: k x n0 n1 n2, double or (x_f32, prec 1 only) float; arithmetic and out
 // (k x g2 weighted half spectrum) in double (prec 0) or float (prec 1)
+// pass_ev (optional, 3 events): recorded before the first pass, between the
+// passes and after the second
 int fft_coulomb(const void* x, int x_f32, int k, const int n[3], const double* coulG, void* out, int prec,
-                void* work, hipStream_t s);
+                void* work, hipStream_t s, hipEvent_t* pass_ev = nullptr);
 inline long half_spectrum_len(const int n[3]) { return 2L * n[0] * n[1] * (n[2] / 2 + 1); }
 
 }  // namespace thc

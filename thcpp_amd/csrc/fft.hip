@@ -424,7 +424,8 @@ int launch_b(dim3 g, size_t lb, const C2<T>* tmid, const T* wt, T* out, const Ff
 }
 
 template <class T, class X>
-int run(const X* x, int k, const int n[3], const double* coulG, void* out, void* work, hipStream_t s) {
+int run(const X* x, int k, const int n[3], const double* coulG, void* out, void* work, hipStream_t s,
+        hipEvent_t* pass_ev) {
   FftParams p;
   p.k = k; p.n0 = n[0]; p.n1 = n[1]; p.n2 = n[2]; p.n2h = n[2] / 2 + 1;
   if (!make_plan(n[2], p.pz) || !make_plan(n[1], p.py) || !make_plan(n[0], p.px)) return THC_ERR_UNSUPPORTED;
@@ -454,13 +455,16 @@ int run(const X* x, int k, const int n[3], const double* coulG, void* out, void*
   p.twz = dz; p.twy = dy; p.twx = dx;
   T* o = reinterpret_cast<T*>(out);
   p.vec_out = (reinterpret_cast<uintptr_t>(out) % (2 * sizeof(T))) == 0;
+  if (pass_ev) THC_HIP(hipEventRecord(pass_ev[0], s));
   if (ipa == 16) THC_TRY((launch_a<T, 16, X>(dim3(n[0], k), la, x, tmid, p, s)));
   else if (ipa == 24) THC_TRY((launch_a<T, 24, X>(dim3(n[0], k), la, x, tmid, p, s)));
   else THC_TRY((launch_a<T, 0, X>(dim3(n[0], k), la, x, tmid, p, s)));
+  if (pass_ev) THC_HIP(hipEventRecord(pass_ev[1], s));
   if (ipb == 16) THC_TRY((launch_b<T, 16>(dim3(n[1], k), lb, tmid, wt, o, p, s)));
   else if (ipb == 24) THC_TRY((launch_b<T, 24>(dim3(n[1], k), lb, tmid, wt, o, p, s)));
   else THC_TRY((launch_b<T, 0>(dim3(n[1], k), lb, tmid, wt, o, p, s)));
   THC_HIP(hipGetLastError());
+  if (pass_ev) THC_HIP(hipEventRecord(pass_ev[2], s));
   return THC_OK;
 }
 
@@ -474,16 +478,16 @@ size_t fft_work_bytes(int k, const int n[3], int prec) {
 }
 
 int fft_coulomb(const void* x, int x_f32, int k, const int n[3], const double* coulG, void* out, int prec,
-                void* work, hipStream_t s) {
+                void* work, hipStream_t s, hipEvent_t* pass_ev) {
   if (k <= 0) return THC_OK;
   for (int d = 0; d < 3; ++d)
     if (n[d] < 1 || n[d] > MAXN) return THC_ERR_UNSUPPORTED;
   if (prec == 0) {
     if (x_f32) return THC_ERR_ARG;  // FP64 arithmetic keeps FP64 input
-    return run<double>(reinterpret_cast<const double*>(x), k, n, coulG, out, work, s);
+    return run<double>(reinterpret_cast<const double*>(x), k, n, coulG, out, work, s, pass_ev);
   }
-  if (x_f32) return run<float>(reinterpret_cast<const float*>(x), k, n, coulG, out, work, s);
-  return run<float>(reinterpret_cast<const double*>(x), k, n, coulG, out, work, s);
+  if (x_f32) return run<float>(reinterpret_cast<const float*>(x), k, n, coulG, out, work, s, pass_ev);
+  return run<float>(reinterpret_cast<const double*>(x), k, n, coulG, out, work, s, pass_ev);
 }
 
 }  // namespace thc
