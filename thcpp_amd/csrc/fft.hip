@@ -16,6 +16,15 @@
 // transformed along y.  Pass B, one workgroup per (mu, y-index): transform along
 // x, weight, store.  Both passes run a Stockham autosort FFT in LDS with radix
 // 8, 4, 2, 3 and 5 stages; twiddles come from a table of e^{-2 pi i j/n}.
+//
+// LDS layout: every transform line is contiguous, lines ls = n rounded up to
+// 32 elements apart, and element pos of line l sits at l ls + (pos ^ h(l)).
+// The XOR spreads the lines of one wave over the banks: a Stockham read takes
+// elements t + q nb of 8 lines (t < 8 across lanes) and a first-stage write
+// elements 8 t + q; h puts the line in bits 0-2 and bits 3-4 of the position,
+// so both touch every bank pair once per 32 lanes.  The transposes between
+// transforms (z-lines to y-lines, the strided x-lines of the global layout)
+// are done on the way into LDS, so no stage reads a strided line.
 #include <algorithm>
 
 #include "common.h"
@@ -55,6 +64,7 @@ struct Plan1D {
 
 struct FftParams {
   int k, n0, n1, n2, n2h;
+  int lsz, lsy, lsx;  // LDS line strides of the z, y and x transforms
   Plan1D pz, py, px;
   const void* twz; const void* twy; const void* twx;
   int vec_out;  // output aligned to two elements: one vector store per complex value
@@ -69,6 +79,15 @@ struct FDiv {
   __device__ explicit FDiv(int dd) : d((unsigned)dd), m(dd > 1 ? 0xFFFFFFFFu / (unsigned)dd + 1u : 0u) {}
   __device__ int div(int e) const { return d > 1 ? (int)__umulhi((unsigned)e, m) : e; }
 };
+
+inline __host__ __device__ int line_stride(int n) { return (n + 31) / 32 * 32; }
+
+// LDS index of element pos of line 'line' (pos < ls, ls a multiple of 32, so
+// pos ^ h stays inside the line)
+__device__ inline int swz(int line, int pos, int ls) {
+  const int h = (line & 7) | ((((line & 3) ^ ((line >> 3) & 3))) << 3);
+  return line * ls + (pos ^ h);
+}
 
 // Radix-R butterfly of one Stockham stage: twiddles w^(j q tstep) on the
 // inputs (ns > 1), then the R-point DFT.
@@ -116,36 +135,35 @@ __device__ inline void butterfly(C2<T> (&x)[R], C2<T> (&y)[R], int n, int ns, in
 // One Stockham stage over 'nlines' lines of length n, read src write dst.
 template <class T, int R>
 __device__ inline void stage(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, int n, int ns, int nlines,
-                             int es, int ls, const C2<T>* __restrict__ tw) {
+                             int ls, const C2<T>* __restrict__ tw) {
   const int nb = n / R;
   const int total = nlines * nb;
   const int tstep = n / (ns * R);
   const FDiv fnb(nb), fns(ns);
   for (int w = threadIdx.x; w < total; w += blockDim.x) {
     const int line = fnb.div(w), t = w - line * nb;
-    const int base = line * ls;
     const int tq = fns.div(t), j = t - tq * ns;
     C2<T> x[R], y[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) x[q] = src[base + (t + q * nb) * es];
+    for (int q = 0; q < R; ++q) x[q] = src[swz(line, t + q * nb, ls)];
     butterfly<T, R>(x, y, n, ns, j, tstep, tw);
     const int d0 = tq * ns * R + j;
 #pragma unroll
-    for (int q = 0; q < R; ++q) dst[base + (d0 + q * ns) * es] = y[q];
+    for (int q = 0; q < R; ++q) dst[swz(line, d0 + q * ns, ls)] = y[q];
   }
 }
 
 // Full 1-D transforms of 'nlines' lines; returns the buffer holding the result.
 template <class T>
-__device__ C2<T>* fft_lines(C2<T>* a, C2<T>* b, const Plan1D& pl, int nlines, int es, int ls, const C2<T>* tw) {
+__device__ C2<T>* fft_lines(C2<T>* a, C2<T>* b, const Plan1D& pl, int nlines, int ls, const C2<T>* tw) {
   int ns = 1;
   for (int s = 0; s < pl.nst; ++s) {
     const int r = pl.rad[s];
-    if (r == 8) stage<T, 8>(a, b, pl.n, ns, nlines, es, ls, tw);
-    else if (r == 4) stage<T, 4>(a, b, pl.n, ns, nlines, es, ls, tw);
-    else if (r == 2) stage<T, 2>(a, b, pl.n, ns, nlines, es, ls, tw);
-    else if (r == 3) stage<T, 3>(a, b, pl.n, ns, nlines, es, ls, tw);
-    else stage<T, 5>(a, b, pl.n, ns, nlines, es, ls, tw);
+    if (r == 8) stage<T, 8>(a, b, pl.n, ns, nlines, ls, tw);
+    else if (r == 4) stage<T, 4>(a, b, pl.n, ns, nlines, ls, tw);
+    else if (r == 2) stage<T, 2>(a, b, pl.n, ns, nlines, ls, tw);
+    else if (r == 3) stage<T, 3>(a, b, pl.n, ns, nlines, ls, tw);
+    else stage<T, 5>(a, b, pl.n, ns, nlines, ls, tw);
     __syncthreads();
     ns *= r;
     C2<T>* t = a; a = b; b = t;
@@ -162,7 +180,7 @@ __device__ C2<T>* fft_lines(C2<T>* a, C2<T>* b, const Plan1D& pl, int nlines, in
 constexpr int FT = 256;  // threads per workgroup of both passes
 
 template <class T, int R, int IPV>
-__device__ inline void stage_ip(C2<T>* buf, int n, int ns, int nlines, int es, int ls, const C2<T>* __restrict__ tw) {
+__device__ inline void stage_ip(C2<T>* buf, int n, int ns, int nlines, int ls, const C2<T>* __restrict__ tw) {
   constexpr int NB = IPV / R > 0 ? IPV / R : 1;
   const int nb = n / R;
   const int total = nlines * nb;
@@ -175,7 +193,7 @@ __device__ inline void stage_ip(C2<T>* buf, int n, int ns, int nlines, int es, i
     if (w < total) {
       const int line = fnb.div(w), t = w - line * nb;
 #pragma unroll
-      for (int q = 0; q < R; ++q) x[i][q] = buf[line * ls + (t + q * nb) * es];
+      for (int q = 0; q < R; ++q) x[i][q] = buf[swz(line, t + q * nb, ls)];
     }
   }
   __syncthreads();
@@ -189,22 +207,22 @@ __device__ inline void stage_ip(C2<T>* buf, int n, int ns, int nlines, int es, i
       butterfly<T, R>(x[i], y, n, ns, j, tstep, tw);
       const int d0 = tq * ns * R + j;
 #pragma unroll
-      for (int q = 0; q < R; ++q) buf[line * ls + (d0 + q * ns) * es] = y[q];
+      for (int q = 0; q < R; ++q) buf[swz(line, d0 + q * ns, ls)] = y[q];
     }
   }
   __syncthreads();
 }
 
 template <class T, int IPV>
-__device__ void fft_lines_ip(C2<T>* buf, const Plan1D& pl, int nlines, int es, int ls, const C2<T>* tw) {
+__device__ void fft_lines_ip(C2<T>* buf, const Plan1D& pl, int nlines, int ls, const C2<T>* tw) {
   int ns = 1;
   for (int s = 0; s < pl.nst; ++s) {
     const int r = pl.rad[s];
-    if (r == 8) stage_ip<T, 8, IPV>(buf, pl.n, ns, nlines, es, ls, tw);
-    else if (r == 4) stage_ip<T, 4, IPV>(buf, pl.n, ns, nlines, es, ls, tw);
-    else if (r == 2) stage_ip<T, 2, IPV>(buf, pl.n, ns, nlines, es, ls, tw);
-    else if (r == 3) stage_ip<T, 3, IPV>(buf, pl.n, ns, nlines, es, ls, tw);
-    else stage_ip<T, 5, IPV>(buf, pl.n, ns, nlines, es, ls, tw);
+    if (r == 8) stage_ip<T, 8, IPV>(buf, pl.n, ns, nlines, ls, tw);
+    else if (r == 4) stage_ip<T, 4, IPV>(buf, pl.n, ns, nlines, ls, tw);
+    else if (r == 2) stage_ip<T, 2, IPV>(buf, pl.n, ns, nlines, ls, tw);
+    else if (r == 3) stage_ip<T, 3, IPV>(buf, pl.n, ns, nlines, ls, tw);
+    else stage_ip<T, 5, IPV>(buf, pl.n, ns, nlines, ls, tw);
     ns *= r;
   }
 }
@@ -226,9 +244,9 @@ template <class T, int IPV, class X>  // IPV = 0: ping-pong stages
 __global__ __launch_bounds__(FT) void pass_a(const X* __restrict__ x, C2<T>* __restrict__ t, FftParams p) {
   constexpr bool ip = IPV > 0;
   extern __shared__ unsigned char smem_raw[];
-  const int n1 = p.n1, n2 = p.n2, n2h = p.n2h;
+  const int n1 = p.n1, n2 = p.n2, n2h = p.n2h, lsz = p.lsz, lsy = p.lsy;
   const int n1p = (n1 + 1) / 2;
-  const int cap = max(n1p * n2, n1 * n2h);
+  const int cap = max(n1p * lsz, n2h * lsy);
   C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
   C2<T>* bufb = ip ? bufa : bufa + cap;  // in-place stages use one buffer
   C2<T>* twz = bufb + cap;  // twiddle tables staged in LDS (each stage reads them)
@@ -255,22 +273,24 @@ __global__ __launch_bounds__(FT) void pass_a(const X* __restrict__ x, C2<T>* __r
 #pragma unroll
     for (int b = 0; b < LB; ++b) {
       const int e = e0 + b * (int)blockDim.x;
-      if (e < tot) bufa[e] = C2<T>{(T)re[b], (T)im[b]};
+      const int r = fn2.div(e), z = e - r * n2;
+      if (e < tot) bufa[swz(r, z, lsz)] = C2<T>{(T)re[b], (T)im[b]};
     }
   }
   __syncthreads();
-  // unpack: A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / (2i)
+  // unpack: A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / (2i),
+  // written transposed: y-line kz holds A and B of every row pair
   C2<T>* fin;
   if (ip) {
-    fft_lines_ip<T, IPV>(bufa, p.pz, n1p, 1, n2, twz);
+    fft_lines_ip<T, IPV>(bufa, p.pz, n1p, lsz, twz);
     C2<T> z1[UPV], z2[UPV];
 #pragma unroll
     for (int i = 0; i < UPV; ++i) {
       const int e = threadIdx.x + i * FT;
       if (e < n1p * n2h) {
         const int r = fn2h.div(e), kz = e - r * n2h;
-        z1[i] = bufa[r * n2 + kz];
-        z2[i] = bufa[r * n2 + (kz == 0 ? 0 : n2 - kz)];
+        z1[i] = bufa[swz(r, kz, lsz)];
+        z2[i] = bufa[swz(r, kz == 0 ? 0 : n2 - kz, lsz)];
       }
     }
     __syncthreads();
@@ -279,32 +299,35 @@ __global__ __launch_bounds__(FT) void pass_a(const X* __restrict__ x, C2<T>* __r
       const int e = threadIdx.x + i * FT;
       if (e < n1p * n2h) {
         const int r = fn2h.div(e), kz = e - r * n2h;
-        bufa[(2 * r) * n2h + kz] = C2<T>{(z1[i].x + z2[i].x) * T(0.5), (z1[i].y - z2[i].y) * T(0.5)};
+        bufa[swz(kz, 2 * r, lsy)] = C2<T>{(z1[i].x + z2[i].x) * T(0.5), (z1[i].y - z2[i].y) * T(0.5)};
         if (2 * r + 1 < n1)
-          bufa[(2 * r + 1) * n2h + kz] = C2<T>{(z1[i].y + z2[i].y) * T(0.5), (z2[i].x - z1[i].x) * T(0.5)};
+          bufa[swz(kz, 2 * r + 1, lsy)] = C2<T>{(z1[i].y + z2[i].y) * T(0.5), (z2[i].x - z1[i].x) * T(0.5)};
       }
     }
     __syncthreads();
-    fft_lines_ip<T, IPV>(bufa, p.py, n2h, n2h, 1, twy);
+    fft_lines_ip<T, IPV>(bufa, p.py, n2h, lsy, twy);
     fin = bufa;
   } else {
-    C2<T>* res = fft_lines<T>(bufa, bufb, p.pz, n1p, 1, n2, twz);
+    C2<T>* res = fft_lines<T>(bufa, bufb, p.pz, n1p, lsz, twz);
     C2<T>* h = (res == bufa) ? bufb : bufa;
     for (int e = threadIdx.x; e < n1p * n2h; e += blockDim.x) {
       const int r = fn2h.div(e), kz = e - r * n2h;
-      const C2<T> z1 = res[r * n2 + kz];
-      const C2<T> z2 = res[r * n2 + (kz == 0 ? 0 : n2 - kz)];
+      const C2<T> z1 = res[swz(r, kz, lsz)];
+      const C2<T> z2 = res[swz(r, kz == 0 ? 0 : n2 - kz, lsz)];
       const C2<T> a = C2<T>{(z1.x + z2.x) * T(0.5), (z1.y - z2.y) * T(0.5)};
       const C2<T> bb = C2<T>{(z1.y + z2.y) * T(0.5), (z2.x - z1.x) * T(0.5)};
-      h[(2 * r) * n2h + kz] = a;
-      if (2 * r + 1 < n1) h[(2 * r + 1) * n2h + kz] = bb;
+      h[swz(kz, 2 * r, lsy)] = a;
+      if (2 * r + 1 < n1) h[swz(kz, 2 * r + 1, lsy)] = bb;
     }
     __syncthreads();
     C2<T>* other = (h == bufa) ? bufb : bufa;
-    fin = fft_lines<T>(h, other, p.py, n2h, n2h, 1, twy);
+    fin = fft_lines<T>(h, other, p.py, n2h, lsy, twy);
   }
   C2<T>* dst = t + (mu * p.n0 + ix) * (long)n1 * n2h;
-  for (int e = threadIdx.x; e < n1 * n2h; e += blockDim.x) dst[e] = fin[e];
+  for (int e = threadIdx.x; e < n1 * n2h; e += blockDim.x) {
+    const int y = fn2h.div(e), kz = e - y * n2h;
+    dst[e] = fin[swz(kz, y, lsy)];
+  }
 }
 
 // Half-spectrum weights sqrt(w(kz) coulG(kx, ky, kz)) rounded to T, w = 1 for
@@ -327,10 +350,10 @@ __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const 
                                              T* __restrict__ out, FftParams p) {
   constexpr bool ip = IPV > 0;
   extern __shared__ unsigned char smem_raw[];
-  const int n0 = p.n0, n1 = p.n1, n2 = p.n2, n2h = p.n2h;
+  const int n0 = p.n0, n1 = p.n1, n2h = p.n2h, lsx = p.lsx;
   C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
-  C2<T>* bufb = ip ? bufa : bufa + n0 * n2h;
-  C2<T>* twx = bufb + n0 * n2h;  // twiddle table staged in LDS
+  C2<T>* bufb = ip ? bufa : bufa + n2h * lsx;
+  C2<T>* twx = bufb + n2h * lsx;  // twiddle table staged in LDS
   for (int j = threadIdx.x; j < n0; j += blockDim.x) twx[j] = reinterpret_cast<const C2<T>*>(p.twx)[j];
   const long mu = blockIdx.y;
   const int iy = blockIdx.x;
@@ -349,20 +372,21 @@ __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const 
 #pragma unroll
     for (int b = 0; b < LB; ++b) {
       const int e = e0 + b * (int)blockDim.x;
-      if (e < tot) bufa[e] = v[b];
+      const int ix = fn2h.div(e), kz = e - ix * n2h;
+      if (e < tot) bufa[swz(kz, ix, lsx)] = v[b];  // x-line kz
     }
   }
   __syncthreads();
   C2<T>* fin = bufa;
-  if (ip) fft_lines_ip<T, IPV>(bufa, p.px, n2h, n2h, 1, twx);
-  else fin = fft_lines<T>(bufa, bufb, p.px, n2h, n2h, 1, twx);
+  if (ip) fft_lines_ip<T, IPV>(bufa, p.px, n2h, lsx, twx);
+  else fin = fft_lines<T>(bufa, bufb, p.px, n2h, lsx, twx);
   typedef T v2 __attribute__((ext_vector_type(2)));
   T* dst = out + mu * (long)n0 * n1 * n2h * 2;
   for (int e = threadIdx.x; e < n0 * n2h; e += blockDim.x) {
     const int kx = fn2h.div(e), kz = e - kx * n2h;
     const long h = ((long)kx * n1 + iy) * n2h + kz;
     const T f = wt[h];
-    const C2<T> v = fin[e];
+    const C2<T> v = fin[swz(kz, kx, lsx)];
     if (p.vec_out) {
       *reinterpret_cast<v2*>(dst + 2 * h) = v2{v.x * f, v.y * f};
     } else {
@@ -400,11 +424,12 @@ constexpr int MAXN = 128;
 template <class T>
 size_t lds_a(int n1, int n2, bool ip) {
   const int n2h = n2 / 2 + 1, n1p = (n1 + 1) / 2;
-  return (ip ? 1 : 2) * sizeof(C2<T>) * (size_t)std::max(n1p * n2, n1 * n2h) + sizeof(C2<T>) * (size_t)(n1 + n2);
+  const size_t cap = (size_t)std::max(n1p * line_stride(n2), n2h * line_stride(n1));
+  return (ip ? 1 : 2) * sizeof(C2<T>) * cap + sizeof(C2<T>) * (size_t)(n1 + n2);
 }
 template <class T>
 size_t lds_b(int n0, int n2, bool ip) {
-  return (ip ? 1 : 2) * sizeof(C2<T>) * (size_t)n0 * (n2 / 2 + 1) + sizeof(C2<T>) * (size_t)n0;
+  return (ip ? 1 : 2) * sizeof(C2<T>) * (size_t)(n2 / 2 + 1) * line_stride(n0) + sizeof(C2<T>) * (size_t)n0;
 }
 
 template <class T, int IPV, class X>
@@ -428,6 +453,7 @@ int run(const X* x, int k, const int n[3], const double* coulG, void* out, void*
         hipEvent_t* pass_ev) {
   FftParams p;
   p.k = k; p.n0 = n[0]; p.n1 = n[1]; p.n2 = n[2]; p.n2h = n[2] / 2 + 1;
+  p.lsz = line_stride(n[2]); p.lsy = line_stride(n[1]); p.lsx = line_stride(n[0]);
   if (!make_plan(n[2], p.pz) || !make_plan(n[1], p.py) || !make_plan(n[0], p.px)) return THC_ERR_UNSUPPORTED;
   // in-place stages (one LDS buffer) when every stage and the unpack keep to
   // their per-thread register budgets
