@@ -312,13 +312,15 @@ typedef struct thc_system {
 
 /* Factorise many independent systems of uneven sizes (the reference runs one
  * system per thcpp.x invocation, src/main.cpp:18-77; this is that loop).
- * Systems are spread, largest first, over up to max_streams HIP streams of
- * the context (<= 0: 4), each with its own workspace and driven by its own
- * host thread, so that small systems overlap on the GPU.  Each system's
- * results equal thc_factorize's on the same inputs.  Returns THC_OK when
- * every system returned THC_OK, else the first error status by system order
- * (or, without errors, the first warning); per-system statuses are in
- * systems[i].status. */
+ * The k-means of all systems run together on the context's stream, every
+ * kernel launch covering one step of every system (kstats.elapsed_ms is the
+ * time of that whole batched k-means); the ISDF stages are then spread,
+ * largest first, over up to max_streams HIP streams (<= 0: 4; at most 16),
+ * each with its own workspace and driven by its own host thread, so that
+ * small systems overlap on the GPU.  Each system's results equal
+ * thc_factorize's on the same inputs.  Returns THC_OK when every system
+ * returned THC_OK, else the first error status by system order (or, without
+ * errors, the first warning); per-system statuses are in systems[i].status. */
 int thc_factorize_batched(thc_ctx *ctx, int nsys, thc_system *systems, const thc_isdf_opts *opts,
                           int max_streams);
 

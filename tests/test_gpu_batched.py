@@ -92,3 +92,23 @@ def test_batched_error_isolated(ctx, batch):
 def test_batched_empty(ctx):
     st, outs = ctx.factorize_batched([])
     assert st == L.THC_OK and outs == []
+
+
+def test_batched_kmeans_uneven_loops(ctx, batch):
+    """Systems with different iteration counts, seeds and a threshold that
+    stops one loop after its first iteration, run in one batched k-means:
+    each system's points, iteration count and V equal its own factorisation."""
+    systems = []
+    for i, (max_it, thr, seed) in enumerate([(3, -1.0, 0), (7, -1.0, 5), (12, -1.0, 2), (6, 1e10, 0)]):
+        d = as_dict(batch[i])
+        d["max_it"], d["threshold"], d["seed"] = max_it, thr, seed
+        systems.append(d)
+    print("batched k-means, uneven loops: %d systems" % len(systems), flush=True)
+    st, outs = ctx.factorize_batched(systems, precision=L.PREC_FP64, max_streams=2)
+    assert st == L.THC_OK, (st, [o["status"] for o in outs], L.lib().thc_last_error())
+    for s, d, o in zip(batch, systems, outs):
+        one = ctx.factorize(s.grid, s.rho, s.aoR, s.thc_cfac, d["max_it"], s.n, s.coulG, threshold=d["threshold"],
+                            seed=d["seed"])
+        assert o["iters"] == (1 if d["threshold"] > 0 else d["max_it"])
+        assert torch.equal(o["interp_idx"], one["interp_idx"])
+        assert torch.equal(o["V"], one["V"])

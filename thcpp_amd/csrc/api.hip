@@ -751,27 +751,40 @@ int thc_factorize(thc_ctx* ctx, const double* grid, const double* rho, const dou
 int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isdf_opts* opts, int max_streams) {
   if (!ctx || nsys < 0 || (nsys > 0 && !sys)) return THC_ERR_ARG;
   if (nsys == 0) return THC_OK;
-  const int nl = std::min(std::min(max_streams > 0 ? max_streams : 4, nsys), 16);
   THC_HIP(hipSetDevice(ctx->device));
-  // lane 0 is the context itself; the others are created once and kept
+  // 1. the k-means of every system in one batched run on the context's stream
+  std::vector<KmSpec> spec(nsys);
+  std::vector<int> kst(nsys, THC_OK);
+  std::vector<thc_kmeans_stats> kstats(nsys);
+  for (int i = 0; i < nsys; ++i) {
+    const thc_system& q = sys[i];
+    const long k = (long)q.thc_cfac * q.m;
+    spec[i] = KmSpec{q.grid, q.rho, q.n, (int)std::min<long>(k, 0x7fffffffL), q.max_it, q.threshold, q.seed,
+                     q.interp_idx};
+    if (q.thc_cfac < 1 || q.m < 1 || k > q.n || k > 0x7fffffffL) spec[i].k = -1;  // refused below
+    kstats[i] = q.kstats;  // keeps the caller's diffs / pcts arrays
+  }
+  THC_TRY(kmeans_run_many(ctx, nsys, spec.data(), kst.data(), kstats.data()));
+  for (int i = 0; i < nsys; ++i) {
+    sys[i].kstats = kstats[i];
+    sys[i].status = kst[i];
+  }
+  // 2. the ISDF stages, spread largest first over up to max_streams lanes (the
+  //    context and up to 15 more it keeps), each driven by its own host thread
+  const int nl = std::min(std::min(max_streams > 0 ? max_streams : 4, nsys), 16);
   while ((int)ctx->lanes.size() < nl - 1) {
     thc_ctx* c = nullptr;
     THC_TRY(thc_ctx_create(ctx->device, nullptr, &c));
     c->use_graphs = ctx->use_graphs;
     ctx->lanes.push_back(c);
   }
-  // with several lanes the streams outnumber the hardware queues: keep each
-  // lane's k-means on its own stream
-  const bool overlap0 = ctx->km_overlap;
-  ctx->km_overlap = nl == 1;
-  for (thc_ctx* c : ctx->lanes) c->km_overlap = nl == 1;
-  // longest first onto the least loaded lane (cost ~ solve + k-means work)
-  std::vector<int> order(nsys);
+  std::vector<int> order;
   std::vector<double> cost(nsys);
   for (int i = 0; i < nsys; ++i) {
-    order[i] = i;
+    if (sys[i].status != THC_OK && sys[i].status < 100) continue;  // k-means failed: no ISDF
+    order.push_back(i);
     const double k = (double)sys[i].thc_cfac * sys[i].m;
-    cost[i] = (double)sys[i].n * k * (k + sys[i].m + 4.0 * std::max(sys[i].max_it, 0));
+    cost[i] = (double)sys[i].n * k * (k + sys[i].m);
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
   std::vector<std::vector<int>> plan(nl);
@@ -782,6 +795,8 @@ int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isd
     load[j] += cost[i];
   }
   std::vector<std::string> errs(nsys);
+  for (int i = 0; i < nsys; ++i)
+    if (sys[i].status != THC_OK && sys[i].status < 100) errs[i] = "k-means of this system failed";
   auto work = [&](int lane) {
     thc_ctx* c = lane == 0 ? ctx : ctx->lanes[lane - 1];
     const bool dev_ok = hipSetDevice(c->device) == hipSuccess;
@@ -792,8 +807,10 @@ int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isd
         errs[i] = "hipSetDevice failed in a batch worker";
         continue;
       }
-      q.status = thc_factorize(c, q.grid, q.rho, q.aoR, q.n, q.m, q.thc_cfac, q.max_it, q.threshold, q.seed,
-                               q.fft_grid, q.coulG, opts, q.interp_idx, q.v, q.luv, &q.kstats, &q.istats);
+      const int k = q.thc_cfac * q.m;
+      const int st2 = isdf_run(c, q.aoR, nullptr, q.n, q.m, 0, false, q.interp_idx, k, q.fft_grid, q.coulG, opts,
+                               q.v, q.luv, nullptr, &q.istats);
+      if (st2 != THC_OK) q.status = st2;  // else keep the k-means status (OK or a warning)
       if (q.status != THC_OK && q.status < 100) errs[i] = g_last_error;
     }
   };
@@ -807,7 +824,6 @@ int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isd
   }
   work(0);
   for (auto& t : threads) t.join();
-  ctx->km_overlap = overlap0;
   int first_warn = THC_OK;
   for (int i = 0; i < nsys; ++i) {
     if (sys[i].status != THC_OK && sys[i].status < 100) {

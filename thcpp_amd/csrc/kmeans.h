@@ -57,4 +57,20 @@ int kmeans_update(thc_ctx* ctx, const double* rho, const double* grid, long n, c
 int kmeans_map_to_grid(thc_ctx* ctx, const double* grid, long n, const double* centroids, int k,
                        int* interp_idx);
 
+// One system of a batched k-means (kmeans_run_many).
+struct KmSpec {
+  const double* grid;
+  const double* rho;
+  long n;
+  int k, max_it;
+  double threshold;
+  unsigned int seed;
+  int* interp_idx;  // out, k ints
+};
+// The k-means of nj systems in one set of launches on the context's stream
+// (each launch covers one step of every system).  status[j] gets what
+// thc_kmeans would return for system j; stats (optional, nj entries) its
+// statistics, elapsed_ms being the device time of the whole batch.
+int kmeans_run_many(thc_ctx* ctx, int nj, const KmSpec* spec, int* status, thc_kmeans_stats* stats);
+
 }  // namespace thc

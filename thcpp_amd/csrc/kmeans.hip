@@ -17,6 +17,7 @@
 #pragma clang fp contract(off)
 #include <cfloat>
 #include <climits>
+#include <vector>
 
 #include "common.h"
 #include "kmeans.h"
@@ -71,9 +72,9 @@ __device__ inline int wave_sum(int v) {
 
 // ------------------------------------------------------------------ setup
 
-__global__ __launch_bounds__(256) void bbox_partial(const double* grid, long n, double* part) {
+__device__ __forceinline__ void bbox_partial_b(const double* grid, long n, double* part, int bid, int nblk) {
   double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x)
+  for (long i = bid * 256L + threadIdx.x; i < n; i += 256L * nblk)
     for (int d = 0; d < 3; ++d) {
       const double v = grid[i * 3 + d];
       lo[d] = fmin(lo[d], v);
@@ -90,16 +91,22 @@ __global__ __launch_bounds__(256) void bbox_partial(const double* grid, long n, 
     const int d = threadIdx.x;
     double r = s[d][0];
     for (int q = 1; q < 4; ++q) r = d < 3 ? fmin(r, s[d][q]) : fmax(r, s[d][q]);
-    part[blockIdx.x * 6 + d] = r;
+    part[bid * 6 + d] = r;
   }
 }
+__global__ __launch_bounds__(256) void bbox_partial(const double* grid, long n, double* part) {
+  bbox_partial_b(grid, n, part, (int)blockIdx.x, (int)gridDim.x);
+}
 
-__global__ void bbox_final(const double* part, int nb, double* box) {
+__device__ __forceinline__ void bbox_final_b(const double* part, int nb, double* box, int bid, int nblk) {
   const int d = threadIdx.x;
   if (d >= 6) return;
   double r = part[d];
   for (int b = 1; b < nb; ++b) r = d < 3 ? fmin(r, part[b * 6 + d]) : fmax(r, part[b * 6 + d]);
   box[d] = r;
+}
+__global__ void bbox_final(const double* part, int nb, double* box) {
+  bbox_final_b(part, nb, box, (int)blockIdx.x, (int)gridDim.x);
 }
 
 __device__ inline unsigned spread3(unsigned v) {
@@ -108,9 +115,9 @@ __device__ inline unsigned spread3(unsigned v) {
   return r;
 }
 
-__global__ __launch_bounds__(256) void morton_keys(const double* grid, long n, const double* box, int cells,
-                                                   int* keys) {
-  const long i = blockIdx.x * 256L + threadIdx.x;
+__device__ __forceinline__ void morton_keys_b(const double* grid, long n, const double* box, int cells,
+                                                   int* keys, int bid, int nblk) {
+  const long i = bid * 256L + threadIdx.x;
   if (i >= n) return;
   unsigned c[3];
   for (int d = 0; d < 3; ++d) {
@@ -121,20 +128,28 @@ __global__ __launch_bounds__(256) void morton_keys(const double* grid, long n, c
   }
   keys[i] = (int)((spread3(c[0]) << 2) | (spread3(c[1]) << 1) | spread3(c[2]));
 }
+__global__ __launch_bounds__(256) void morton_keys(const double* grid, long n, const double* box, int cells,
+                                                   int* keys) {
+  morton_keys_b(grid, n, box, cells, keys, (int)blockIdx.x, (int)gridDim.x);
+}
 
-__global__ __launch_bounds__(256) void gather_sorted(const double* grid, const int* perm, long n, double* sx,
-                                                     double* sy, double* sz) {
-  const long p = blockIdx.x * 256L + threadIdx.x;
+__device__ __forceinline__ void gather_sorted_b(const double* grid, const int* perm, long n, double* sx,
+                                                     double* sy, double* sz, int bid, int nblk) {
+  const long p = bid * 256L + threadIdx.x;
   if (p >= n) return;
   const long i = perm[p];
   sx[p] = grid[i * 3 + 0];
   sy[p] = grid[i * 3 + 1];
   sz[p] = grid[i * 3 + 2];
 }
+__global__ __launch_bounds__(256) void gather_sorted(const double* grid, const int* perm, long n, double* sx,
+                                                     double* sy, double* sz) {
+  gather_sorted_b(grid, perm, n, sx, sy, sz, (int)blockIdx.x, (int)gridDim.x);
+}
 
-__global__ __launch_bounds__(256) void tile_boxes(const double* sx, const double* sy, const double* sz, long n,
-                                                  double* boxes) {
-  const long p = blockIdx.x * (long)TP + threadIdx.x;
+__device__ __forceinline__ void tile_boxes_b(const double* sx, const double* sy, const double* sz, long n,
+                                                  double* boxes, int bid, int nblk) {
+  const long p = bid * (long)TP + threadIdx.x;
   double v[3];
   const bool ok = p < n;
   v[0] = ok ? sx[p] : 0;
@@ -151,20 +166,27 @@ __global__ __launch_bounds__(256) void tile_boxes(const double* sx, const double
     const int d = threadIdx.x;
     double r = s[d][0];
     for (int q = 1; q < 4; ++q) r = d < 3 ? fmin(r, s[d][q]) : fmax(r, s[d][q]);
-    boxes[blockIdx.x * 6 + d] = r;
+    boxes[bid * 6 + d] = r;
   }
 }
+__global__ __launch_bounds__(256) void tile_boxes(const double* sx, const double* sy, const double* sz, long n,
+                                                  double* boxes) {
+  tile_boxes_b(sx, sy, sz, n, boxes, (int)blockIdx.x, (int)gridDim.x);
+}
 
-__global__ __launch_bounds__(256) void init_centroids(const double* grid, const int* idx, int k, double* cen) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void init_centroids_b(const double* grid, const int* idx, int k, double* cen, int bid, int nblk) {
+  const int j = bid * 256 + threadIdx.x;
   if (j >= k) return;
   const long i = idx[j];
   cen[j * 3 + 0] = grid[i * 3 + 0];
   cen[j * 3 + 1] = grid[i * 3 + 1];
   cen[j * 3 + 2] = grid[i * 3 + 2];
 }
+__global__ __launch_bounds__(256) void init_centroids(const double* grid, const int* idx, int k, double* cen) {
+  init_centroids_b(grid, idx, k, cen, (int)blockIdx.x, (int)gridDim.x);
+}
 
-__global__ void init_state(State* st, int mode) {
+__device__ __forceinline__ void init_state_b(State* st, int mode, int bid, int nblk) {
   if (threadIdx.x == 0) {
     st->done = 0;
     st->final_buf = 0;
@@ -177,17 +199,20 @@ __global__ void init_state(State* st, int mode) {
     st->last_diff = 0.0;
   }
 }
+__global__ void init_state(State* st, int mode) {
+  init_state_b(st, mode, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // ------------------------------------------------------------------ stable counting sort
 
 // Per index-order tile of CS_TILE points: histogram of keys.
-__global__ __launch_bounds__(256) void cs_hist(const int* keys, long n, int nbins, int* tile_count,
-                                               const State* st) {
+__device__ __forceinline__ void cs_hist_b(const int* keys, long n, int nbins, int* tile_count,
+                                               const State* st, int bid, int nblk) {
   if (st && st->done) return;
   __shared__ int h[MAXB];
   for (int b = threadIdx.x; b < nbins; b += 256) h[b] = 0;
   __syncthreads();
-  const long base = (long)blockIdx.x * CS_TILE;
+  const long base = (long)bid * CS_TILE;
   for (int e = threadIdx.x; e < CS_TILE; e += 256) {
     const long i = base + e;
     if (i < n) {
@@ -196,19 +221,23 @@ __global__ __launch_bounds__(256) void cs_hist(const int* keys, long n, int nbin
     }
   }
   __syncthreads();
-  int* out = tile_count + (long)blockIdx.x * nbins;
+  int* out = tile_count + (long)bid * nbins;
   for (int b = threadIdx.x; b < nbins; b += 256) out[b] = h[b];
+}
+__global__ __launch_bounds__(256) void cs_hist(const int* keys, long n, int nbins, int* tile_count,
+                                               const State* st) {
+  cs_hist_b(keys, n, nbins, tile_count, st, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Column scan over tiles, in place: tile_count[t][b] -> exclusive offset of
 // tile t within bin b; total[b] = count of bin b.  A workgroup takes 64 bins;
 // its 4 waves scan a quarter of the tiles each, then add the earlier quarters.
-__global__ __launch_bounds__(256) void cs_colscan(int* tile_count, int ntiles, int nbins, int* total,
-                                                  const State* st) {
+__device__ __forceinline__ void cs_colscan_b(int* tile_count, int ntiles, int nbins, int* total,
+                                                  const State* st, int bid, int nblk) {
   if (st && st->done) return;
   __shared__ int part[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int b = blockIdx.x * 64 + lane;
+  const int b = bid * 64 + lane;
   const int per = (ntiles + 3) / 4;
   const int t0 = wave * per, t1 = min(ntiles, t0 + per);
   // chunks of 16 tiles: every load of a chunk is issued before its first use
@@ -243,9 +272,13 @@ __global__ __launch_bounds__(256) void cs_colscan(int* tile_count, int ntiles, i
     if (wave == 3) total[b] = off;
   }
 }
+__global__ __launch_bounds__(256) void cs_colscan(int* tile_count, int ntiles, int nbins, int* total,
+                                                  const State* st) {
+  cs_colscan_b(tile_count, ntiles, nbins, total, st, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // Exclusive scan of total[0..nbins) into start (one workgroup of 1024).
-__global__ __launch_bounds__(1024) void cs_totscan(const int* total, int nbins, int* start, const State* st) {
+__device__ __forceinline__ void cs_totscan_b(const int* total, int nbins, int* start, const State* st, int bid, int nblk) {
   if (st && st->done) return;
   __shared__ int s[1024];
   const int t = threadIdx.x;
@@ -272,22 +305,25 @@ __global__ __launch_bounds__(1024) void cs_totscan(const int* total, int nbins, 
     }
   }
 }
+__global__ __launch_bounds__(1024) void cs_totscan(const int* total, int nbins, int* start, const State* st) {
+  cs_totscan_b(total, nbins, start, st, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // Stable scatter: out[start[key] + rank] = index, rank counting earlier
 // indices with the same key.  Chunks of 256 go in order; inside a chunk the
 // four waves go in order; inside a wave, lanes with equal keys are ranked by
 // lane through a 'nbits'-ballot match.
-__global__ __launch_bounds__(256) void cs_scatter(const int* keys, long n, int nbins, int nbits,
+__device__ __forceinline__ void cs_scatter_b(const int* keys, long n, int nbins, int nbits,
                                                   const int* tile_off, const int* start, int* out,
-                                                  const State* st) {
+                                                  const State* st, int bid, int nblk) {
   if (st && st->done) return;
   __shared__ int cnt[MAXB];
   __shared__ int slot[256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int* toff = tile_off + (long)blockIdx.x * nbins;
+  const int* toff = tile_off + (long)bid * nbins;
   for (int b = tid; b < nbins; b += 256) cnt[b] = start[b] + toff[b];
   __syncthreads();
-  const long base = (long)blockIdx.x * CS_TILE;
+  const long base = (long)bid * CS_TILE;
   for (int ch = 0; ch < CS_TILE / 256; ++ch) {
     const long i = base + ch * 256 + tid;
     int key = i < n ? keys[i] : 0;
@@ -314,20 +350,25 @@ __global__ __launch_bounds__(256) void cs_scatter(const int* keys, long n, int n
     __syncthreads();
   }
 }
+__global__ __launch_bounds__(256) void cs_scatter(const int* keys, long n, int nbins, int nbits,
+                                                  const int* tile_off, const int* start, int* out,
+                                                  const State* st) {
+  cs_scatter_b(keys, n, nbins, nbits, tile_off, start, out, st, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // ------------------------------------------------------------------ Lloyd iteration
 
-__global__ __launch_bounds__(256) void assign(const double* sx, const double* sy, const double* sz,
+__device__ __forceinline__ void assign_b(const double* sx, const double* sy, const double* sz,
                                               const int* perm, long n, const double* boxes,
                                               const double* cen, int k, int* map_new, const int* map_old,
-                                              State* st, int par) {
+                                              State* st, int par, int bid, int nblk) {
   if (st && st->done) return;
   __shared__ int cidx[MAXB];
   __shared__ double ccx[CAND_CAP], ccy[CAND_CAP], ccz[CAND_CAP];
   __shared__ double redd[4];
   __shared__ int redi[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const double* b = boxes + (long)blockIdx.x * 6;
+  const double* b = boxes + (long)bid * 6;
   double box[6];
   for (int d = 0; d < 6; ++d) box[d] = b[d];
 
@@ -375,7 +416,7 @@ __global__ __launch_bounds__(256) void assign(const double* sx, const double* sy
   }
   __syncthreads();
 
-  const long p = (long)blockIdx.x * TP + tid;
+  const long p = (long)bid * TP + tid;
   int changed = 0;
   if (p < n) {
     const double gx = sx[p], gy = sy[p], gz = sz[p];
@@ -424,6 +465,12 @@ __global__ __launch_bounds__(256) void assign(const double* sx, const double* sy
     if (lane == 0 && changed) atomicAdd(&st->changed[par], changed);
   }
 }
+__global__ __launch_bounds__(256) void assign(const double* sx, const double* sy, const double* sz,
+                                              const int* perm, long n, const double* boxes,
+                                              const double* cen, int k, int* map_new, const int* map_old,
+                                              State* st, int par) {
+  assign_b(sx, sy, sz, perm, n, boxes, cen, k, map_new, map_old, st, par, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // One wave per cluster.  Lanes gather the members' rho and rho * x (each
 // product rounded on its own, as the reference's multiply-then-add does) into
@@ -431,13 +478,13 @@ __global__ __launch_bounds__(256) void assign(const double* sx, const double* sy
 // sequential sums (w, x, y, z) over them.
 constexpr int CS_CHUNK = 256;
 
-__global__ __launch_bounds__(256) void centroid_sums(const int* order, const int* start, const int* total,
+__device__ __forceinline__ void centroid_sums_b(const int* order, const int* start, const int* total,
                                                      const double* rho, const double* grid, int k,
-                                                     double* cen_out, State* st, int par) {
+                                                     double* cen_out, State* st, int par, int bid, int nblk) {
   if (st && st->done) return;
   __shared__ double buf[4][4][CS_CHUNK];  // [wave][w, x, y, z][member]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int c = blockIdx.x * 4 + wave;
+  const int c = bid * 4 + wave;
   if (c >= k) return;
   const int s = start[c], cnt = total[c];
   double acc = 0.0;  // lane q < 4 owns accumulator q
@@ -466,12 +513,17 @@ __global__ __launch_bounds__(256) void centroid_sums(const int* order, const int
   if (lane == 0 && w < 1e-8) atomicOr(&st->serr[par], 1);
   if (lane >= 1 && lane < 4) cen_out[c * 3 + lane - 1] = acc / w;
 }
+__global__ __launch_bounds__(256) void centroid_sums(const int* order, const int* start, const int* total,
+                                                     const double* rho, const double* grid, int k,
+                                                     double* cen_out, State* st, int par) {
+  centroid_sums_b(order, start, total, rho, grid, k, cen_out, st, par, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // normed_difference(new, cur) with the reference's sequential sum, % changed,
 // and the loop control of KMeansSolver::kernel (kmeans.cpp:233-257).
-__global__ __launch_bounds__(1024) void diff_control(const double* nxt, const double* cur, int k, long n,
+__device__ __forceinline__ void diff_control_b(const double* nxt, const double* cur, int k, long n,
                                                      double threshold, int it, int max_it, int nxt_buf,
-                                                     State* st, double* diffs, double* pcts) {
+                                                     State* st, double* diffs, double* pcts, int bid, int nblk) {
   if (st->done) return;
   extern __shared__ double terms[];
   __shared__ double red[4][16];
@@ -543,16 +595,21 @@ __global__ __launch_bounds__(1024) void diff_control(const double* nxt, const do
     }
   }
 }
+__global__ __launch_bounds__(1024) void diff_control(const double* nxt, const double* cur, int k, long n,
+                                                     double threshold, int it, int max_it, int nxt_buf,
+                                                     State* st, double* diffs, double* pcts) {
+  diff_control_b(nxt, cur, k, n, threshold, it, max_it, nxt_buf, st, diffs, pcts, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // ------------------------------------------------------------------ map_to_grid
 
 // Nearest grid point per centroid (lexicographic distance/index minimum over
 // the grid tiles the bounds cannot exclude).
-__global__ __launch_bounds__(256) void map_to_grid(const double* sx, const double* sy, const double* sz,
+__device__ __forceinline__ void map_to_grid_b(const double* sx, const double* sy, const double* sz,
                                                    const int* perm, long n, const double* boxes, int ntiles,
                                                    const double* cen3, long cstride, int latest_buf,
-                                                   const State* st, int* out) {
-  const int c = blockIdx.x;
+                                                   const State* st, int* out, int bid, int nblk) {
+  const int c = bid;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int mode = st ? st->mode : MODE_LATEST;
   if (mode == MODE_ZEROS || (st && st->error)) {
@@ -605,9 +662,15 @@ __global__ __launch_bounds__(256) void map_to_grid(const double* sx, const doubl
     out[c] = bi;
   }
 }
+__global__ __launch_bounds__(256) void map_to_grid(const double* sx, const double* sy, const double* sz,
+                                                   const int* perm, long n, const double* boxes, int ntiles,
+                                                   const double* cen3, long cstride, int latest_buf,
+                                                   const State* st, int* out) {
+  map_to_grid_b(sx, sy, sz, perm, n, boxes, ntiles, cen3, cstride, latest_buf, st, out, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // Sort the k indices ascending (bitonic in LDS) and count adjacent repeats.
-__global__ __launch_bounds__(1024) void sort_indices(int* idx, int k, State* st) {
+__device__ __forceinline__ void sort_indices_b(int* idx, int k, State* st, int bid, int nblk) {
   __shared__ int s[MAXB];
   int p2 = 1;
   while (p2 < k) p2 <<= 1;
@@ -633,6 +696,9 @@ __global__ __launch_bounds__(1024) void sort_indices(int* idx, int k, State* st)
   }
   d = wave_sum(d);
   if ((threadIdx.x & 63) == 0 && d && st) atomicAdd(&st->dups, d);
+}
+__global__ __launch_bounds__(1024) void sort_indices(int* idx, int k, State* st) {
+  sort_indices_b(idx, k, st, (int)blockIdx.x, (int)gridDim.x);
 }
 
 }  // namespace km
@@ -813,12 +879,196 @@ static int enqueue_kmeans(const Layout& L, const Bufs& b, const double* grid, co
   return THC_OK;
 }
 
+// ------------------------------------------------------------------ batches
+//
+// A batch runs the k-means of many systems together: every launch below
+// covers one step (setup, assignment, sort, centroid sums, loop control, map
+// to grid) of all the systems, so a 20-iteration batch of 32 systems is about
+// 160 launches instead of 32 times that.  Each workgroup finds its system in
+// a prefix table of per-system block counts and runs the same device code as
+// the single-system kernels with that system's sizes and buffers, so every
+// system's result is bit-identical to its own thc_kmeans.
+
+struct Job {
+  Layout L;
+  Bufs b;
+  const double* grid;
+  const double* rho;
+  double threshold;
+  int* interp_idx;
+};
+
+// block-count tables, one row of nj + 1 prefix sums per kind
+enum { JK_NBB, JK_N256, JK_NCS, JK_COLSP, JK_COLK, JK_TILES, JK_K256, JK_K4, JK_K, JK_COUNT };
+
+inline int job_blocks(const Layout& L, int kind) {
+  switch (kind) {
+    case JK_NBB: return L.nbb;
+    case JK_N256: return (int)ceil_div(L.n, 256);
+    case JK_NCS: return L.ncs;
+    case JK_COLSP: return ceil_div(L.nbins_sp, 64);
+    case JK_COLK: return ceil_div(L.k, 64);
+    case JK_TILES: return L.ntiles;
+    case JK_K256: return ceil_div(L.k, 256);
+    case JK_K4: return ceil_div(L.k, 4);
+    default: return L.k;
+  }
+}
+
+// system of workgroup b: off[j] <= b < off[j + 1]
+__device__ inline int job_of(const int* off, int nj, int b) {
+  int lo = 0, hi = nj;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+#define KM_JOB(OFF)                                 \
+  const int j_ = job_of(OFF, nj, (int)blockIdx.x);  \
+  const Job& q = jobs[j_];                          \
+  const int bid = (int)blockIdx.x - OFF[j_], nblk = OFF[j_ + 1] - OFF[j_];
+
+__global__ void init_state_m(const Job* jobs) { init_state_b(jobs[blockIdx.x].b.st, MODE_LATEST, 0, 1); }
+
+__global__ __launch_bounds__(256) void bbox_partial_m(const Job* jobs, const int* off, int nj) {
+  KM_JOB(off)
+  bbox_partial_b(q.grid, q.L.n, q.b.part, bid, nblk);
+}
+
+__global__ void bbox_final_m(const Job* jobs) {
+  const Job& q = jobs[blockIdx.x];
+  bbox_final_b(q.b.part, q.L.nbb, q.b.box, 0, 1);
+}
+
+__global__ __launch_bounds__(256) void morton_keys_m(const Job* jobs, const int* off, int nj) {
+  KM_JOB(off)
+  morton_keys_b(q.grid, q.L.n, q.b.box, q.L.cells, q.b.keys, bid, nblk);
+}
+
+__global__ __launch_bounds__(256) void gather_sorted_m(const Job* jobs, const int* off, int nj) {
+  KM_JOB(off)
+  gather_sorted_b(q.grid, q.b.perm, q.L.n, q.b.sx, q.b.sy, q.b.sz, bid, nblk);
+}
+
+__global__ __launch_bounds__(256) void tile_boxes_m(const Job* jobs, const int* off, int nj) {
+  KM_JOB(off)
+  tile_boxes_b(q.b.sx, q.b.sy, q.b.sz, q.L.n, q.b.boxes, bid, nblk);
+}
+
+__global__ __launch_bounds__(256) void init_centroids_m(const Job* jobs, const int* off, int nj) {
+  KM_JOB(off)
+  init_centroids_b(q.grid, q.b.idx0, q.L.k, q.b.cen3, bid, nblk);
+}
+
+// Counting sort of one batch: mode 0 sorts the Morton keys into perm (setup),
+// mode 1 the iteration's cluster map into order.  Systems past their last
+// iteration skip mode 1.
+struct SortArgs {
+  const int* keys;
+  int nbins, nbits;
+  int* out;
+  const State* st;
+};
+__device__ inline bool sort_args(const Job& q, int mode, int it, SortArgs& a) {
+  if (mode == 0) {
+    a = SortArgs{q.b.keys, q.L.nbins_sp, q.L.nbits_sp, q.b.perm, nullptr};
+    return true;
+  }
+  a = SortArgs{q.b.map[it & 1], q.L.k, q.L.nbits_k, q.b.order, q.b.st};
+  return it < q.L.max_it;
+}
+
+__global__ __launch_bounds__(256) void cs_hist_m(const Job* jobs, const int* off, int nj, int mode, int it) {
+  KM_JOB(off)
+  SortArgs a;
+  if (!sort_args(q, mode, it, a)) return;
+  cs_hist_b(a.keys, q.L.n, a.nbins, q.b.tc, a.st, bid, nblk);
+}
+
+__global__ __launch_bounds__(256) void cs_colscan_m(const Job* jobs, const int* off, int nj, int mode, int it) {
+  KM_JOB(off)
+  SortArgs a;
+  if (!sort_args(q, mode, it, a)) return;
+  cs_colscan_b(q.b.tc, q.L.ncs, a.nbins, q.b.total, a.st, bid, nblk);
+}
+
+__global__ __launch_bounds__(1024) void cs_totscan_m(const Job* jobs, int mode, int it) {
+  const Job& q = jobs[blockIdx.x];
+  SortArgs a;
+  if (!sort_args(q, mode, it, a)) return;
+  cs_totscan_b(q.b.total, a.nbins, q.b.start, a.st, 0, 1);
+}
+
+__global__ __launch_bounds__(256) void cs_scatter_m(const Job* jobs, const int* off, int nj, int mode, int it) {
+  KM_JOB(off)
+  SortArgs a;
+  if (!sort_args(q, mode, it, a)) return;
+  cs_scatter_b(a.keys, q.L.n, a.nbins, a.nbits, q.b.tc, q.b.start, a.out, a.st, bid, nblk);
+}
+
+__global__ __launch_bounds__(256) void assign_m(const Job* jobs, const int* off, int nj, int it) {
+  KM_JOB(off)
+  if (it >= q.L.max_it) return;
+  const int par = it & 1;
+  assign_b(q.b.sx, q.b.sy, q.b.sz, q.b.perm, q.L.n, q.b.boxes, q.b.cen3 + (it % 3) * 3L * q.L.k, q.L.k,
+           q.b.map[par], it == 0 ? nullptr : q.b.map[par ^ 1], q.b.st, par, bid, nblk);
+}
+
+__global__ __launch_bounds__(256) void centroid_sums_m(const Job* jobs, const int* off, int nj, int it) {
+  KM_JOB(off)
+  if (it >= q.L.max_it) return;
+  centroid_sums_b(q.b.order, q.b.start, q.b.total, q.rho, q.grid, q.L.k, q.b.cen3 + ((it + 1) % 3) * 3L * q.L.k,
+                  q.b.st, it & 1, bid, nblk);
+}
+
+__global__ __launch_bounds__(1024) void diff_control_m(const Job* jobs, int it) {
+  const Job& q = jobs[blockIdx.x];
+  if (it >= q.L.max_it) return;
+  const long cs = 3L * q.L.k;
+  diff_control_b(q.b.cen3 + ((it + 1) % 3) * cs, q.b.cen3 + (it % 3) * cs, q.L.k, q.L.n, q.threshold, it,
+                 q.L.max_it, (it + 1) % 3, q.b.st, q.b.diffs, q.b.pcts, 0, 1);
+}
+
+__global__ __launch_bounds__(256) void map_to_grid_m(const Job* jobs, const int* off, int nj) {
+  KM_JOB(off)
+  map_to_grid_b(q.b.sx, q.b.sy, q.b.sz, q.b.perm, q.L.n, q.b.boxes, q.L.ntiles, q.b.cen3, 3L * q.L.k,
+                q.L.max_it % 3, q.b.st, q.interp_idx, bid, nblk);
+}
+
+__global__ __launch_bounds__(1024) void sort_indices_m(const Job* jobs) {
+  const Job& q = jobs[blockIdx.x];
+  sort_indices_b(q.interp_idx, q.L.k, q.b.st, 0, 1);
+}
+
+// every system's State, diffs and pcts into one block for a single copy back:
+// system j at out + j * stride bytes
+constexpr long STATE_SLOT = (sizeof(State) + 63) / 64 * 64;  // State, then diffs and pcts
+
+__global__ void collect_m(const Job* jobs, char* out, long stride, int max_it) {
+  const Job& q = jobs[blockIdx.x];
+  char* o = out + blockIdx.x * stride;
+  const int t = threadIdx.x;
+  if (t == 0) *reinterpret_cast<State*>(o) = *q.b.st;
+  double* d = reinterpret_cast<double*>(o + STATE_SLOT);
+  for (int i = t; i < max_it; i += blockDim.x) {
+    d[i] = i < q.L.max_it ? q.b.diffs[i] : 0.0;
+    d[max_it + i] = i < q.L.max_it ? q.b.pcts[i] : 0.0;
+  }
+}
+
+#undef KM_JOB
+
 }  // namespace km
 
 static int km_attrs_done = 0;
 static int km_set_attrs() {
   if (km_attrs_done) return THC_OK;
   THC_HIP(hipFuncSetAttribute((const void*)km::diff_control, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              3 * km::MAXB * (int)sizeof(double)));
+  THC_HIP(hipFuncSetAttribute((const void*)km::diff_control_m, hipFuncAttributeMaxDynamicSharedMemorySize,
                               3 * km::MAXB * (int)sizeof(double)));
   km_attrs_done = 1;
   return THC_OK;
@@ -901,6 +1151,157 @@ int kmeans_run(thc_ctx* ctx, const double* grid, const double* rho, long n, int 
   }
   if (hst->error) return THC_ERR_EMPTY_CLUSTER;
   if (hst->dups) return THC_WARN_DUPLICATE_INDICES;
+  return THC_OK;
+}
+
+
+extern "C" int thc_seed_indices_impl(long n, int k, unsigned int seed, int* out);
+
+int kmeans_run_many(thc_ctx* ctx, int nj, const KmSpec* spec, int* status, thc_kmeans_stats* stats) {
+  using namespace km;
+  if (!ctx || nj < 0 || (nj > 0 && (!spec || !status))) return THC_ERR_ARG;
+  if (nj == 0) return THC_OK;
+  THC_HIP(hipSetDevice(ctx->device));
+  THC_TRY(km_set_attrs());
+  // host-side checks and layouts; a system that fails them gets its status
+  // and drops out of the batch
+  std::vector<int> live;
+  std::vector<Layout> lay(nj);
+  std::vector<std::vector<int>> init(nj);
+  size_t need = 0;
+  int max_it = 1, maxk = 1;
+  long idx_total = 0;
+  for (int j = 0; j < nj; ++j) {
+    const KmSpec& q = spec[j];
+    status[j] = THC_OK;
+    if (!q.grid || !q.rho || !q.interp_idx || q.n < 1 || q.k < 1 || q.k > q.n || q.max_it < 1 ||
+        q.n > 0x7fffffffL) {
+      status[j] = THC_ERR_ARG;
+      continue;
+    }
+    if (q.k > MAXB) {
+      status[j] = THC_ERR_UNSUPPORTED;
+      continue;
+    }
+    init[j].resize(q.k);
+    if (thc_seed_indices_impl(q.n, q.k, q.seed, init[j].data())) {
+      status[j] = THC_ERR_ARG;
+      continue;
+    }
+    lay[j] = plan(q.n, q.k, q.max_it);
+    need += bytes_for(lay[j], true);
+    max_it = std::max(max_it, q.max_it);
+    maxk = std::max(maxk, q.k);
+    idx_total += q.k;
+    live.push_back(j);
+  }
+  const int nl = (int)live.size();
+  if (nl == 0) return THC_OK;
+  // device blob: the job table, the block-count prefix tables, the starting
+  // indices; the results block comes back through collect_m
+  const size_t job_bytes = align_up(sizeof(Job) * nl, 256);
+  const size_t off_bytes = align_up(sizeof(int) * (size_t)JK_COUNT * (nl + 1), 256);
+  const size_t idx_bytes = align_up(sizeof(int) * (size_t)idx_total, 256);
+  const size_t blob = job_bytes + off_bytes + idx_bytes;
+  const long res_stride = (long)align_up(STATE_SLOT + 2 * sizeof(double) * max_it, 64);
+  const size_t res_bytes = (size_t)res_stride * nl;
+  need += blob + align_up(res_bytes, 256) + 1024;
+  if (need > ctx->kmws.cap) {
+    if (ctx->kmgraph) ctx->kmgraph->reset();
+    THC_TRY(ctx->kmws.reserve(need));
+  }
+  ctx->kmws.reset();
+  char* dblob = ctx->kmws.take<char>(blob);
+  char* dres = ctx->kmws.take<char>(res_bytes);
+  if (!dblob || !dres) return THC_ERR_NO_MEMORY;
+  THC_TRY(ensure_pinned(ctx, std::max(blob, res_bytes)));
+  char* hb = reinterpret_cast<char*>(ctx->pinned);
+  Job* hj = reinterpret_cast<Job*>(hb);
+  int* hoff = reinterpret_cast<int*>(hb + job_bytes);
+  int* hidx = reinterpret_cast<int*>(hb + job_bytes + off_bytes);
+  int* didx = reinterpret_cast<int*>(dblob + job_bytes + off_bytes);
+  long ipos = 0;
+  for (int t = 0; t < nl; ++t) {
+    const int j = live[t];
+    Job& J = hj[t];
+    J.L = lay[j];
+    if (!carve(ctx->kmws, J.L, J.b, true)) return THC_ERR_NO_MEMORY;
+    J.b.idx0 = didx + ipos;  // read by init_centroids only
+    std::copy(init[j].begin(), init[j].end(), hidx + ipos);
+    ipos += spec[j].k;
+    J.grid = spec[j].grid;
+    J.rho = spec[j].rho;
+    J.threshold = spec[j].threshold;
+    J.interp_idx = spec[j].interp_idx;
+  }
+  for (int kind = 0; kind < JK_COUNT; ++kind) {
+    int* o = hoff + kind * (nl + 1);
+    o[0] = 0;
+    for (int t = 0; t < nl; ++t) o[t + 1] = o[t] + job_blocks(hj[t].L, kind);
+  }
+  long total_blocks[JK_COUNT];
+  for (int kind = 0; kind < JK_COUNT; ++kind) total_blocks[kind] = hoff[kind * (nl + 1) + nl];
+  for (int kind = 0; kind < JK_COUNT; ++kind)
+    if (total_blocks[kind] > 0x7fffffffL) return THC_ERR_ARG;
+  hipStream_t s = ctx->stream;
+  const Job* dj = reinterpret_cast<const Job*>(dblob);
+  const int* doff = reinterpret_cast<const int*>(dblob + job_bytes);
+  auto off = [&](int kind) { return doff + kind * (nl + 1); };
+  auto grid_of = [&](int kind) { return dim3((unsigned)total_blocks[kind]); };
+  THC_HIP(hipEventRecord(ctx->ev[0], s));
+  THC_HIP(hipMemcpyAsync(dblob, hb, blob, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(init_state_m, dim3(nl), dim3(64), 0, s, dj);
+  hipLaunchKernelGGL(bbox_partial_m, grid_of(JK_NBB), dim3(256), 0, s, dj, off(JK_NBB), nl);
+  hipLaunchKernelGGL(bbox_final_m, dim3(nl), dim3(64), 0, s, dj);
+  hipLaunchKernelGGL(morton_keys_m, grid_of(JK_N256), dim3(256), 0, s, dj, off(JK_N256), nl);
+  auto sort = [&](int mode, int it) {
+    const int col = mode == 0 ? JK_COLSP : JK_COLK;
+    hipLaunchKernelGGL(cs_hist_m, grid_of(JK_NCS), dim3(256), 0, s, dj, off(JK_NCS), nl, mode, it);
+    hipLaunchKernelGGL(cs_colscan_m, grid_of(col), dim3(256), 0, s, dj, off(col), nl, mode, it);
+    hipLaunchKernelGGL(cs_totscan_m, dim3(nl), dim3(1024), 0, s, dj, mode, it);
+    hipLaunchKernelGGL(cs_scatter_m, grid_of(JK_NCS), dim3(256), 0, s, dj, off(JK_NCS), nl, mode, it);
+  };
+  sort(0, 0);
+  hipLaunchKernelGGL(gather_sorted_m, grid_of(JK_N256), dim3(256), 0, s, dj, off(JK_N256), nl);
+  hipLaunchKernelGGL(tile_boxes_m, grid_of(JK_TILES), dim3(256), 0, s, dj, off(JK_TILES), nl);
+  hipLaunchKernelGGL(init_centroids_m, grid_of(JK_K256), dim3(256), 0, s, dj, off(JK_K256), nl);
+  const size_t dyn = (size_t)3 * maxk * sizeof(double);
+  for (int it = 0; it < max_it; ++it) {
+    hipLaunchKernelGGL(assign_m, grid_of(JK_TILES), dim3(256), 0, s, dj, off(JK_TILES), nl, it);
+    sort(1, it);
+    hipLaunchKernelGGL(centroid_sums_m, grid_of(JK_K4), dim3(256), 0, s, dj, off(JK_K4), nl, it);
+    hipLaunchKernelGGL(diff_control_m, dim3(nl), dim3(1024), dyn, s, dj, it);
+  }
+  hipLaunchKernelGGL(map_to_grid_m, grid_of(JK_K), dim3(256), 0, s, dj, off(JK_K), nl);
+  hipLaunchKernelGGL(sort_indices_m, dim3(nl), dim3(1024), 0, s, dj);
+  hipLaunchKernelGGL(collect_m, dim3(nl), dim3(256), 0, s, dj, dres, res_stride, max_it);
+  THC_HIP(hipGetLastError());
+  // the pinned buffer still holds the blob being copied: read results back
+  // only after that copy (stream order) into the same buffer
+  THC_HIP(hipMemcpyAsync(hb, dres, res_bytes, hipMemcpyDeviceToHost, s));
+  THC_HIP(hipEventRecord(ctx->ev[1], s));
+  THC_HIP(hipStreamSynchronize(s));
+  float ms = 0;
+  THC_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  for (int t = 0; t < nl; ++t) {
+    const int j = live[t];
+    const char* r = hb + t * res_stride;
+    const State* hst = reinterpret_cast<const State*>(r);
+    const double* hd = reinterpret_cast<const double*>(r + STATE_SLOT);
+    const double* hp = hd + max_it;
+    if (stats) {
+      thc_kmeans_stats& o = stats[j];
+      o.iterations = hst->iters;
+      o.converged = hst->mode == MODE_FINAL;
+      o.duplicates = hst->dups;
+      o.interp_zeroed = hst->mode == MODE_ZEROS;
+      o.last_diff = hst->last_diff;
+      if (o.diffs) std::copy(hd, hd + hst->iters, o.diffs);
+      if (o.pcts) std::copy(hp, hp + hst->iters, o.pcts);
+      o.elapsed_ms = ms;
+    }
+    status[j] = hst->error ? THC_ERR_EMPTY_CLUSTER : (hst->dups ? THC_WARN_DUPLICATE_INDICES : THC_OK);
+  }
   return THC_OK;
 }
 
