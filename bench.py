@@ -103,7 +103,7 @@ def cpu_oracle(s):
     total = sum(t.values())
     base = {"value": round(1.0 / total, 5), "unit": "factorizations/s", "cores": cores, "kind": "port",
             "sample": "one complete factorisation of the same system, no extrapolation; stage seconds " +
-                      " ".join("%s %.1f" % kv for kv in t.items())}
+                      " ".join("%s %.3f" % kv for kv in t.items())}
     return base, {"idx": idx, "X": x, "V": v}
 
 

@@ -183,3 +183,33 @@ def test_kmeans_full_size_prefix(ctx):
     np.testing.assert_array_equal(out["diffs"], ref["diffs"])
     np.testing.assert_array_equal(out["pcts"], ref["pcts"])
     np.testing.assert_array_equal(out["interp_idx"].cpu().numpy(), ref["interp_idx"])
+
+
+def test_device_sqrt_correctly_rounded(ctx):
+    """The device FP64 square root of the distance kernels (thc_device_sqrt)
+    against numpy's IEEE sqrt, bit for bit: log-uniform arguments over the
+    whole normal range, subnormals, exact squares, the doubles next to exact
+    squares and next to the midpoints between adjacent square roots (the
+    hardest cases for a correctly rounded sqrt), 0, -0 and inf."""
+    import ctypes
+    rng = np.random.default_rng(11)
+    parts = [np.exp(rng.uniform(np.log(1e-300), np.log(1e300), 1 << 20)),
+             rng.uniform(0, 1, 1 << 14) * 2.0 ** -1022,  # subnormals
+             np.array([0.0, -0.0, np.inf, 1.0, 4.0, 2.0, 0.5, 2.0 ** -1074, np.finfo(np.float64).max])]
+    s = np.exp(rng.uniform(np.log(1e-150), np.log(1e150), 1 << 18))
+    sq = s * s  # squares of doubles, rounded
+    parts += [sq, np.nextafter(sq, 0), np.nextafter(sq, np.inf)]
+    mid = (s + np.nextafter(s, np.inf)) / 2  # midpoints between adjacent roots
+    m2 = mid * mid
+    parts += [m2, np.nextafter(m2, 0), np.nextafter(m2, np.inf)]
+    x = np.concatenate(parts)
+    x = x[np.isfinite(x) | np.isinf(x)]
+    xd = torch.from_numpy(x).to(ctx.device)
+    yd = torch.empty_like(xd)
+    print("device sqrt: %d arguments" % x.size, flush=True)
+    st = L.lib().thc_device_sqrt(ctx.h, ctypes.c_void_p(xd.data_ptr()), x.size, ctypes.c_void_p(yd.data_ptr()))
+    assert st == L.THC_OK
+    got = yd.cpu().numpy()
+    ref = np.sqrt(x)
+    bad = np.flatnonzero(got.view(np.int64) != ref.view(np.int64))
+    assert bad.size == 0, (bad.size, x[bad[:5]], got[bad[:5]], ref[bad[:5]])

@@ -71,6 +71,7 @@ SIGNATURES = {
     "thc_cholesky": (_i, [_vp, _vp, _i, ctypes.POINTER(_i)]),
     "thc_cct_rank": (_i, [_vp, _vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_d)]),
     "thc_cholesky_noalloc": (_i, [_vp, _vp, _i, ctypes.POINTER(_i)]),
+    "thc_device_sqrt": (_i, [_vp, _vp, _l, _vp]),
     "thc_least_squares": (_i, [_vp, _vp, _vp, _l, _i, _i, _vp, ctypes.POINTER(_i)]),
     "thc_fft_vectors": (_i, [_vp, _vp, _i, ctypes.POINTER(_i), _vp, _vp]),
     "thc_construct_muv": (_i, [_vp, _vp, _i, _l, _vp]),

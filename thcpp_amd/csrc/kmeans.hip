@@ -1378,4 +1378,23 @@ int kmeans_map_to_grid(thc_ctx* ctx, const double* grid, long n, const double* c
   return hst.dups ? THC_WARN_DUPLICATE_INDICES : THC_OK;
 }
 
+namespace km {
+// y = sqrt(x) with the square root the distance kernels above use
+__global__ __launch_bounds__(256) void device_sqrt(const double* x, long n, double* y) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x) y[i] = sqrt(x[i]);
+}
+}  // namespace km
+
 }  // namespace thc
+
+extern "C" int thc_device_sqrt(thc_ctx* ctx, const double* x, long n, double* y) {
+  using namespace thc;
+  if (!ctx || n < 0 || (n > 0 && (!x || !y))) return THC_ERR_ARG;
+  if (n == 0) return THC_OK;
+  THC_HIP(hipSetDevice(ctx->device));
+  const long blocks = std::min<long>(8192, (n + 255) / 256);
+  hipLaunchKernelGGL(km::device_sqrt, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, x, n, y);
+  THC_HIP(hipGetLastError());
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  return THC_OK;
+}
