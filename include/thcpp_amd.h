@@ -197,6 +197,33 @@ int thc_cct(thc_ctx *ctx, const double *czt, long n, const int *idx, int k, doub
  * Returns THC_WARN_RANK_DEFICIENT when the rank is below k. */
 int thc_cct_rank(thc_ctx *ctx, const double *cct, int k, int *rank, double *smax);
 
+/* ---------------------------------------------------------------- GEMM tile tuning
+ * No reference counterpart (ScaLAPACK chooses its own blocking).  The FP64 /
+ * FP32 GEMMs of every stage pick a tile configuration by a built-in rule on
+ * the tile count, unless the process-wide tuning table has an entry for the
+ * shape class.  The table is written by the project's autotuner
+ * (python -m thcpp_amd.autotune), which times every configuration with
+ * thc_gemm_time on the shapes of a workload. */
+enum thc_gemm_cfg {
+  THC_GEMM_AUTO = 0,   /* built-in rule */
+  THC_GEMM_TINY = 1,   /* 32 x 32 x 16 tile on one wave */
+  THC_GEMM_SMALL = 2,  /* 64 x 64 x 16 on 4 waves */
+  THC_GEMM_LARGE = 3,  /* 128 x 128 x 16 on 8 waves, register-staged */
+  THC_GEMM_DTL = 4,    /* 128 x 128 x 16 on 8 waves, direct-to-LDS loads (FP64 plain store only) */
+  THC_GEMM_DEEPK = 5   /* 128 x 128 x 32 on 8 waves, one workgroup per CU */
+};
+/* Replace the table: n entries of 5 ints (ceil(log2 m), ceil(log2 n),
+ * ceil(log2 k), flags, thc_gemm_cfg) with flags = 1 gathered operand |
+ * 2 lower triangle | 4 square epilogue | 8 triangular operands.  n = 0 clears
+ * it.  A configuration that does not apply to a call falls back to the rule. */
+int thc_gemm_tuning(const int *entries, int n);
+/* Average device time (ms, over reps launches after one warm-up) of one FP64
+ * GEMM on configuration cfg: kind 0 is C (m x n) = A (m x k) B (n x k)^T, the
+ * solve's layout; kind 1 is the ZC^T GEMM, C = (A A[idx]^T)^2 elementwise
+ * with n gathered rows.  Operands live in the context's workspace.
+ * THC_ERR_UNSUPPORTED when cfg does not apply to the shape. */
+int thc_gemm_time(thc_ctx *ctx, long m, long n, long k, int kind, int cfg, int reps, double *ms);
+
 /* In-place lower Cholesky of a k x k row-major SPD matrix (strict upper part
  * zeroed).  *info = 0 or the 1-based failing leading minor (pzpotrf 'L',
  * src/matrix_operations.h:259-267). */

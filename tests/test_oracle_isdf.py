@@ -242,3 +242,4 @@ def test_fft_and_muv_fast_options_agree():
     assert rel(b, a) < 1e-15
     m = IO.construct_muv(a, cg)
     assert rel(IO.construct_muv(a, cg, real_part=True), m.real) < 1e-14
+

@@ -23,6 +23,7 @@ THC_WARN_LUV_NOT_POSDEF = 101
 THC_WARN_RANK_DEFICIENT = 102
 
 PREC_FP64, PREC_FP32, PREC_MIXED = 0, 1, 3
+GEMM_AUTO, GEMM_TINY, GEMM_SMALL, GEMM_LARGE, GEMM_DTL, GEMM_DEEPK = 0, 1, 2, 3, 4, 5
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -72,6 +73,8 @@ SIGNATURES = {
     "thc_cct_rank": (_i, [_vp, _vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_d)]),
     "thc_cholesky_noalloc": (_i, [_vp, _vp, _i, ctypes.POINTER(_i)]),
     "thc_device_sqrt": (_i, [_vp, _vp, _l, _vp]),
+    "thc_gemm_tuning": (_i, [_vp, _i]),
+    "thc_gemm_time": (_i, [_vp, _l, _l, _l, _i, _i, _i, ctypes.POINTER(_d)]),
     "thc_least_squares": (_i, [_vp, _vp, _vp, _l, _i, _i, _vp, ctypes.POINTER(_i)]),
     "thc_fft_vectors": (_i, [_vp, _vp, _i, ctypes.POINTER(_i), _vp, _vp]),
     "thc_construct_muv": (_i, [_vp, _vp, _i, _l, _vp]),

@@ -7,11 +7,30 @@ reference's functions (src/kmeans.cpp, src/interpolating_vectors.cpp).
 from __future__ import annotations
 
 import ctypes
+import json
+import os
 
 import numpy as np
 import torch
 
 from . import _lib as L
+
+# GEMM tile-configuration table written by the autotuner (python -m thcpp_amd.autotune)
+TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "gemm_gfx950.json")
+_tuning_applied = [False]
+
+
+def apply_tuning(path=TUNING_FILE):
+    """Install the GEMM tuning table of 'path' (process-wide); a missing file
+    clears the table, leaving the built-in rule.  Returns the entry count."""
+    entries = []
+    if path and os.path.exists(path):
+        with open(path) as f:
+            entries = json.load(f)["entries"]
+    flat = (ctypes.c_int * (5 * len(entries)))(*[int(v) for e in entries for v in e])
+    L.check(L.lib().thc_gemm_tuning(flat if entries else None, len(entries)), "thc_gemm_tuning")
+    _tuning_applied[0] = True
+    return len(entries)
 
 
 def _p(t):
@@ -31,7 +50,11 @@ class Context:
         """stream: a torch.cuda.Stream or raw hipStream_t handle for the library
         to run on.  By default it is torch's current stream on the device; when
         that is the legacy default stream (handle 0) the library creates a
-        blocking stream of its own, which HIP orders after torch's work there."""
+        blocking stream of its own, which HIP orders after torch's work there.
+        The first context of a process installs the committed GEMM tuning
+        table (apply_tuning)."""
+        if not _tuning_applied[0]:
+            apply_tuning()
         self.device = torch.device("cuda", device)
         if stream is None:
             stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -53,6 +76,16 @@ class Context:
         if cur != 0 and cur != self._stream:
             torch.cuda.current_stream(self.device).synchronize()
         return L.lib()
+
+    def gemm_time(self, m, n, k, kind=0, cfg=0, reps=3):
+        """Device ms of one FP64 GEMM on tile configuration cfg (thc_gemm_time);
+        None when cfg does not apply to the shape."""
+        ms = ctypes.c_double()
+        st = self._lib().thc_gemm_time(self.h, m, n, k, kind, cfg, reps, ctypes.byref(ms))
+        if st == L.THC_ERR_UNSUPPORTED:
+            return None
+        L.check(st, "thc_gemm_time")
+        return ms.value
 
     def close(self):
         if self.h:

@@ -98,6 +98,17 @@ __global__ void check_idx(const int* idx, int k, long n, int* bad) {
 
 inline int grid_for(long count) { return (int)std::min<long>(8192, std::max<long>(1, (count + 255) / 256)); }
 
+// deterministic operands of the GEMM timing hook: values in [-0.5, 0.5),
+// gathered rows spread over [0, rows)
+__global__ void fill_pattern(double* a, long count) {
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < count; e += 256L * gridDim.x)
+    a[e] = (double)((unsigned long)(e * 2654435761UL) % 1000UL) * 1e-3 - 0.5;
+}
+__global__ void fill_spread(int* idx, int count, long rows) {
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < count; e += 256 * gridDim.x)
+    idx[e] = (int)(((long)e * rows) / count);
+}
+
 struct IsdfBufs {
   double *z, *x, *c, *l, *ci, *r, *v;
   double *phim, *phio;  // Phi rows at the points (k x m), occupied set (k x m_occ)
@@ -630,6 +641,52 @@ int thc_cholesky(thc_ctx* ctx, double* a, int k, int* info) {
   THC_HIP(hipMemcpyAsync(&h, d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   THC_HIP(hipStreamSynchronize(ctx->stream));
   if (info) *info = h;
+  return THC_OK;
+}
+
+int thc_gemm_tuning(const int* entries, int n) { return gemm_set_tuning(entries, n); }
+
+int thc_gemm_time(thc_ctx* ctx, long m, long n, long k, int kind, int cfg, int reps, double* ms) {
+  if (!ctx || !ms || m < 1 || n < 1 || k < 1 || (kind != 0 && kind != 1) || cfg < 0 || cfg >= GEMM_CFG_COUNT ||
+      reps < 1 || reps > 1000)
+    return THC_ERR_ARG;
+  if (kind == 1 && (n > 0x7fffffffL || m > 0x7fffffffL)) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  const long nb = kind == 0 ? n : 0;
+  const size_t need = carve_bytes<double>((size_t)m * k) + carve_bytes<double>((size_t)std::max(nb, 1L) * k) +
+                      carve_bytes<double>((size_t)m * n) + carve_bytes<int>((size_t)n) + 1024;
+  THC_TRY(ctx->ws.reserve(need));
+  ctx->ws.reset();
+  double* a = ctx->ws.take<double>((size_t)m * k);
+  double* b = ctx->ws.take<double>((size_t)std::max(nb, 1L) * k);
+  double* c = ctx->ws.take<double>((size_t)m * n);
+  int* idx = ctx->ws.take<int>((size_t)n);
+  if (!a || !b || !c || !idx) return THC_ERR_NO_MEMORY;
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(fill_pattern, dim3(grid_for(m * k)), dim3(256), 0, s, a, m * k);
+  if (kind == 0) hipLaunchKernelGGL(fill_pattern, dim3(grid_for(n * k)), dim3(256), 0, s, b, n * k);
+  else hipLaunchKernelGGL(fill_spread, dim3(grid_for(n)), dim3(256), 0, s, idx, (int)n, m);
+  THC_HIP(hipGetLastError());
+  GemmArgs g = {};
+  g.m = m; g.n = n; g.k = k;
+  g.a = a; g.lda = k; g.a_trans = 0;
+  if (kind == 0) {
+    g.b = b; g.ldb = k; g.b_trans = 1;
+  } else {  // the ZC^T GEMM: B rows gathered from A, square epilogue
+    g.b = a; g.ldb = k; g.b_trans = 1; g.b_rows = idx; g.b_nrows = m;
+    g.epi_square = 1;
+  }
+  g.c = c; g.ldc = n;
+  g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
+  g.cfg = cfg;
+  THC_TRY(gemm_f64(g, s));  // warm-up; refuses a configuration that does not apply
+  THC_HIP(hipEventRecord(ctx->ev[0], s));
+  for (int r = 0; r < reps; ++r) THC_TRY(gemm_f64(g, s));
+  THC_HIP(hipEventRecord(ctx->ev[1], s));
+  THC_HIP(hipStreamSynchronize(s));
+  float t = 0;
+  THC_HIP(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
+  *ms = t / reps;
   return THC_OK;
 }
 

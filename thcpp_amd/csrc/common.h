@@ -109,6 +109,7 @@ struct GemmArgs {
   int lower;
   int tri;       // GEMM_TRI_* bits: known zero blocks of A / B that bound each tile's k range
   int c_f32;     // FP64 GEMM storing C as float (plain store, beta = 0)
+  int cfg;       // GEMM_CFG_*: force a tile configuration (0: the tuning table, then the built-in rule)
   int batch;
   long stride_a, stride_b, stride_c;
 };
@@ -119,6 +120,21 @@ enum {
   GEMM_TRI_B_KGE_N = 2,  // B[q][j] = 0 for q < j  (B lower triangular)
   GEMM_TRI_A_KLE_M = 4,  // A[i][q] = 0 for q > i  (A lower triangular)
 };
+// Tile configurations of gemm_f64 / gemm_f32 (the tuning table's values).
+enum {
+  GEMM_CFG_AUTO = 0,
+  GEMM_CFG_TINY = 1,   // 32 x 32 x 16 on one wave
+  GEMM_CFG_SMALL = 2,  // 64 x 64 x 16 on 4 waves
+  GEMM_CFG_LARGE = 3,  // 128 x 128 x 16 on 8 waves, register-staged
+  GEMM_CFG_DTL = 4,    // 128 x 128 x 16 on 8 waves, direct-to-LDS (FP64, plain store, k-contiguous operands)
+  GEMM_CFG_DEEPK = 5,  // 128 x 128 x 32 on 8 waves, one workgroup per CU
+  GEMM_CFG_COUNT = 6
+};
+// Process-wide tuning table: entries of (ceil log2 m, ceil log2 n, ceil log2 k,
+// flags, config), flags = 1 gathered operand | 2 lower | 4 square epilogue |
+// 8 triangular operands.  n = 0 clears it.
+int gemm_set_tuning(const int* entries, int n);
+int gemm_tune_flags(const GemmArgs& g);
 int gemm_f64(const GemmArgs& g, hipStream_t s);
 int gemm_f32(const GemmArgs& g, hipStream_t s);
 // Lower-triangle SYRK with split inner dimension and FP64 reduction:

@@ -20,6 +20,9 @@
 // when the operands are 16-byte aligned); only edge tiles test bounds.
 // Rows gathered through an index list (Phi_mu, on either operand) are
 // resolved once per tile.
+#include <mutex>
+#include <vector>
+
 #include "common.h"
 
 namespace thc {
@@ -658,12 +661,58 @@ int launch_dtl(const GemmArgs& g, hipStream_t s) {
   return THC_OK;
 }
 
+// ---- tuning table (written by the autotuner, thcpp_amd/autotune.py) --------
+
+struct TuneEntry {
+  int mb, nb, kb, flags, cfg;
+};
+std::mutex tune_mu;
+std::vector<TuneEntry> tune_tab;
+
+int ceil_log2(long v) {
+  int b = 0;
+  while ((1L << b) < v) ++b;
+  return b;
+}
+
+int tuned_cfg(const GemmArgs& g) {
+  const int mb = ceil_log2(g.m), nb = ceil_log2(g.n), kb = ceil_log2(g.k), fl = gemm_tune_flags(g);
+  std::lock_guard<std::mutex> lock(tune_mu);
+  for (const TuneEntry& e : tune_tab)
+    if (e.mb == mb && e.nb == nb && e.kb == kb && e.flags == fl) return e.cfg;
+  return GEMM_CFG_AUTO;
+}
+
+// launch on configuration cfg; 1 when cfg does not apply to these arguments
+template <class T>
+int launch_cfg(const GemmArgs& g, int cfg, hipStream_t s) {
+  switch (cfg) {
+    case GEMM_CFG_TINY: return launch<T, CfgTiny>(g, s);
+    case GEMM_CFG_SMALL: return launch<T, CfgSmall>(g, s);
+    case GEMM_CFG_LARGE: return launch<T, CfgLarge>(g, s);
+    case GEMM_CFG_DTL:
+      if (sizeof(T) == 8 && dtl_fits(g)) return launch_dtl(g, s);
+      return 1;
+    case GEMM_CFG_DEEPK:
+      if (!g.lower) return launch<T, CfgDeepK>(g, s);
+      return 1;
+    default: return 1;
+  }
+}
+
 template <class T>
 int gemm_dispatch(const GemmArgs& g, hipStream_t s) {
   if (g.m <= 0 || g.n <= 0) return THC_OK;
   if (g.lower && g.m != g.n) return THC_ERR_ARG;
   if (g.c_f32 && (sizeof(T) != 8 || g.beta != 0.0 || g.epi_square || g.epi_hadamard || g.epi_addsquare))
     return THC_ERR_ARG;
+  if (g.cfg < 0 || g.cfg >= GEMM_CFG_COUNT) return THC_ERR_ARG;
+  const int cfg = g.cfg != GEMM_CFG_AUTO ? g.cfg : tuned_cfg(g);
+  if (cfg != GEMM_CFG_AUTO) {
+    const int r = launch_cfg<T>(g, cfg, s);
+    if (r != 1) return r;
+    if (g.cfg != GEMM_CFG_AUTO) return THC_ERR_UNSUPPORTED;  // a forced configuration that does not apply
+  }
   const long tiles128 = ((g.m + 127) / 128) * ((g.n + 127) / 128) * g.batch;
   long tiles64 = ((g.m + 63) / 64) * ((g.n + 63) / 64);
   if (g.lower) tiles64 = (tiles64 + (g.m + 63) / 64) / 2;
@@ -684,6 +733,23 @@ int gemm_dispatch(const GemmArgs& g, hipStream_t s) {
 constexpr int SY_BM = CfgLarge::BM, SY_BN = CfgLarge::BN, SY_BK = CfgLarge::BK;
 
 }  // namespace
+
+int gemm_tune_flags(const GemmArgs& g) {
+  return (g.a_rows || g.b_rows ? 1 : 0) | (g.lower ? 2 : 0) | (g.epi_square ? 4 : 0) | (g.tri ? 8 : 0);
+}
+
+int gemm_set_tuning(const int* entries, int n) {
+  if (n < 0 || (n > 0 && !entries)) return THC_ERR_ARG;
+  std::vector<TuneEntry> t(n);
+  for (int i = 0; i < n; ++i) {
+    const int* e = entries + 5 * i;
+    if (e[4] < GEMM_CFG_AUTO || e[4] >= GEMM_CFG_COUNT) return THC_ERR_ARG;
+    t[i] = TuneEntry{e[0], e[1], e[2], e[3], e[4]};
+  }
+  std::lock_guard<std::mutex> lock(tune_mu);
+  tune_tab.swap(t);
+  return THC_OK;
+}
 
 int gemm_f64(const GemmArgs& g, hipStream_t s) { return gemm_dispatch<double>(g, s); }
 int gemm_f32(const GemmArgs& g, hipStream_t s) { return gemm_dispatch<float>(g, s); }
