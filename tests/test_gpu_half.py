@@ -44,13 +44,18 @@ def test_czt_half(ctx, n, m, mo, k):
     assert rel(z.cpu(), IO.czt_half(a, occ, idx)) < 1e-14
 
 
-def test_isdf_half_all_occupied_equals_full(ctx, cfg0, cfg0_kmeans):
+@pytest.mark.parametrize("prec", [L.PREC_FP64, L.PREC_MIXED])
+def test_isdf_half_all_occupied_equals_full(ctx, cfg0, cfg0_kmeans, prec):
     """With every orbital occupied, Pocc o P = P o P: the half-rotated Muv is
-    V of the full path, bit for bit (same kernels, product order unchanged)."""
+    V of the full path, bit for bit (same kernels, product order unchanged).
+    Mixed: the full path writes the solve's ZC^T digits from the ZC^T GEMM's
+    epilogue, the half-rotated path splits the stored P o Pocc afterwards
+    (and its scale sqrt(rho rho) is rho exactly), so this also shows that the
+    two give the same digits."""
     s = cfg0
     idx = cfg0_kmeans["interp_idx"]
-    full = ctx.isdf(s.aoR, idx, s.n, s.coulG, want_luv=False)
-    half = ctx.isdf_half(s.aoR, s.aoR, idx, s.n, s.coulG)
+    full = ctx.isdf(s.aoR, idx, s.n, s.coulG, precision=prec, want_luv=False)
+    half = ctx.isdf_half(s.aoR, s.aoR, idx, s.n, s.coulG, precision=prec)
     assert full["status"] == L.THC_OK and half["status"] == L.THC_OK
     assert torch.equal(half["Muv"], full["V"])
 

@@ -271,6 +271,11 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   //    operands along k and keeps the 4 GB store row-contiguous
   //    Half-rotated (setup_CZt_half): Zt = P o Pocc, P stored first, then the
   //    occupied-orbital product multiplied into it in the epilogue.
+  //    Mixed path without refinement: nothing but the solve reads Zt, so the
+  //    GEMM's epilogue writes the solve's int8 digit planes of Zt directly
+  //    (the same values as splitting the FP64 Zt afterwards) and Zt itself is
+  //    never stored.
+  const bool fuse_split = prec == THC_PREC_MIXED && !aoR_half && !cplx && refine == 0;
   {
     GemmArgs g = {};
     g.m = n; g.n = k; g.k = m;
@@ -278,6 +283,10 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
     g.b = aoR; g.ldb = m; g.b_trans = 1; g.b_rows = idx; g.b_nrows = n;
     g.c = B.z; g.ldc = k; g.c_trans = 0;
     g.alpha = 1.0; g.beta = 0.0; g.epi_square = aoR_half ? 0 : 1; g.batch = 1;
+    if (fuse_split) {
+      THC_TRY(oz_solve_prepare(k, n, idx, aoR, m, B.solve_w, g, s));
+      g.epi_square = 0;
+    }
     THC_TRY(gemm_f64(g, s));
     if (aoR_half) {
       GemmArgs h = g;
@@ -305,7 +314,7 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   if (prec == THC_PREC_MIXED) {
     // exact int8 digit products of the scaled operands (ozaki.hip)
     THC_TRY(oz_solve(B.ci, k, B.z, k, n, idx, aoR, m, aoR_half, aoR_half ? m_occ : 0, B.x, x_f32, n, B.solve_w, s,
-                     ev[8], ev[9]));
+                     ev[8], ev[9], fuse_split ? 1 : 0));
   } else {
     GemmArgs g = {};
     g.m = k; g.n = n; g.k = k;

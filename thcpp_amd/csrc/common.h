@@ -110,6 +110,18 @@ struct GemmArgs {
   int tri;       // GEMM_TRI_* bits: known zero blocks of A / B that bound each tile's k range
   int c_f32;     // FP64 GEMM storing C as float (plain store, beta = 0)
   int cfg;       // GEMM_CFG_*: force a tile configuration (0: the tuning table, then the built-in rule)
+  // digit epilogue of the mixed solve (FP64 only, B rows gathered, both
+  // operands k-contiguous, 128 x 128 x 32 tiles): instead of C, write the 7
+  // int8 digit planes of x = (acc^2 * dig_inv2[col]) / dig_s[row] into
+  // dig_out, k-blocked (byte (row, col) of plane q at q * dig_nr * dig_kp +
+  // (col / 32) * dig_nr * 32 + row * 32 + col % 32), zero for rows >= m or
+  // cols >= n, rows < dig_nr and cols < dig_kp
+  int epi_digits;
+  signed char* dig_out;
+  const double* dig_s;
+  const double* dig_inv2;
+  long dig_nr;
+  int dig_kp;
   int batch;
   long stride_a, stride_b, stride_c;
 };
@@ -180,7 +192,13 @@ int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ld
 size_t oz_solve_work_bytes(int k, long n);
 int oz_solve(const double* ainv, long lda, const double* zt, int k, long n, const int* idx, const double* phi_a,
              int ma, const double* phi_b, int mb, void* x, int x_f32, long ldx, void* work, hipStream_t s,
-             hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+             hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, int zt_split = 0);
+// With the ZC^T digits written by the ZC^T GEMM's digit epilogue: oz_solve_prepare
+// computes s and the point scales into the solve workspace and fills the
+// epilogue fields of g (dig_*); oz_solve with zt_split = 1 then skips them and
+// the split of Zt (zt unused).
+int oz_solve_prepare(int k, long n, const int* idx, const double* phi_a, int ma, void* work, GemmArgs& g,
+                     hipStream_t s);
 
 // ---- QR with column pivoting (qrcp.hip) ---------------------------------------
 // First k pivots of QRCP of the orbital-product matrix (occ = nullptr: all

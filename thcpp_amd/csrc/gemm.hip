@@ -21,6 +21,7 @@
 // Rows gathered through an index list (Phi_mu, on either operand) are
 // resolved once per tile.
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -47,7 +48,8 @@ template <> struct Mfma<float> {
   __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-enum { EPI_STORE = 0, EPI_SQUARE = 1, EPI_PARTIAL = 2, EPI_HADAMARD = 3, EPI_ADDSQUARE = 4 };
+enum { EPI_STORE = 0, EPI_SQUARE = 1, EPI_PARTIAL = 2, EPI_HADAMARD = 3, EPI_ADDSQUARE = 4, EPI_DIGITS = 5 };
+constexpr int DIG_PLANES = 7;  // digits of the mixed solve's ZC^T operand (ozaki.hip)
 
 struct KArgs {
   long m, n, k;
@@ -60,6 +62,11 @@ struct KArgs {
   int c_f32;
   long stride_a, stride_b, stride_c;
   long kchunk;  // split-K chunk (EPI_PARTIAL)
+  signed char* dig_out;  // EPI_DIGITS (see GemmArgs)
+  const double* dig_s;
+  const double* dig_inv2;
+  long dig_nr;
+  int dig_kp;
 };
 
 __device__ inline bool tile_coords(const KArgs& p, int BM, int BN, int& tm, int& tn) {
@@ -151,8 +158,8 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs p) {
   static_assert(WM * WTM == BM && WN * WTN == BN, "wave grid must cover the tile");
   constexpr int A_LINES = AK ? BM : BK, A_LEN = AK ? BK + 1 : BM + 1;
   constexpr int B_LINES = BK_ ? BN : BK, B_LEN = BK_ ? BK + 1 : BN + 1;
-  __shared__ T As[2][A_LINES * A_LEN];
-  __shared__ T Bs[2][B_LINES * B_LEN];
+  __shared__ __attribute__((aligned(16))) T As[2][A_LINES * A_LEN];
+  __shared__ __attribute__((aligned(16))) T Bs[2][B_LINES * B_LEN];
   typedef typename Mfma<T>::acc_t acc_t;
   typedef TileMap<T, BM, BK, AK, NT> MA;
   typedef TileMap<T, BN, BK, BK_, NT> MB;
@@ -311,6 +318,68 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(KArgs p) {
     }
   }
 
+  if constexpr (EPI == EPI_DIGITS) {
+    // x = (acc^2 * inv2[col]) / s[row] cut into DIG_PLANES digits (x *= 128,
+    // d = trunc(x), x -= d: exact in FP64), the same arithmetic as the
+    // standalone split of ZC^T.  All planes of the tile are staged in the
+    // free operand LDS (planes 0-3 in As, 4-6 in Bs) as [32-column
+    // block][row][32 bytes], then, after one barrier, stored as whole 16-byte
+    // pieces of the k-blocked planes: a 4 KB contiguous run per block.
+    static_assert(sizeof(T) == 8 && BM == 128 && BN == 128 && DIG_PLANES == 7, "digit epilogue: FP64 128 x 128");
+    static_assert(sizeof(As) >= 4 * BM * BN && sizeof(Bs) >= 3 * BM * BN, "seven staged planes fit the operand LDS");
+    signed char* stg_a = reinterpret_cast<signed char*>(&As[0][0]);
+    signed char* stg_b = reinterpret_cast<signed char*>(&Bs[0][0]);
+    double x[TM][TN][4];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const long r = m0 + wm * WTM + i * 16 + Mfma<T>::row(lane, v);
+          const long cc = n0 + wn * WTN + j * 16 + l16;
+          double xv = 0.0;
+          if (r < p.m && cc < p.n) {
+            const double sr = p.dig_s[r];
+            const double isr = sr > 0.0 ? 1.0 / sr : 0.0;
+            const double a = p.alpha * acc[i][j][v];
+            xv = a * a * p.dig_inv2[cc] * isr;
+          }
+          x[i][j][v] = xv;
+        }
+    __syncthreads();  // every wave is past its last fragment read of As / Bs
+#pragma unroll
+    for (int q = 0; q < DIG_PLANES; ++q) {
+      signed char* stg = q < 4 ? stg_a + q * (BM * BN) : stg_b + (q - 4) * (BM * BN);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int rl = wm * WTM + i * 16 + Mfma<T>::row(lane, v);
+            const int cl = wn * WTN + j * 16 + l16;
+            x[i][j][v] *= 128.0;
+            const double d = trunc(x[i][j][v]);
+            x[i][j][v] -= d;
+            stg[(cl >> 5) * (BM * 32) + rl * 32 + (cl & 31)] = (signed char)(int)d;
+          }
+    }
+    __syncthreads();
+    const long plane = p.dig_nr * (long)p.dig_kp;
+#pragma unroll
+    for (int q = 0; q < DIG_PLANES; ++q) {
+      const signed char* stg = q < 4 ? stg_a + q * (BM * BN) : stg_b + (q - 4) * (BM * BN);
+      for (int t = tid; t < BM * BN / 16; t += NT) {
+        const int cb = t / (BM * 2), rem = t % (BM * 2), rl = rem >> 1, h = rem & 1;
+        const long gr = m0 + rl, gc = n0 + cb * 32;
+        if (gr < p.dig_nr && gc < p.dig_kp)
+          *reinterpret_cast<int4*>(p.dig_out + q * plane + (gc >> 5) * p.dig_nr * 32 + gr * 32 + h * 16) =
+              *reinterpret_cast<const int4*>(stg + cb * (BM * 32) + rl * 32 + h * 16);
+      }
+    }
+    return;
+  }
   if (EPI == EPI_PARTIAL) {
     const long tile_lin = (long)tm * (tm + 1) / 2 + tn;
     const long ntiles = gridDim.x;
@@ -561,6 +630,7 @@ KArgs make_args(const GemmArgs& g) {
   p.tri = g.tri;
   p.c_f32 = g.c_f32;
   p.stride_a = g.stride_a; p.stride_b = g.stride_b; p.stride_c = g.stride_c;
+  p.dig_out = g.dig_out; p.dig_s = g.dig_s; p.dig_inv2 = g.dig_inv2; p.dig_nr = g.dig_nr; p.dig_kp = g.dig_kp;
   return p;
 }
 
@@ -614,6 +684,26 @@ int launch_layout(const GemmArgs& g, const KArgs& p, dim3 grid, hipStream_t s) {
 template <class T, class C>
 int launch(const GemmArgs& g, hipStream_t s) {
   const KArgs p = make_args(g);
+  if (g.epi_digits) {
+    // B rows gathered, both operands k-contiguous, the deep-k tile (its
+    // operand LDS holds one staged digit plane)
+    if constexpr (std::is_same<T, double>::value && std::is_same<C, CfgDeepK>::value) {
+      if (g.lower || g.tri || g.a_rows || !g.b_rows || g.a_trans != 0 || g.b_trans != 1 || g.batch != 1 ||
+          g.epi_square || g.epi_hadamard || g.epi_addsquare || !g.dig_out || !g.dig_s || !g.dig_inv2 ||
+          g.dig_kp % 32 != 0 || g.dig_nr % 64 != 0)
+        return THC_ERR_ARG;
+      const long nblk = ((g.m + C::BM - 1) / C::BM) * ((g.n + C::BN - 1) / C::BN);
+      if (nblk <= 0) return THC_OK;
+      if (nblk > 0x7fffffffL) return THC_ERR_ARG;
+      const int vw = 16 / (int)sizeof(T);
+      const bool vec = aligned16(g.a) && aligned16(g.b) && g.lda % vw == 0 && g.ldb % vw == 0;
+      launch_vec<T, C, EPI_DIGITS, true, true, false, true>(p, vec, dim3((unsigned)nblk), s);
+      THC_HIP(hipGetLastError());
+      return THC_OK;
+    } else {
+      return THC_ERR_ARG;
+    }
+  }
   const long nbm = (g.m + C::BM - 1) / C::BM, nbn = (g.n + C::BN - 1) / C::BN;
   const long nblk = g.lower ? nbm * (nbm + 1) / 2 : nbm * nbn;
   if (nblk <= 0 || g.batch <= 0) return THC_OK;
@@ -707,6 +797,7 @@ int gemm_dispatch(const GemmArgs& g, hipStream_t s) {
   if (g.c_f32 && (sizeof(T) != 8 || g.beta != 0.0 || g.epi_square || g.epi_hadamard || g.epi_addsquare))
     return THC_ERR_ARG;
   if (g.cfg < 0 || g.cfg >= GEMM_CFG_COUNT) return THC_ERR_ARG;
+  if (g.epi_digits) return launch<T, CfgDeepK>(g, s);
   const int cfg = g.cfg != GEMM_CFG_AUTO ? g.cfg : tuned_cfg(g);
   if (cfg != GEMM_CFG_AUTO) {
     const int r = launch_cfg<T>(g, cfg, s);

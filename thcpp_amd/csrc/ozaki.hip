@@ -662,6 +662,37 @@ SvPlan sv_plan(int k, long n) {
   return p;
 }
 
+// the solve workspace: digit planes of A' and Zs, A' row exponents, s, inv2
+struct SvWork {
+  signed char *SA, *SZ;
+  int* EA;
+  double *sc, *inv2;
+};
+SvWork sv_carve(const SvPlan& P, long n, void* work) {
+  SvWork W;
+  char* w = reinterpret_cast<char*>(work);
+  W.SA = reinterpret_cast<signed char*>(w);
+  w += align_up((size_t)SV_D * P.kr * P.kp, 256);
+  W.SZ = reinterpret_cast<signed char*>(w);
+  w += align_up((size_t)SV_D * P.nr * P.kp, 256);
+  W.EA = reinterpret_cast<int*>(w);
+  w += align_up(sizeof(int) * (size_t)P.kr, 256);
+  W.sc = reinterpret_cast<double*>(w);
+  w += align_up(sizeof(double) * (size_t)n, 256);
+  W.inv2 = reinterpret_cast<double*>(w);
+  return W;
+}
+
+// s(r) and the point scales inv2
+int sv_scales(const SvPlan& P, const SvWork& W, int k, long n, const int* idx, const double* phi_a, int ma,
+              const double* phi_b, int mb, hipStream_t s) {
+  const int gs = (int)std::min<long>(8192, (n + 255) / 256);
+  hipLaunchKernelGGL(sv_scale, dim3(gs), dim3(256), 0, s, phi_a, ma, phi_b, mb, n, W.sc);
+  hipLaunchKernelGGL(sv_point_scale, dim3((P.kp + 255) / 256), dim3(256), 0, s, W.sc, n, idx, k, P.kp, W.inv2);
+  THC_HIP(hipGetLastError());
+  return THC_OK;
+}
+
 }  // namespace
 
 size_t oz_solve_work_bytes(int k, long n) {
@@ -671,31 +702,39 @@ size_t oz_solve_work_bytes(int k, long n) {
          align_up(sizeof(double) * (size_t)P.kp, 256) + 1024;
 }
 
-int oz_solve(const double* ainv, long lda, const double* zt, int k, long n, const int* idx, const double* phi_a,
-             int ma, const double* phi_b, int mb, void* x, int x_f32, long ldx, void* work, hipStream_t s,
-             hipEvent_t e0, hipEvent_t e1) {
+int oz_solve_prepare(int k, long n, const int* idx, const double* phi_a, int ma, void* work, GemmArgs& g,
+                     hipStream_t s) {
   if (k <= 0 || n <= 0) return THC_OK;
   const SvPlan P = sv_plan(k, n);
-  char* w = reinterpret_cast<char*>(work);
-  signed char* SA = reinterpret_cast<signed char*>(w);
-  w += align_up((size_t)SV_D * P.kr * P.kp, 256);
-  signed char* SZ = reinterpret_cast<signed char*>(w);
-  w += align_up((size_t)SV_D * P.nr * P.kp, 256);
-  int* EA = reinterpret_cast<int*>(w);
-  w += align_up(sizeof(int) * (size_t)P.kr, 256);
-  double* sc = reinterpret_cast<double*>(w);
-  w += align_up(sizeof(double) * (size_t)n, 256);
-  double* inv2 = reinterpret_cast<double*>(w);
-  const int gs = (int)std::min<long>(8192, (n + 255) / 256);
-  hipLaunchKernelGGL(sv_scale, dim3(gs), dim3(256), 0, s, phi_a, ma, phi_b, mb, n, sc);
-  THC_HIP(hipGetLastError());
+  const SvWork W = sv_carve(P, n, work);
+  THC_TRY(sv_scales(P, W, k, n, idx, phi_a, ma, nullptr, 0, s));
+  g.epi_digits = 1;
+  g.dig_out = W.SZ;
+  g.dig_s = W.sc;
+  g.dig_inv2 = W.inv2;
+  g.dig_nr = P.nr;
+  g.dig_kp = P.kp;
+  return THC_OK;
+}
+
+namespace {
+int oz_solve_impl(const double* ainv, long lda, const double* zt, int k, long n, const int* idx,
+                  const double* phi_a, int ma, const double* phi_b, int mb, void* x, int x_f32, long ldx,
+                  void* work, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int zt_split) {
+  const SvPlan P = sv_plan(k, n);
+  const SvWork W = sv_carve(P, n, work);
+  signed char* SA = W.SA;
+  signed char* SZ = W.SZ;
+  int* EA = W.EA;
+  double* sc = W.sc;
+  if (!zt_split) THC_TRY(sv_scales(P, W, k, n, idx, phi_a, ma, phi_b, mb, s));
   hipLaunchKernelGGL(sv_split_rows, dim3(P.kr), dim3(256), 0, s, ainv, lda, k, P.kr, P.kp, sc, n, idx, SA, EA);
   THC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(sv_point_scale, dim3((P.kp + 255) / 256), dim3(256), 0, s, sc, n, idx, k, P.kp, inv2);
-  THC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(sv_split_zt, dim3((unsigned)(P.nr / 64), (unsigned)(P.kp / 32)), dim3(256), 0, s, zt, n, k, P.nr,
-                     P.kp, sc, inv2, SZ);
-  THC_HIP(hipGetLastError());
+  if (!zt_split) {
+    hipLaunchKernelGGL(sv_split_zt, dim3((unsigned)(P.nr / 64), (unsigned)(P.kp / 32)), dim3(256), 0, s, zt, n, k,
+                       P.nr, P.kp, sc, W.inv2, SZ);
+    THC_HIP(hipGetLastError());
+  }
   const long blocks = (P.ngroups + 7) / 8 * 256;  // every group gets its 32 workgroups on one XCD
   if (blocks > 0x7fffffffL) return THC_ERR_ARG;
   if (e0) THC_HIP(hipEventRecord(e0, s));
@@ -708,6 +747,14 @@ int oz_solve(const double* ainv, long lda, const double* zt, int k, long n, cons
   THC_HIP(hipGetLastError());
   if (e1) THC_HIP(hipEventRecord(e1, s));
   return THC_OK;
+}
+}  // namespace
+
+int oz_solve(const double* ainv, long lda, const double* zt, int k, long n, const int* idx, const double* phi_a,
+             int ma, const double* phi_b, int mb, void* x, int x_f32, long ldx, void* work, hipStream_t s,
+             hipEvent_t e0, hipEvent_t e1, int zt_split) {
+  if (k <= 0 || n <= 0) return THC_OK;
+  return oz_solve_impl(ainv, lda, zt, k, n, idx, phi_a, ma, phi_b, mb, x, x_f32, ldx, work, s, e0, e1, zt_split);
 }
 
 }  // namespace thc
