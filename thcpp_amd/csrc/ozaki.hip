@@ -1,7 +1,9 @@
 // V = B B^T for the mixed-precision path, evaluated exactly in int8 MFMA
 // arithmetic on splits of B (an Ozaki-style emulation of the FP64 Gram).
 //
-// Split: every row of B is cut into chunks of OZ_CH columns; each (row, chunk)
+// Split: every row of B is cut into chunks of ch columns (8192, halved down to
+// 1024 while the lower tiles times chunks give fewer than 2048 workgroups, so
+// that small systems still spread over the CUs); each (row, chunk)
 // gets a power-of-two exponent e with max |b| < 2^e, and b 2^-e in (-1, 1) is
 // written as OZ_S signed 7-bit digits, b = 2^e sum_p d_p 2^-7(p+1) + r,
 // |r| < 2^(e - 7 OZ_S).  Every step (scale by 128, truncate, subtract) is
@@ -10,7 +12,7 @@
 // 2^-7(p+q+2) sum_g d_p,rg d_q,cg.  The inner sums are int8 x int8 products
 // accumulated in int32 by v_mfma_i32_16x16x64_i8, grouped by p + q (products
 // with the same p + q share a scale), exact as long as
-// OZ_S * OZ_CH * 127^2 < 2^31.  Digit products with p + q >= OZ_S are
+// OZ_S * ch * 127^2 < 2^31.  Digit products with p + q >= OZ_S are
 // dropped; they are below 2^-7(OZ_S+1) of |b_r| |b_c| per term.
 // Each workgroup takes one 128 x 128 lower tile and one chunk, converts its
 // groups to double, scales by 2^(e_r + e_c) and writes a partial tile; the
@@ -31,7 +33,8 @@ namespace {
 typedef int i4 __attribute__((ext_vector_type(4)));
 
 constexpr int OZ_S = 5;       // digits per value
-constexpr int OZ_CH = 8192;   // columns per chunk (one exponent per row and chunk)
+constexpr int OZ_CH = 8192;   // largest chunk (one exponent per row and chunk)
+constexpr int OZ_CH_MIN = 1024;  // smallest: one 4-column run per thread of oz_split
 constexpr int OZ_T = 128;     // tile edge
 constexpr int OZ_KS = 64;     // k per MFMA step
 static_assert((long)OZ_S * OZ_CH * 127 * 127 < 2147483647L, "int32 group sums must not overflow");
@@ -69,14 +72,13 @@ __device__ inline void load4(const double* p, bool vec, long g, long g2, double 
 // digits S[p][row][g] (g < g2p, zero padded) and exponents E[row][chunk].
 // Thread t takes runs of 4 consecutive columns (one vector load), and stores
 // each digit plane's 4 bytes as one word.
-template <class T>
+template <class T, int GRP>  // GRP = ch / 1024 runs of 4 columns per thread
 __global__ __launch_bounds__(256) void oz_split(const T* __restrict__ b, long ld, long g2, long g2p, int k,
                                                 int nchunk, signed char* __restrict__ S, int* __restrict__ E) {
   __shared__ double red[4];
   const int c = blockIdx.x, row = blockIdx.y;
   const T* src = b + (long)row * ld;
-  const long g0 = (long)c * OZ_CH;
-  constexpr int GRP = OZ_CH / (256 * 4);
+  const long g0 = (long)c * (GRP * 1024);
   const bool vec = ((reinterpret_cast<uintptr_t>(src) & 15) == 0);
   double v[GRP][4];
   double mx = 0.0;
@@ -123,8 +125,8 @@ typedef __attribute__((address_space(3))) void lds_void;
 __device__ inline int oz_seg(int r, int s) { return (s + (r >> 2)) & 3; }
 
 __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S, long g2, long g2p, int k,
-                                               const int* __restrict__ E, int nchunk, double* __restrict__ work,
-                                               long ntiles) {
+                                               const int* __restrict__ E, int nchunk, int ch,
+                                               double* __restrict__ work, long ntiles) {
   // one variable per buffer: the compiler then knows that the direct loads
   // into one buffer do not alias the fragment reads of the other, and does not
   // wait for the loads before reading
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S
   static_assert(OZ_T == 16 * 8, "one 16-row block of every plane per wave");
   const int r_ld = 16 * wave + (lane >> 2);
   const int ls = ((lane & 3) - (r_ld >> 2)) & 3;  // logical segment stored at physical segment L % 4
-  const long plane = (long)k * g2p, kbase = (long)c * OZ_CH;
+  const long plane = (long)k * g2p, kbase = (long)c * ch;
   const signed char* sa = S + (long)min(tm * OZ_T + r_ld, k - 1) * g2p + kbase + ls * 16;
   const signed char* sb = S + (long)min(tn * OZ_T + r_ld, k - 1) * g2p + kbase + ls * 16;
   auto issue = [&](int ks, signed char* as, signed char* bs) {
@@ -176,7 +178,7 @@ __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S
     }
   };
   // k-steps of this chunk that hold columns < g2 (the rest are zero padding)
-  const int nks = (int)min((long)(OZ_CH / OZ_KS), (g2 - kbase + OZ_KS - 1) / OZ_KS);
+  const int nks = (int)min((long)(ch / OZ_KS), (g2 - kbase + OZ_KS - 1) / OZ_KS);
   // one k-step: fragments from (as, bs) while the next slabs stream into (na, nb)
   auto step = [&](int ks, const signed char* as, const signed char* bs, signed char* na, signed char* nb) {
     if (ks + 1 < nks) issue(ks + 1, na, nb);
@@ -294,19 +296,21 @@ __global__ __launch_bounds__(256) void oz_reduce(const double* __restrict__ work
     if (mr < k && mc + u < k && mr < mc + u) v[mr * ldv + mc + u] = S[c0 + u][r];
 }
 
-void oz_plan(int k, long g2, int& nchunk, long& g2p, long& ntiles) {
-  nchunk = (int)((g2 + OZ_CH - 1) / OZ_CH);
-  g2p = (long)nchunk * OZ_CH;
+void oz_plan(int k, long g2, int& ch, int& nchunk, long& g2p, long& ntiles) {
   const long nb = (k + OZ_T - 1) / OZ_T;
   ntiles = nb * (nb + 1) / 2;
+  ch = OZ_CH;
+  while (ch > OZ_CH_MIN && ntiles * ((g2 + ch - 1) / ch) < 2048) ch /= 2;
+  nchunk = (int)((g2 + ch - 1) / ch);
+  g2p = (long)nchunk * ch;
 }
 
 }  // namespace
 
 size_t oz_syrk_work_bytes(int k, long g2) {
-  int nchunk;
+  int ch, nchunk;
   long g2p, ntiles;
-  oz_plan(k, g2, nchunk, g2p, ntiles);
+  oz_plan(k, g2, ch, nchunk, g2p, ntiles);
   return align_up((size_t)OZ_S * k * g2p, 256) + align_up(sizeof(int) * (size_t)k * nchunk, 256) +
          align_up(sizeof(double) * (size_t)nchunk * ntiles * OZ_T * OZ_T, 256) + 256;
 }
@@ -314,26 +318,33 @@ size_t oz_syrk_work_bytes(int k, long g2) {
 int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s,
             hipEvent_t e0, hipEvent_t e1) {
   if (k <= 0) return THC_OK;
-  int nchunk;
+  int ch, nchunk;
   long g2p, ntiles;
-  oz_plan(k, g2, nchunk, g2p, ntiles);
+  oz_plan(k, g2, ch, nchunk, g2p, ntiles);
   char* w = reinterpret_cast<char*>(work);
   signed char* S = reinterpret_cast<signed char*>(w);
   w += align_up((size_t)OZ_S * k * g2p, 256);
   int* E = reinterpret_cast<int*>(w);
   w += align_up(sizeof(int) * (size_t)k * nchunk, 256);
   double* part = reinterpret_cast<double*>(w);
-  if (prec == 0)
-    hipLaunchKernelGGL(oz_split<double>, dim3(nchunk, k), dim3(256), 0, s, (const double*)b, ld, g2, g2p, k, nchunk,
-                       S, E);
-  else
-    hipLaunchKernelGGL(oz_split<float>, dim3(nchunk, k), dim3(256), 0, s, (const float*)b, ld, g2, g2p, k, nchunk,
-                       S, E);
+  const dim3 sg(nchunk, k);
+  auto split = [&](auto tag) {
+    typedef decltype(tag) T;
+    const T* bt = reinterpret_cast<const T*>(b);
+    switch (ch / OZ_CH_MIN) {
+      case 1: hipLaunchKernelGGL((oz_split<T, 1>), sg, dim3(256), 0, s, bt, ld, g2, g2p, k, nchunk, S, E); break;
+      case 2: hipLaunchKernelGGL((oz_split<T, 2>), sg, dim3(256), 0, s, bt, ld, g2, g2p, k, nchunk, S, E); break;
+      case 4: hipLaunchKernelGGL((oz_split<T, 4>), sg, dim3(256), 0, s, bt, ld, g2, g2p, k, nchunk, S, E); break;
+      default: hipLaunchKernelGGL((oz_split<T, 8>), sg, dim3(256), 0, s, bt, ld, g2, g2p, k, nchunk, S, E); break;
+    }
+  };
+  if (prec == 0) split(double());
+  else split(float());
   THC_HIP(hipGetLastError());
   const long slots = (ntiles * nchunk + 7) / 8 * 8;  // every w < ntiles * nchunk gets one workgroup
   if (slots > 0x7fffffffL) return THC_ERR_ARG;
   if (e0) THC_HIP(hipEventRecord(e0, s));
-  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, S, g2, g2p, k, E, nchunk, part, ntiles);
+  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, S, g2, g2p, k, E, nchunk, ch, part, ntiles);
   THC_HIP(hipGetLastError());
   if (e1) THC_HIP(hipEventRecord(e1, s));
   hipLaunchKernelGGL(oz_reduce, dim3((unsigned)(ntiles * (OZ_T / RB) * (OZ_T / RB))), dim3(256), 0, s, part, nchunk,
