@@ -391,6 +391,7 @@ namespace {
 // ============================================================================
 
 constexpr int SV_D = 7;
+constexpr int SV_GM = 4, SV_GN = 8;  // an XCD's 32 workgroups: SV_GM mu-tiles x SV_GN r-tiles
 constexpr int SV_TM = 128, SV_TN = 64, SV_KS = 32;
 constexpr int SV_ABLK = SV_D * SV_TM * SV_KS / 1024;  // 1 KB blocks of 32 rows x 32 bytes
 constexpr int SV_BBLK = SV_D * SV_TN * SV_KS / 1024;
@@ -591,15 +592,15 @@ __global__ __launch_bounds__(512) void sv_gemm(const signed char* __restrict__ S
                                               int mt, long nt, long ngroups) {
   __shared__ __attribute__((aligned(16))) signed char L0[SV_NBLK * 1024], L1[SV_NBLK * 1024],
       L2[SV_NBLK * 1024];
-  // XCD-aware order: groups of 4 mu-tiles x 8 r-tiles (32 workgroups, one per CU
+  // XCD-aware order: groups of SV_GM mu-tiles x SV_GN r-tiles (32 workgroups, one per CU
   // of an XCD) are dealt to the XCDs in turn; workgroup i runs on XCD i % 8
   const long b = blockIdx.x;
   const long g = (b / 256) * 8 + (b % 8);  // group
   const int w = (int)((b / 8) % 32);       // member of the group
   if (g >= ngroups) return;
-  const long ngm = (mt + 3) / 4;
-  const long tm = (g % ngm) * 4 + (w % 4);
-  const long tn = (g / ngm) * 8 + (w / 4);
+  const long ngm = (mt + SV_GM - 1) / SV_GM;
+  const long tm = (g % ngm) * SV_GM + (w % SV_GM);
+  const long tn = (g / ngm) * SV_GN + (w / SV_GM);
   if (tm >= mt || tn >= nt) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 3, wn = wave >> 2;
@@ -669,7 +670,7 @@ SvPlan sv_plan(int k, long n) {
   p.kr = p.mt * SV_TM;
   p.nt = (n + SV_TN - 1) / SV_TN;
   p.nr = p.nt * SV_TN;
-  p.ngroups = (long)((p.mt + 3) / 4) * ((p.nt + 7) / 8);
+  p.ngroups = (long)((p.mt + SV_GM - 1) / SV_GM) * ((p.nt + SV_GN - 1) / SV_GN);
   return p;
 }
 
