@@ -131,10 +131,12 @@ size_t isdf_bytes(long n, int k, int m, int m_occ, bool cplx, const int fft[3], 
   if (own_x) b += carve_bytes<double>((size_t)k * n);        // x
   b += 4 * carve_bytes<double>(kk);                          // c, l, ci, v
   if (refine > 0) b += carve_bytes<double>((size_t)k * n);   // r
-  b += prec == 0 ? carve_bytes<double>((size_t)k * g2) : carve_bytes<float>((size_t)k * g2);
+  // the weighted spectrum (the mixed path's FFT writes V's digit planes instead)
+  if (prec == 0) b += carve_bytes<double>((size_t)k * g2);
+  else if (prec != THC_PREC_MIXED) b += carve_bytes<float>((size_t)k * g2);
   const int fprec = prec == 0 ? 0 : 1;  // FFT / spectrum precision
   b += chol_work_bytes(k) + 256 + spd_inverse_work_bytes(k) + 256 + fft_work_bytes(k, fft, fprec) + 256 +
-       (prec == THC_PREC_MIXED ? oz_syrk_work_bytes(k, g2) : syrk_work_bytes(k, g2, fprec)) + 256;
+       (prec == THC_PREC_MIXED ? oz_lines_work_bytes(k, fft) : syrk_work_bytes(k, g2, fprec)) + 256;
   if (prec == THC_PREC_MIXED) b += oz_solve_work_bytes(k, n) + 256;
   if (rank) b += tridiag_work_bytes(k) + 256;
   b += carve_bytes<int>(16);
@@ -158,16 +160,17 @@ bool isdf_carve(Arena& a, long n, int k, int m, int m_occ, bool cplx, const int 
   B.v = a.take<double>(kk);
   B.r = refine > 0 ? a.take<double>((size_t)k * n) : nullptr;
   if (prec == 0) B.b = a.take<double>((size_t)k * g2);
-  else B.b = a.take<float>((size_t)k * g2);
+  else if (prec != THC_PREC_MIXED) B.b = a.take<float>((size_t)k * g2);
   B.chol_w = a.take<char>(chol_work_bytes(k));
   B.inv_w = a.take<char>(spd_inverse_work_bytes(k));
   const int fprec = prec == 0 ? 0 : 1;
   B.fft_w = a.take<char>(fft_work_bytes(k, fft, fprec));
-  B.syrk_w = a.take<char>(prec == THC_PREC_MIXED ? oz_syrk_work_bytes(k, g2) : syrk_work_bytes(k, g2, fprec));
+  B.syrk_w = a.take<char>(prec == THC_PREC_MIXED ? oz_lines_work_bytes(k, fft) : syrk_work_bytes(k, g2, fprec));
   B.solve_w = prec == THC_PREC_MIXED ? a.take<char>(oz_solve_work_bytes(k, n)) : nullptr;
   B.rank_w = rank ? a.take<char>(tridiag_work_bytes(k)) : nullptr;
   B.info = a.take<int>(16);
-  return B.phim && B.phio && B.z && B.x && B.c && B.l && B.ci && B.v && (refine <= 0 || B.r) && B.b &&
+  return B.phim && B.phio && B.z && B.x && B.c && B.l && B.ci && B.v && (refine <= 0 || B.r) &&
+         (prec == THC_PREC_MIXED || B.b) &&
          B.chol_w && B.inv_w && B.fft_w && B.syrk_w && (prec != THC_PREC_MIXED || B.solve_w) &&
          (!rank || B.rank_w) && B.info;
 }
@@ -349,11 +352,15 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   }
   THC_HIP(hipEventRecord(ev[4], s));
   // 4. weighted half spectrum
-  THC_TRY(fft_coulomb(B.x, x_f32, k, fft, coulG, B.b, prec == THC_PREC_FP64 ? 0 : 1, B.fft_w, s, ev + 12));
+  //    (mixed: pass B writes V's int8 digit planes, one exponent per y-line)
+  FftDigits dig = {};
+  if (prec == THC_PREC_MIXED) dig = oz_lines_layout(k, fft, B.syrk_w);
+  THC_TRY(fft_coulomb(B.x, x_f32, k, fft, coulG, B.b, prec == THC_PREC_FP64 ? 0 : 1, B.fft_w, s, ev + 12,
+                      prec == THC_PREC_MIXED ? &dig : nullptr));
   THC_HIP(hipEventRecord(ev[5], s));
   // 5. V = B B^T, FP64 sum of the split-K partials (exact int8 digit products on the mixed path)
   if (prec == THC_PREC_MIXED) {
-    THC_TRY(oz_syrk(B.b, k, g2, g2, 1, v_out, k, B.syrk_w, s, ev[10], ev[11]));
+    THC_TRY(oz_syrk_lines(dig, k, fft, v_out, k, B.syrk_w, s, ev[10], ev[11]));
   } else {
     THC_TRY(syrk_splitk(B.b, k, g2, g2, prec == THC_PREC_FP64 ? 0 : 1, v_out, k, B.syrk_w, s, ev[10], ev[11]));
   }

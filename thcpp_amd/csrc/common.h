@@ -179,12 +179,29 @@ void tridiag_rank(const double* d, const double* e, int k, int* rank, double* sm
 int zero_upper(double* a, int k, long lda, hipStream_t s);
 int symmetrize_from_lower(double* a, int k, long lda, hipStream_t s);
 
-// ---- int8-emulated Gram (ozaki.hip) ----------------------------------------
-// V = B B^T (B: k x g2, f64 if prec 0, f32 if prec 1, row stride ld) from
-// exact int8 digit products, written full symmetric into v.
-size_t oz_syrk_work_bytes(int k, long g2);
-int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s,
-            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+constexpr int OZ_DIGITS = 5;  // digits per value of the mixed V (ozaki.hip, FFT pass B)
+// Mixed-path output of the FFT: instead of the weighted half spectrum, its
+// OZ_DIGITS int8 digit planes for the V contraction.  Each (row mu, y-line)
+// of pass B is one chunk: one power-of-two exponent e[mu * n1 + iy] (the
+// line's max |value| < 2^e) and columns iy * lp + 2 j + (0 re, 1 im) for
+// the j-th complex value of the line (j = kx * n2h + kz), zero up to lp; byte
+// (row, col) of plane q at q * k * g2p + row * g2p + col, g2p = n1 * lp.
+struct FftDigits {
+  signed char* s;
+  int* e;
+  long g2p;
+  int lp;
+};
+
+// ---- int8-emulated Gram and solve (ozaki.hip) ---------------------------------
+// V = B B^T from the digit planes the FFT's pass B writes (FftDigits, one
+// chunk per y-line), by exact int8 digit products, written full symmetric
+// into v; oz_lines_layout carves the planes and the partial tiles from 'work'.
+int oz_line_len(const int n[3]);
+size_t oz_lines_work_bytes(int k, const int n[3]);
+FftDigits oz_lines_layout(int k, const int n[3], void* work);
+int oz_syrk_lines(const FftDigits& d, int k, const int n[3], double* v, long ldv, void* work, hipStream_t s,
+                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // X = ainv * Z (k x n) from Zt (n x k row-major, ZC^T transposed) by exact
 // int8 digit products of the scaled operands ainv diag(s(idx)) and
 // Zt / (s(r) s(idx)), s(r) = sqrt(sum phi_a(r)^2 sum phi_b(r)^2) (phi_b null:
@@ -215,7 +232,7 @@ size_t fft_work_bytes(int k, const int n[3], int prec);
 // pass_ev (optional, 3 events): recorded before the first pass, between the
 // passes and after the second
 int fft_coulomb(const void* x, int x_f32, int k, const int n[3], const double* coulG, void* out, int prec,
-                void* work, hipStream_t s, hipEvent_t* pass_ev = nullptr);
+                void* work, hipStream_t s, hipEvent_t* pass_ev = nullptr, const FftDigits* dig = nullptr);
 inline long half_spectrum_len(const int n[3]) { return 2L * n[0] * n[1] * (n[2] / 2 + 1); }
 
 }  // namespace thc

@@ -14,7 +14,10 @@
 // Pass A, one workgroup per (mu, x-plane): two real z-rows are packed as one
 // complex row, transformed along z, unpacked into the half spectrum, then
 // transformed along y.  Pass B, one workgroup per (mu, y-index): transform along
-// x, weight, store.  Both passes run a Stockham autosort FFT in LDS with radix
+// x, weight, store -- or, for the mixed path's V, cut the weighted line into
+// int8 digits with one exponent per line and store the digit planes
+// (FftDigits), so the spectrum itself is never written.  Both passes run a
+// Stockham autosort FFT in LDS with radix
 // 8, 4, 2, 3 and 5 stages; twiddles come from a table of e^{-2 pi i j/n}.
 //
 // LDS layout: every transform line is contiguous, lines ls = n rounded up to
@@ -26,6 +29,7 @@
 // transforms (z-lines to y-lines, the strided x-lines of the global layout)
 // are done on the way into LDS, so no stage reads a strided line.
 #include <algorithm>
+#include <type_traits>
 
 #include "common.h"
 
@@ -68,6 +72,10 @@ struct FftParams {
   Plan1D pz, py, px;
   const void* twz; const void* twy; const void* twx;
   int vec_out;  // output aligned to two elements: one vector store per complex value
+  signed char* dig_s;  // digit output of pass B (FftDigits): planes, exponents, strides
+  int* dig_e;
+  long dig_plane, dig_row;
+  int dig_lp;
 };
 
 // Division by a run-time divisor d >= 1 of dividends e with e * d < 2^32:
@@ -344,8 +352,9 @@ __global__ void weight_kernel(const double* __restrict__ coulG, int n1, int n2, 
   wt[i] = (T)sqrt(w * coulG[row * n2 + kz]);
 }
 
-// t: (k, n0, n1, n2h) complex -> out (k, n0, n1, n2h, 2) weighted by wt.
-template <class T, int IPV>  // IPV = 0: ping-pong stages
+// t: (k, n0, n1, n2h) complex -> out (k, n0, n1, n2h, 2) weighted by wt, or
+// (DIG) its int8 digit planes (FftDigits).
+template <class T, int IPV, bool DIG>  // IPV = 0: ping-pong stages
 __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const T* __restrict__ wt,
                                              T* __restrict__ out, FftParams p) {
   constexpr bool ip = IPV > 0;
@@ -380,6 +389,54 @@ __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const 
   C2<T>* fin = bufa;
   if (ip) fft_lines_ip<T, IPV>(bufa, p.px, n2h, lsx, twx);
   else fin = fft_lines<T>(bufa, bufb, p.px, n2h, lsx, twx);
+  if constexpr (DIG) {
+    // the line's weighted values a = v.x f, b = v.y f (rounded to T as the
+    // stored spectrum would be); exponent: max |value| < 2^ex; then each value
+    // times 2^-ex cut into digits (x *= 128, d = trunc(x), x -= d: exact in FP64)
+    __shared__ double red[FT / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // (the weighted values go back into the thread's own LDS slots for the
+    // digit pass)
+    double mx = 0.0;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+      const int kx = fn2h.div(e), kz = e - kx * n2h;
+      const T f = wt[((long)kx * n1 + iy) * n2h + kz];
+      C2<T>& slot = fin[swz(kz, kx, lsx)];
+      const C2<T> v = slot;
+      const T a = v.x * f, b = v.y * f;
+      slot = C2<T>{a, b};
+      mx = fmax(mx, fmax(fabs((double)a), fabs((double)b)));
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    if (lane == 0) red[wv] = mx;
+    __syncthreads();
+    mx = red[0];
+    for (int w = 1; w < FT / 64; ++w) mx = fmax(mx, red[w]);
+    int ex = 0;
+    if (mx > 0.0) frexp(mx, &ex);
+    if (threadIdx.x == 0) p.dig_e[mu * n1 + iy] = ex;
+    signed char* row = p.dig_s + mu * p.dig_row + (long)iy * p.dig_lp;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+      const int kx = fn2h.div(e), kz = e - kx * n2h;
+      const C2<T> v = fin[swz(kz, kx, lsx)];
+      double xa = ldexp((double)v.x, -ex), xb = ldexp((double)v.y, -ex);
+#pragma unroll
+      for (int q = 0; q < OZ_DIGITS; ++q) {
+        xa *= 128.0;
+        const double da = trunc(xa);
+        xa -= da;
+        xb *= 128.0;
+        const double db = trunc(xb);
+        xb -= db;
+        *reinterpret_cast<unsigned short*>(row + q * p.dig_plane + 2 * e) =
+            (unsigned short)(((unsigned)(int)da & 0xffu) | (((unsigned)(int)db & 0xffu) << 8));
+      }
+    }
+    for (int c = 2 * tot + threadIdx.x; c < p.dig_lp; c += blockDim.x)
+#pragma unroll
+      for (int q = 0; q < OZ_DIGITS; ++q) row[q * p.dig_plane + c] = 0;
+    return;
+  }
   typedef T v2 __attribute__((ext_vector_type(2)));
   T* dst = out + mu * (long)n0 * n1 * n2h * 2;
   for (int e = threadIdx.x; e < n0 * n2h; e += blockDim.x) {
@@ -442,16 +499,35 @@ int launch_a(dim3 g, size_t la, const X* x, C2<T>* tmid, const FftParams& p, hip
 
 template <class T, int IPV>
 int launch_b(dim3 g, size_t lb, const C2<T>* tmid, const T* wt, T* out, const FftParams& p, hipStream_t s) {
-  THC_HIP(hipFuncSetAttribute((const void*)pass_b<T, IPV>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
-  hipLaunchKernelGGL((pass_b<T, IPV>), g, dim3(FT), lb, s, tmid, wt, out, p);
+  if (p.dig_s) {
+    if constexpr (std::is_same<T, float>::value) {
+      THC_HIP(hipFuncSetAttribute((const void*)pass_b<T, IPV, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lb));
+      hipLaunchKernelGGL((pass_b<T, IPV, true>), g, dim3(FT), lb, s, tmid, wt, out, p);
+    } else {
+      return THC_ERR_ARG;  // digits come from the FP32 transform only
+    }
+  } else {
+    THC_HIP(hipFuncSetAttribute((const void*)pass_b<T, IPV, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lb));
+    hipLaunchKernelGGL((pass_b<T, IPV, false>), g, dim3(FT), lb, s, tmid, wt, out, p);
+  }
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
 
 template <class T, class X>
 int run(const X* x, int k, const int n[3], const double* coulG, void* out, void* work, hipStream_t s,
-        hipEvent_t* pass_ev) {
-  FftParams p;
+        hipEvent_t* pass_ev, const FftDigits* dig) {
+  FftParams p = {};
+  if (dig) {
+    p.dig_s = dig->s;
+    p.dig_e = dig->e;
+    p.dig_row = dig->g2p;
+    p.dig_plane = (long)k * dig->g2p;
+    p.dig_lp = dig->lp;
+    if (dig->lp < 2L * n[0] * (n[2] / 2 + 1) || dig->lp % 2 || dig->g2p != (long)n[1] * dig->lp) return THC_ERR_ARG;
+  }
   p.k = k; p.n0 = n[0]; p.n1 = n[1]; p.n2 = n[2]; p.n2h = n[2] / 2 + 1;
   p.lsz = line_stride(n[2]); p.lsy = line_stride(n[1]); p.lsx = line_stride(n[0]);
   if (!make_plan(n[2], p.pz) || !make_plan(n[1], p.py) || !make_plan(n[0], p.px)) return THC_ERR_UNSUPPORTED;
@@ -504,16 +580,17 @@ size_t fft_work_bytes(int k, const int n[3], int prec) {
 }
 
 int fft_coulomb(const void* x, int x_f32, int k, const int n[3], const double* coulG, void* out, int prec,
-                void* work, hipStream_t s, hipEvent_t* pass_ev) {
+                void* work, hipStream_t s, hipEvent_t* pass_ev, const FftDigits* dig) {
   if (k <= 0) return THC_OK;
   for (int d = 0; d < 3; ++d)
     if (n[d] < 1 || n[d] > MAXN) return THC_ERR_UNSUPPORTED;
   if (prec == 0) {
     if (x_f32) return THC_ERR_ARG;  // FP64 arithmetic keeps FP64 input
-    return run<double>(reinterpret_cast<const double*>(x), k, n, coulG, out, work, s, pass_ev);
+    if (dig) return THC_ERR_ARG;
+    return run<double>(reinterpret_cast<const double*>(x), k, n, coulG, out, work, s, pass_ev, nullptr);
   }
-  if (x_f32) return run<float>(reinterpret_cast<const float*>(x), k, n, coulG, out, work, s, pass_ev);
-  return run<float>(reinterpret_cast<const double*>(x), k, n, coulG, out, work, s, pass_ev);
+  if (x_f32) return run<float>(reinterpret_cast<const float*>(x), k, n, coulG, out, work, s, pass_ev, dig);
+  return run<float>(reinterpret_cast<const double*>(x), k, n, coulG, out, work, s, pass_ev, dig);
 }
 
 }  // namespace thc

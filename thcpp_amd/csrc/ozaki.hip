@@ -1,10 +1,10 @@
 // V = B B^T for the mixed-precision path, evaluated exactly in int8 MFMA
 // arithmetic on splits of B (an Ozaki-style emulation of the FP64 Gram).
 //
-// Split: every row of B is cut into chunks of ch columns (8192, halved down to
-// 1024 while the lower tiles times chunks give fewer than 2048 workgroups, so
-// that small systems still spread over the CUs); each (row, chunk)
-// gets a power-of-two exponent e with max |b| < 2^e, and b 2^-e in (-1, 1) is
+// Split (done by the FFT's pass B as it produces B, fft.hip): the columns of
+// a row of B come in chunks, one per y-line of the half spectrum (2 n0
+// (n2/2+1) values, zero padded to a multiple of 64); each (row, chunk) gets a
+// power-of-two exponent e with max |b| < 2^e, and b 2^-e in (-1, 1) is
 // written as OZ_S signed 7-bit digits, b = 2^e sum_p d_p 2^-7(p+1) + r,
 // |r| < 2^(e - 7 OZ_S).  Every step (scale by 128, truncate, subtract) is
 // exact in double.
@@ -12,8 +12,9 @@
 // 2^-7(p+q+2) sum_g d_p,rg d_q,cg.  The inner sums are int8 x int8 products
 // accumulated in int32 by v_mfma_i32_16x16x64_i8, grouped by p + q (products
 // with the same p + q share a scale), exact as long as
-// OZ_S * ch * 127^2 < 2^31.  Digit products with p + q >= OZ_S are
-// dropped; they are below 2^-7(OZ_S+1) of |b_r| |b_c| per term.
+// OZ_S * chunk * 127^2 < 2^31 (chunks up to 26624 columns: any grid edge up
+// to 128).  Digit products with p + q >= OZ_S are dropped; they are below
+// 2^-7(OZ_S+1) of |b_r| |b_c| per term.
 // Each workgroup takes one 128 x 128 lower tile and one chunk, converts its
 // groups to double, scales by 2^(e_r + e_c) and writes a partial tile; the
 // partials are summed in chunk order in FP64.
@@ -32,12 +33,9 @@ namespace {
 
 typedef int i4 __attribute__((ext_vector_type(4)));
 
-constexpr int OZ_S = 5;       // digits per value
-constexpr int OZ_CH = 8192;   // largest chunk (one exponent per row and chunk)
-constexpr int OZ_CH_MIN = 1024;  // smallest: one 4-column run per thread of oz_split
+constexpr int OZ_S = OZ_DIGITS;  // digits per value
 constexpr int OZ_T = 128;     // tile edge
 constexpr int OZ_KS = 64;     // k per MFMA step
-static_assert((long)OZ_S * OZ_CH * 127 * 127 < 2147483647L, "int32 group sums must not overflow");
 
 __device__ inline double block_max(double v, double* red) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
@@ -47,70 +45,6 @@ __device__ inline double block_max(double v, double* red) {
   v = red[0];
   for (int i = 1; i < (int)(blockDim.x >> 6); ++i) v = fmax(v, red[i]);
   return v;
-}
-
-// four consecutive values of a row as doubles (vector loads when aligned)
-__device__ inline void load4(const float* p, bool vec, long g, long g2, double (&v)[4]) {
-  if (vec && g + 4 <= g2) {
-    const float4 q = *reinterpret_cast<const float4*>(p + g);
-    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-  } else {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = g + j < g2 ? (double)p[g + j] : 0.0;
-  }
-}
-__device__ inline void load4(const double* p, bool vec, long g, long g2, double (&v)[4]) {
-  if (vec && g + 4 <= g2) {
-    const double2 q0 = *reinterpret_cast<const double2*>(p + g), q1 = *reinterpret_cast<const double2*>(p + g + 2);
-    v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
-  } else {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = g + j < g2 ? p[g + j] : 0.0;
-  }
-}
-
-// digits S[p][row][g] (g < g2p, zero padded) and exponents E[row][chunk].
-// Thread t takes runs of 4 consecutive columns (one vector load), and stores
-// each digit plane's 4 bytes as one word.
-template <class T, int GRP>  // GRP = ch / 1024 runs of 4 columns per thread
-__global__ __launch_bounds__(256) void oz_split(const T* __restrict__ b, long ld, long g2, long g2p, int k,
-                                                int nchunk, signed char* __restrict__ S, int* __restrict__ E) {
-  __shared__ double red[4];
-  const int c = blockIdx.x, row = blockIdx.y;
-  const T* src = b + (long)row * ld;
-  const long g0 = (long)c * (GRP * 1024);
-  const bool vec = ((reinterpret_cast<uintptr_t>(src) & 15) == 0);
-  double v[GRP][4];
-  double mx = 0.0;
-#pragma unroll
-  for (int q = 0; q < GRP; ++q) {
-    load4(src, vec, g0 + ((long)q * 256 + threadIdx.x) * 4, g2, v[q]);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) mx = fmax(mx, fabs(v[q][j]));
-  }
-  mx = block_max(mx, red);
-  int ex = 0;
-  if (mx > 0.0) frexp(mx, &ex);  // mx < 2^ex
-  if (threadIdx.x == 0) E[(long)row * nchunk + c] = ex;
-#pragma unroll
-  for (int q = 0; q < GRP; ++q) {
-    const long g = g0 + ((long)q * 256 + threadIdx.x) * 4;
-    double x[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) x[j] = ldexp(v[q][j], -ex);
-#pragma unroll
-    for (int p = 0; p < OZ_S; ++p) {
-      unsigned w = 0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        x[j] *= 128.0;
-        const double d = trunc(x[j]);
-        x[j] -= d;
-        w |= ((unsigned)(int)d & 0xffu) << (8 * j);
-      }
-      *reinterpret_cast<unsigned*>(S + ((long)p * k + row) * g2p + g) = w;
-    }
-  }
 }
 
 // One lower 128 x 128 tile (tm >= tn) and one chunk per workgroup; 8 waves
@@ -296,55 +230,63 @@ __global__ __launch_bounds__(256) void oz_reduce(const double* __restrict__ work
     if (mr < k && mc + u < k && mr < mc + u) v[mr * ldv + mc + u] = S[c0 + u][r];
 }
 
-void oz_plan(int k, long g2, int& ch, int& nchunk, long& g2p, long& ntiles) {
-  const long nb = (k + OZ_T - 1) / OZ_T;
-  ntiles = nb * (nb + 1) / 2;
-  ch = OZ_CH;
-  while (ch > OZ_CH_MIN && ntiles * ((g2 + ch - 1) / ch) < 2048) ch /= 2;
-  nchunk = (int)((g2 + ch - 1) / ch);
-  g2p = (long)nchunk * ch;
-}
 
 }  // namespace
 
-size_t oz_syrk_work_bytes(int k, long g2) {
-  int ch, nchunk;
-  long g2p, ntiles;
-  oz_plan(k, g2, ch, nchunk, g2p, ntiles);
-  return align_up((size_t)OZ_S * k * g2p, 256) + align_up(sizeof(int) * (size_t)k * nchunk, 256) +
-         align_up(sizeof(double) * (size_t)nchunk * ntiles * OZ_T * OZ_T, 256) + 256;
+
+// ---- V from the FFT's digit planes: one chunk per (row, y-line) ----------------
+
+int oz_line_len(const int n[3]) {
+  const long len = 2L * n[0] * (n[2] / 2 + 1);
+  return (int)((len + OZ_KS - 1) / OZ_KS * OZ_KS);
 }
 
-int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ldv, void* work, hipStream_t s,
-            hipEvent_t e0, hipEvent_t e1) {
-  if (k <= 0) return THC_OK;
-  int ch, nchunk;
+static void oz_lines_plan(int k, const int n[3], int& lp, long& g2p, long& ntiles) {
+  lp = oz_line_len(n);
+  g2p = (long)n[1] * lp;
+  const long nb = (k + OZ_T - 1) / OZ_T;
+  ntiles = nb * (nb + 1) / 2;
+}
+
+size_t oz_lines_work_bytes(int k, const int n[3]) {
+  int lp;
   long g2p, ntiles;
-  oz_plan(k, g2, ch, nchunk, g2p, ntiles);
+  oz_lines_plan(k, n, lp, g2p, ntiles);
+  return align_up((size_t)OZ_S * k * g2p, 256) + align_up(sizeof(int) * (size_t)k * n[1], 256) +
+         align_up(sizeof(double) * (size_t)n[1] * ntiles * OZ_T * OZ_T, 256) + 256;
+}
+
+FftDigits oz_lines_layout(int k, const int n[3], void* work) {
+  int lp;
+  long g2p, ntiles;
+  oz_lines_plan(k, n, lp, g2p, ntiles);
   char* w = reinterpret_cast<char*>(work);
-  signed char* S = reinterpret_cast<signed char*>(w);
+  FftDigits d;
+  d.s = reinterpret_cast<signed char*>(w);
   w += align_up((size_t)OZ_S * k * g2p, 256);
-  int* E = reinterpret_cast<int*>(w);
-  w += align_up(sizeof(int) * (size_t)k * nchunk, 256);
-  double* part = reinterpret_cast<double*>(w);
-  const dim3 sg(nchunk, k);
-  auto split = [&](auto tag) {
-    typedef decltype(tag) T;
-    const T* bt = reinterpret_cast<const T*>(b);
-    switch (ch / OZ_CH_MIN) {
-      case 1: hipLaunchKernelGGL((oz_split<T, 1>), sg, dim3(256), 0, s, bt, ld, g2, g2p, k, nchunk, S, E); break;
-      case 2: hipLaunchKernelGGL((oz_split<T, 2>), sg, dim3(256), 0, s, bt, ld, g2, g2p, k, nchunk, S, E); break;
-      case 4: hipLaunchKernelGGL((oz_split<T, 4>), sg, dim3(256), 0, s, bt, ld, g2, g2p, k, nchunk, S, E); break;
-      default: hipLaunchKernelGGL((oz_split<T, 8>), sg, dim3(256), 0, s, bt, ld, g2, g2p, k, nchunk, S, E); break;
-    }
-  };
-  if (prec == 0) split(double());
-  else split(float());
-  THC_HIP(hipGetLastError());
+  d.e = reinterpret_cast<int*>(w);
+  d.g2p = g2p;
+  d.lp = lp;
+  return d;
+}
+
+int oz_syrk_lines(const FftDigits& d, int k, const int n[3], double* v, long ldv, void* work, hipStream_t s,
+                  hipEvent_t e0, hipEvent_t e1) {
+  if (k <= 0) return THC_OK;
+  static_assert((long)OZ_S * 26624 * 127 * 127 < 2147483647L, "int32 sums of a 128-edge grid's line");
+  int lp;
+  long g2p, ntiles;
+  oz_lines_plan(k, n, lp, g2p, ntiles);
+  if (lp != d.lp || g2p != d.g2p || lp > 26624) return THC_ERR_ARG;
+  const int nchunk = n[1];
+  double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + align_up((size_t)OZ_S * k * g2p, 256) +
+                                           align_up(sizeof(int) * (size_t)k * nchunk, 256));
   const long slots = (ntiles * nchunk + 7) / 8 * 8;  // every w < ntiles * nchunk gets one workgroup
   if (slots > 0x7fffffffL) return THC_ERR_ARG;
   if (e0) THC_HIP(hipEventRecord(e0, s));
-  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, S, g2, g2p, k, E, nchunk, ch, part, ntiles);
+  // every column of a chunk is data or zero padding: g2 = g2p
+  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, d.s, g2p, g2p, k, d.e, nchunk, lp, part,
+                     ntiles);
   THC_HIP(hipGetLastError());
   if (e1) THC_HIP(hipEventRecord(e1, s));
   hipLaunchKernelGGL(oz_reduce, dim3((unsigned)(ntiles * (OZ_T / RB) * (OZ_T / RB))), dim3(256), 0, s, part, nchunk,
@@ -352,6 +294,7 @@ int oz_syrk(const void* b, int k, long g2, long ld, int prec, double* v, long ld
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
+
 
 }  // namespace thc
 
