@@ -36,7 +36,7 @@ SV_PRODUCTS = SV_DIGITS * (SV_DIGITS + 1) // 2
 # FETCH_SIZE / WRITE_SIZE counters scaled by the same counters on a 1 GiB device copy (fetch x 2.0,
 # write x 1.0); the summary with its commands is profiles/isdf_diamond64_n64_k2048_hbm_traffic.txt.
 # Other workloads report null.
-SOLVE_TRAFFIC_BYTES = {"diamond64_n64_k2048": 20974770 * 1024 * 2.0 + 2097152 * 1024 * 1.0}
+SOLVE_TRAFFIC_BYTES = {"diamond64_n64_k2048": 18202105 * 1024 * 2.0 + 2097152 * 1024 * 1.0}
 METRIC = "ISDF factorizations/s + % roofline; mixed vs FP64 GPU at <=1e-6 rel V error"
 STAGES = ["zct", "cct_chol_inv", "solve", "fft", "V", "luv", "isdf_total", "refine", "solve_gemm_kernel",
           "V_kernel", "fft_pass_a", "fft_pass_b"]
@@ -126,8 +126,9 @@ def rel_rows(dev, host):
 def batched_leg(ctx, steps, warmup):
     """The uneven 32-cell batch (side 6-14 bohr, 8-64 atoms, grids 24^3-48^3,
     N_mu = 8-10 M) through thc_factorize_batched, mixed precision, on 4
-    streams and, for comparison, on 1; then the plain FP64 path one system
-    at a time (one stream), the baseline of the batched speed-up."""
+    streams and, for comparison, on 1, and in FP64 on 1; then the plain FP64
+    GPU path -- thc_factorize one system after the other, no batching -- the
+    baseline of the batched speed-up."""
     import torch
     from thcpp_amd import _lib as L
     from thcpp_amd import synthetic
@@ -150,8 +151,24 @@ def batched_leg(ctx, steps, warmup):
         if st != L.THC_OK:
             raise RuntimeError("batched factorisation status %d" % st)
         out[name] = {"ms_per_batch": round(dt * 1e3, 3), "systems_per_s": round(len(batch) / dt, 2)}
-    out["mixed_streams_4_over_fp64_streams_1"] = round(out["fp64_streams_1"]["ms_per_batch"] /
-                                                       out["streams_4"]["ms_per_batch"], 3)
+    cells = [(ctx.f64(s.grid), ctx.f64(s.rho), ctx.f64(s.aoR), ctx.f64(s.coulG), s) for s in batch]
+
+    def one_at_a_time():
+        for g, r, a, cg, s in cells:
+            o = ctx.factorize(g, r, a, s.thc_cfac, s.max_it, s.n, cg, seed=s.seed, precision=L.PREC_FP64)
+            if o["status"] != L.THC_OK:
+                raise RuntimeError("factorisation status %d" % o["status"])
+    for _ in range(warmup):
+        one_at_a_time()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one_at_a_time()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    out["fp64_one_at_a_time"] = {"ms_per_batch": round(dt * 1e3, 3), "systems_per_s": round(len(batch) / dt, 2)}
+    out["mixed_streams_4_over_fp64_one_at_a_time"] = round(out["fp64_one_at_a_time"]["ms_per_batch"] /
+                                                           out["streams_4"]["ms_per_batch"], 3)
     return out
 
 
