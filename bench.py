@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 PEAK = {"hbm_gbs": 8000.0, "fp64_tflops": 78.6, "fp32_tflops": 157.3, "bf16_tflops": 2500.0, "int8_tops": 5000.0}
 SV_DIGITS = 7                        # digits of the int8 solve; products with p + q < 7
 SV_PRODUCTS = SV_DIGITS * (SV_DIGITS + 1) // 2
+OZ_V_DIGITS = 5                      # digits per value of the int8 V (written by FFT pass B)
 # HBM bytes per launch of the mixed path's solve GEMM (sv_gemm) on diamond64_n64_k2048, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE counters scaled by the same counters on a 1 GiB device copy (fetch x 2.0,
 # write x 1.0); the summary with its commands is profiles/isdf_diamond64_n64_k2048_hbm_traffic.txt.
@@ -207,17 +208,23 @@ def small_cell_leg(ctx, steps, with_cpu):
     return out
 
 
-def bandwidth(stages, s, x_bytes, es):
+def bandwidth(stages, s, x_bytes, es, digits=False):
     """Achieved GB/s of the HBM-bound stages from their device times: each FFT
     pass reads and writes the k-row block once (pass A: x in, the complex
     z/y-transformed half spectrum out; pass B: that spectrum and the weights
-    in, the weighted spectrum out); k-means reads the coordinates (24 B) and
-    the density (8 B) and writes the assignment (4 B) of every point once per
-    iteration."""
+    in, the weighted spectrum out -- or, with digits, V's five int8 digit
+    planes, each y-line of 2 n0 (n2/2+1) bytes padded to a multiple of 64,
+    and one int32 exponent per (row, y-line)); k-means reads the coordinates
+    (24 B) and the density (8 B) and writes the assignment (4 B) of every
+    point once per iteration."""
     K, N = s.K, s.N
     nh = s.n[0] * s.n[1] * (s.n[2] // 2 + 1)
     a_bytes = K * N * x_bytes + K * nh * 2 * es
-    b_bytes = 2 * K * nh * 2 * es + nh * es
+    if digits:
+        lp = (2 * s.n[0] * (s.n[2] // 2 + 1) + 63) // 64 * 64
+        b_bytes = K * nh * 2 * es + nh * es + OZ_V_DIGITS * K * s.n[1] * lp + 4 * K * s.n[1]
+    else:
+        b_bytes = 2 * K * nh * 2 * es + nh * es
     km_bytes = 36.0 * N * s.max_it
     out = {}
     for name, nbytes, ms in (("fft_pass_a", a_bytes, stages[10]), ("fft_pass_b", b_bytes, stages[11]),
@@ -297,7 +304,7 @@ def main():
         "fp64_path": {"value": round(1000.0 / res["fp64"]["ms"], 3), "ms_per_step": round(res["fp64"]["ms"], 3),
                       "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES + ["kmeans"], res["fp64"]["stages"])}},
         "mixed_over_fp64_speedup": round(res["fp64"]["ms"] / ms, 3),
-        "bandwidth": {"mixed": bandwidth(st, s, 4, 4), "fp64": bandwidth(res["fp64"]["stages"], s, 8, 8)},
+        "bandwidth": {"mixed": bandwidth(st, s, 4, 4, digits=True), "fp64": bandwidth(res["fp64"]["stages"], s, 8, 8)},
         "rel_V_error_mixed_vs_fp64": rel(mx["V"], res["fp64"]["V"]),
         "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES + ["kmeans"], st)},
         "roofline": {"kernel": "solve GEMM X = CC^T^-1 ZC^T as exact int8 digit products (sv_gemm, "

@@ -70,7 +70,7 @@ enum thc_precision {
   THC_PREC_MIXED = 3  /* solve X = (CC^T)^-1 ZC^T from exact int8 MFMA products of 7 signed 7-bit
                          digits of the rho-scaled operands (28 digit products, int32 sums, FP64
                          epilogue); FP32 FFT; V from exact int8 MFMA products of 5 digits per value
-                         (per row and 8192-column chunk exponents), summed in FP64.  Relative V error
+                         (per row and y-line exponents; the FFT writes the digits), summed in FP64.  Relative V error
                          against the FP64 path about 2e-7 on 64^3 grids with N_mu 1024-2048 */
 };
 
@@ -320,6 +320,29 @@ int thc_czt_half(thc_ctx *ctx, const double *aoR, const double *aoR_half, long n
 int thc_isdf_half(thc_ctx *ctx, const double *aoR, const double *aoR_half, long n, int m, int m_occ,
                   const int *idx, int k, const int fft_grid[3], const double *coulG,
                   const thc_isdf_opts *opts, double *muv, double *xi, thc_isdf_stats *stats);
+
+/* Half-rotated ZC^T for complex orbitals, k x n complex (re, im interleaved):
+ * Pocc o conj(P) with P[mu][r] = sum_i conj(phi_i(r_mu)) phi_i(r) over the
+ * m orbitals and Pocc the same over the m_occ occupied ones
+ * (IVecs::setup_CZt_half, src/interpolating_vectors.cpp:143-161, with complex
+ * aoR / aoR_half).  aoR_ri: n x m and aoR_half_ri: n x m_occ complex,
+ * interleaved (n x 2m, n x 2 m_occ doubles). */
+int thc_czt_half_complex(thc_ctx *ctx, const double *aoR_ri, const double *aoR_half_ri, long n, int m, int m_occ,
+                         const int *idx, int k, double *czt_ri);
+
+/* thc_isdf_half for complex orbitals (src/interpolating_vectors.cpp:143-161,
+ * 331-404 with both FFT directions, 258-273, 317-328): ZC^T and CC^T are
+ * complex (CC^T Hermitian), X = CC^T^-1 ZC^T is complex, and
+ * Muv[mu][nu] = sum_G IVMG[G][mu] coulG(G) IVG[G][nu] with IVG / IVMG the
+ * forward / backward transforms of conj(X[mu]) -- complex symmetric, not
+ * Hermitian.  The solve runs on the real 2k x 2k embedding of CC^T, the FFT
+ * and the G contraction on the 2k real rows Re X, Im X (coulG must be even in
+ * G, as the Coulomb kernel on an FFT grid is).  muv_ri: k x k complex out
+ * (interleaved, full); xi_ri (nullable): X, k x n complex out.  precision as
+ * for thc_isdf (MIXED needs 2k <= 4096); opts->want_luv is ignored. */
+int thc_isdf_half_complex(thc_ctx *ctx, const double *aoR_ri, const double *aoR_half_ri, long n, int m, int m_occ,
+                          const int *idx, int k, const int fft_grid[3], const double *coulG,
+                          const thc_isdf_opts *opts, double *muv_ri, double *xi_ri, thc_isdf_stats *stats);
 
 /* ---------------------------------------------------------------- batches */
 

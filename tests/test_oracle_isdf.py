@@ -243,3 +243,67 @@ def test_fft_and_muv_fast_options_agree():
     m = IO.construct_muv(a, cg)
     assert rel(IO.construct_muv(a, cg, real_part=True), m.real) < 1e-14
 
+
+
+# ---------------------------------------------------------------- complex half-rotated path
+
+def _complex_half_case(seed=3, n=(3, 4, 5), m=3, m_occ=2, k=None):
+    rng = np.random.default_rng(seed)
+    N = int(np.prod(n))
+    phi = rng.standard_normal((N, m)) + 1j * rng.standard_normal((N, m))
+    occ = rng.standard_normal((N, m_occ)) + 1j * rng.standard_normal((N, m_occ))
+    k = k or m * m_occ
+    idx = np.sort(rng.choice(N, k, replace=False))
+    f = [np.fft.fftfreq(q) * q for q in n]
+    g2 = f[0][:, None, None] ** 2 + f[1][None, :, None] ** 2 + f[2][None, None, :] ** 2
+    c = np.where(g2 > 0, 1.0 / np.where(g2 > 0, g2, 1.0), 0.0).reshape(-1)
+    # W[r, r'] = sum_G c(G) e^{i G.(r - r')} by brute force over the grid
+    pts = np.stack(np.meshgrid(*[np.arange(q) for q in n], indexing="ij"), -1).reshape(-1, 3)
+    e = np.exp(2j * np.pi * sum(np.outer(pts[:, d], pts[:, d]) / n[d] for d in range(3)))
+    w = (e * c[None, :]) @ e.conj().T
+    # pair functions rho_ik(r) = phi_occ,i(r) conj(phi_k(r)), one row per (i, k)
+    rho = np.einsum("ri,rk->rik", occ, phi.conj()).reshape(N, -1).T
+    return phi, occ, idx, n, c, w, rho
+
+
+def test_complex_half_solves_the_pair_least_squares():
+    """Pocc o conj(P) = C^H Z and CC^T = C^H C for the pair functions
+    rho_ik(r) = phi_occ,i(r) conj(phi_k(r)) (C = rho at the points, Z = rho on
+    the grid), so X must be the least-squares fit rho ~ C X computed from the
+    explicit pair matrix; a conjugation slip in P or in the Hadamard product
+    fits other functions and fails."""
+    phi, occ, idx, n, c, w, rho = _complex_half_case(k=5)
+    out = IO.isdf_half(phi, occ, idx, n, c, return_all=True)
+    x_ls = np.linalg.lstsq(rho[:, idx], rho, rcond=None)[0]
+    assert rel(out["X"], x_ls) < 1e-11
+    z_wrong = IO.pseudo_dm(occ, idx) * IO.pseudo_dm(phi, idx)
+    assert rel(IO.least_squares(z_wrong[:, idx], z_wrong), x_ls) > 1e-3
+
+
+def test_complex_half_muv_is_the_coulomb_matrix_of_the_vectors():
+    """Muv = IVMG^T IVG with the forward / backward transforms of conj(X) is
+    sum_{r, r'} conj(X_mu(r)) W(r - r') conj(X_nu(r')) with the brute-force
+    kernel W(d) = sum_G coulG(G) e^{i G.d}; complex symmetric for an even
+    coulG.  With as many points as pair functions the interpolation is exact
+    and conj(C) Muv conj(C)^T reproduces the brute-force Coulomb integrals of
+    the conjugated pairs; Muv of X (no conjugation) does not."""
+    phi, occ, idx, n, c, w, rho = _complex_half_case()
+    out = IO.isdf_half(phi, occ, idx, n, c, return_all=True)
+    muv = out["Muv"]
+    v = np.conj(out["X"])
+    assert rel(muv, v @ w @ v.T) < 1e-13
+    assert rel(muv, muv.T) < 1e-13
+    cr = np.conj(rho)
+    c_pts = rho[:, idx]
+    assert rel(np.conj(c_pts) @ muv @ np.conj(c_pts).T, cr @ w @ cr.T) < 1e-11
+    assert rel(out["X"] @ w @ out["X"].T, muv) > 1e-2
+
+
+def test_complex_half_real_orbitals_reduce_to_real_half():
+    s = synthetic.make_system(reps=1, n=8, M=6, thc_cfac=2, max_it=1)
+    idx = np.sort(np.random.default_rng(4).choice(s.N, s.K, replace=False))
+    occ = s.aoR[:, :3]
+    real = IO.isdf_half(s.aoR, occ, idx, s.n, s.coulG)["Muv"]
+    cplx = IO.isdf_half(s.aoR.astype(np.complex128), occ.astype(np.complex128), idx, s.n, s.coulG)["Muv"]
+    assert rel(cplx.real, real) < 1e-12
+    assert np.abs(cplx.imag).max() < 1e-12 * np.abs(real).max()

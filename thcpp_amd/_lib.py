@@ -90,6 +90,9 @@ SIGNATURES = {
     "thc_czt_half": (_i, [_vp, _vp, _vp, _l, _i, _i, _vp, _i, _vp]),
     "thc_isdf_half": (_i, [_vp, _vp, _vp, _l, _i, _i, _vp, _i, ctypes.POINTER(_i), _vp, ctypes.POINTER(IsdfOpts),
                            _vp, _vp, ctypes.POINTER(IsdfStats)]),
+    "thc_czt_half_complex": (_i, [_vp, _vp, _vp, _l, _i, _i, _vp, _i, _vp]),
+    "thc_isdf_half_complex": (_i, [_vp, _vp, _vp, _l, _i, _i, _vp, _i, ctypes.POINTER(_i), _vp,
+                                   ctypes.POINTER(IsdfOpts), _vp, _vp, ctypes.POINTER(IsdfStats)]),
     "thc_factorize_batched": (_i, [_vp, _i, ctypes.POINTER(System), ctypes.POINTER(IsdfOpts), _i]),
 }
 

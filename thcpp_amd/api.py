@@ -284,6 +284,33 @@ class Context:
         return {"status": st, "Muv": muv, "xi": xi, "cct_info": stats.cct_chol_info,
                 "stage_ms": list(stats.stage_ms)}
 
+    def czt_half_complex(self, aoR, aoR_half, idx):
+        """Half-rotated ZC^T of complex orbitals: (K, N) complex128 torch tensor."""
+        a, ah, i = self.f64(self._interleave(aoR)), self.f64(self._interleave(aoR_half)), self.i32(idx)
+        n, m, mo, k = a.shape[0], a.shape[1] // 2, ah.shape[1] // 2, i.shape[0]
+        out = torch.empty((k, n, 2), dtype=torch.float64, device=self.device)
+        L.check(self._lib().thc_czt_half_complex(self.h, _p(a), _p(ah), n, m, mo, _p(i), k, _p(out)),
+                "thc_czt_half_complex")
+        return torch.view_as_complex(out)
+
+    def isdf_half_complex(self, aoR, aoR_half, idx, fft_grid, coulG, precision=L.PREC_FP64, refine=0,
+                          want_xi=False, check_rank=False):
+        """Half-rotated ISDF of complex orbitals: Muv (K, K) and, with want_xi,
+        X (K, N), complex128 torch tensors."""
+        a, ah = self.f64(self._interleave(aoR)), self.f64(self._interleave(aoR_half))
+        i, cg = self.i32(idx), self.f64(coulG).reshape(-1)
+        n, m, mo, k = a.shape[0], a.shape[1] // 2, ah.shape[1] // 2, i.shape[0]
+        muv = torch.empty((k, k, 2), dtype=torch.float64, device=self.device)
+        xi = torch.empty((k, n, 2), dtype=torch.float64, device=self.device) if want_xi else None
+        opts = L.IsdfOpts(precision, refine, 0, 1 if check_rank else 0)
+        stats = L.IsdfStats()
+        fg = (ctypes.c_int * 3)(*[int(q) for q in fft_grid])
+        st = self._lib().thc_isdf_half_complex(self.h, _p(a), _p(ah), n, m, mo, _p(i), k, fg, _p(cg),
+                                           ctypes.byref(opts), _p(muv), _p(xi), ctypes.byref(stats))
+        return {"status": st, "Muv": torch.view_as_complex(muv),
+                "xi": torch.view_as_complex(xi) if xi is not None else None, "cct_info": stats.cct_chol_info,
+                "cct_rank": stats.cct_rank, "stage_ms": list(stats.stage_ms)}
+
     def factorize(self, grid, rho, aoR, thc_cfac, max_it, fft_grid, coulG, threshold=-1.0, seed=0,
                   precision=L.PREC_FP64, refine=0, want_luv=True):
         g, r, a, cg = self.f64(grid), self.f64(rho).reshape(-1), self.f64(aoR), self.f64(coulG).reshape(-1)

@@ -87,6 +87,75 @@ __global__ __launch_bounds__(256) void times_i(const double* phi, long count, do
   }
 }
 
+// Half-rotated ZC^T of complex orbitals in the stacked real layout zt (n x 2k
+// row-major: zt[r][mu] = Re CZt[mu][r], zt[r][k + mu] = Im CZt[mu][r]).  On
+// entry zt holds (Re P, Im P) and t (same layout) (Re Pocc, Im Pocc); CZt =
+// Pocc o conj(P) (setup_CZt_half, src/interpolating_vectors.cpp:152-155).
+__global__ __launch_bounds__(256) void half_complex_combine(double* zt, const double* t, long n, int k) {
+  const long k2 = 2L * k, total = n * k;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += 256L * gridDim.x) {
+    const long r = e / k, o = r * k2 + (e - r * k);
+    const double rp = zt[o], ip = zt[o + k], ro = t[o], io = t[o + k];
+    zt[o] = ro * rp + io * ip;
+    zt[o + k] = io * rp - ro * ip;
+  }
+}
+
+// The real 2k x 2k embedding [[Re C, -Im C], [Im C, Re C]] of the Hermitian
+// CC^T = CZt[:, idx], from the stacked zt: Re C[i][j] = zt[idx[j]][i],
+// Im C[i][j] = zt[idx[j]][k + i]; indices outside [0, n) give zeros (the
+// caller reports them).
+__global__ __launch_bounds__(256) void half_complex_cct(const double* zt, long n, const int* idx, int k,
+                                                        double* c2) {
+  const long k2 = 2L * k, total = k2 * k2;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += 256L * gridDim.x) {
+    const int i = (int)(e / k2), j = (int)(e - (long)i * k2);
+    const int bi = i >= k, bj = j >= k, ii = i - bi * k, jj = j - bj * k;
+    const long q = idx[jj];
+    double cr = 0.0, ci = 0.0;
+    if (q >= 0 && q < n) {
+      cr = zt[q * k2 + ii];
+      ci = zt[q * k2 + k + ii];
+    }
+    c2[e] = bi == bj ? cr : (bi ? ci : -ci);
+  }
+}
+
+// Muv of the half-rotated pass from the Gram matrix g (2k x 2k, full) of the
+// weighted spectra of Re X (rows 0..k-1) and Im X (rows k..2k-1):
+// Muv = G_rr - G_ii - i (G_ri + G_ir), written (re, im) interleaved.
+__global__ __launch_bounds__(256) void half_complex_muv(const double* g, int k, double* muv) {
+  const long k2 = 2L * k, total = (long)k * k;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += 256L * gridDim.x) {
+    const long mu = e / k, nu = e - mu * k;
+    muv[2 * e] = g[mu * k2 + nu] - g[(k + mu) * k2 + k + nu];
+    muv[2 * e + 1] = -(g[mu * k2 + k + nu] + g[(k + mu) * k2 + nu]);
+  }
+}
+
+// out[mu][r] = (x[mu][r], x[k + mu][r]) interleaved, x 2k x n
+__global__ __launch_bounds__(256) void interleave_halves(const double* x, long n, int k, double* out) {
+  const long total = (long)k * n, half = (long)k * n;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += 256L * gridDim.x) {
+    out[2 * e] = x[e];
+    out[2 * e + 1] = x[half + e];
+  }
+}
+
+// out[mu][r] = (zt[r][mu], zt[r][k + mu]) interleaved, zt n x 2k
+__global__ __launch_bounds__(256) void untranspose_stacked(const double* zt, long n, int k, double* out) {
+  const long total = (long)k * n, k2 = 2L * k;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += 256L * gridDim.x) {
+    const long mu = e / n, r = e - mu * n;
+    out[2 * e] = zt[r * k2 + mu];
+    out[2 * e + 1] = zt[r * k2 + k + mu];
+  }
+}
+
+__global__ void repeat_idx(const int* idx, int k, int* idx2) {
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < 2 * k; j += 256 * gridDim.x) idx2[j] = idx[j < k ? j : j - k];
+}
+
 __global__ __launch_bounds__(256) void copy_f64(const double* x, double* y, long count) {
   for (long e = blockIdx.x * 256L + threadIdx.x; e < count; e += 256L * gridDim.x) y[e] = x[e];
 }
@@ -97,6 +166,33 @@ __global__ void check_idx(const int* idx, int k, long n, int* bad) {
 }
 
 inline int grid_for(long count) { return (int)std::min<long>(8192, std::max<long>(1, (count + 255) / 256)); }
+
+// Stacked half-rotated ZC^T of complex orbitals (half_complex_combine) for m
+// orbitals (aoR: n x 2m interleaved) and m_occ occupied ones (aoR_half: n x
+// 2 m_occ).  P[mu][r] = sum_i conj(phi_i(r_mu)) phi_i(r): Re P is Phi against
+// Phi over the 2m interleaved columns, Im P is Phi against i Phi (w = i Phi,
+// wo = i Phi_occ, filled here), rows of the second operand gathered at the
+// points.  t: n x 2k scratch.
+int half_complex_zt(const double* aoR, const double* aoR_half, long n, int m, int m_occ, const int* idx, int k,
+                    double* zt, double* t, double* w, double* wo, hipStream_t s) {
+  hipLaunchKernelGGL(times_i, dim3(grid_for(n * m)), dim3(256), 0, s, aoR, n * m, w);
+  hipLaunchKernelGGL(times_i, dim3(grid_for(n * m_occ)), dim3(256), 0, s, aoR_half, n * m_occ, wo);
+  THC_HIP(hipGetLastError());
+  const struct { const double* a; const double* b; int width; double* c; } prods[4] = {
+      {aoR, aoR, 2 * m, zt}, {aoR, w, 2 * m, zt + k}, {aoR_half, aoR_half, 2 * m_occ, t}, {aoR_half, wo, 2 * m_occ, t + k}};
+  for (const auto& q : prods) {
+    GemmArgs g = {};
+    g.m = n; g.n = k; g.k = q.width;
+    g.a = q.a; g.lda = q.width; g.a_trans = 0;
+    g.b = q.b; g.ldb = q.width; g.b_trans = 1; g.b_rows = idx; g.b_nrows = n;
+    g.c = q.c; g.ldc = 2L * k; g.c_trans = 0;
+    g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
+    THC_TRY(gemm_f64(g, s));
+  }
+  hipLaunchKernelGGL(half_complex_combine, dim3(grid_for(n * k)), dim3(256), 0, s, zt, t, n, k);
+  THC_HIP(hipGetLastError());
+  return THC_OK;
+}
 
 // deterministic operands of the GEMM timing hook: values in [-0.5, 0.5),
 // gathered rows spread over [0, rows)
@@ -113,6 +209,8 @@ struct IsdfBufs {
   double *z, *x, *c, *l, *ci, *r, *v;
   double *phim, *phio;  // Phi rows at the points (k x m), occupied set (k x m_occ)
   double* w;            // i Phi for complex orbitals (n x 2m), else null
+  double* wo;           // i Phi_occ for complex half-rotated runs, else null
+  int* idx2;            // the points twice (columns of the 2k-row real embedding), else null
   void* b;      // weighted half spectrum (f64 or f32)
   void *chol_w, *inv_w, *fft_w, *syrk_w;
   void* solve_w;        // digit planes of the int8 solve (mixed precision)
@@ -120,13 +218,16 @@ struct IsdfBufs {
   int* info;
 };
 
-// m is the real row width of Phi (2 x orbitals for interleaved complex ones)
-size_t isdf_bytes(long n, int k, int m, int m_occ, bool cplx, const int fft[3], int prec, int refine, bool own_x,
-                  bool rank) {
+// m is the real row width of Phi (2 x orbitals for interleaved complex ones),
+// m_occ that of the occupied set; k the rows of the real system solved (2 N_mu
+// for complex half-rotated runs, ch)
+size_t isdf_bytes(long n, int k, int m, int m_occ, bool cplx, bool ch, const int fft[3], int prec, int refine,
+                  bool own_x, bool rank) {
   const long g2 = half_spectrum_len(fft);
   const size_t kk = (size_t)k * k;
   size_t b = carve_bytes<double>((size_t)k * m) + carve_bytes<double>((size_t)k * std::max(m_occ, 1));
   if (cplx) b += carve_bytes<double>((size_t)n * m);
+  if (ch) b += carve_bytes<double>((size_t)n * m_occ) + carve_bytes<int>((size_t)k);
   b += carve_bytes<double>((size_t)k * n);                   // z
   if (own_x) b += carve_bytes<double>((size_t)k * n);        // x
   b += 4 * carve_bytes<double>(kk);                          // c, l, ci, v
@@ -143,8 +244,8 @@ size_t isdf_bytes(long n, int k, int m, int m_occ, bool cplx, const int fft[3], 
   return b;
 }
 
-bool isdf_carve(Arena& a, long n, int k, int m, int m_occ, bool cplx, const int fft[3], int prec, int refine,
-                double* xi, bool rank, IsdfBufs& B) {
+bool isdf_carve(Arena& a, long n, int k, int m, int m_occ, bool cplx, bool ch, const int fft[3], int prec,
+                int refine, double* xi, bool rank, IsdfBufs& B) {
   const long g2 = half_spectrum_len(fft);
   const size_t kk = (size_t)k * k;
   B = IsdfBufs{};
@@ -152,6 +253,9 @@ bool isdf_carve(Arena& a, long n, int k, int m, int m_occ, bool cplx, const int 
   B.phio = a.take<double>((size_t)k * std::max(m_occ, 1));
   B.w = cplx ? a.take<double>((size_t)n * m) : nullptr;
   if (cplx && !B.w) return false;
+  B.wo = ch ? a.take<double>((size_t)n * m_occ) : nullptr;
+  B.idx2 = ch ? a.take<int>((size_t)k) : nullptr;
+  if (ch && (!B.wo || !B.idx2)) return false;
   B.z = a.take<double>((size_t)k * n);
   B.x = xi ? xi : a.take<double>((size_t)k * n);
   B.c = a.take<double>(kk);
@@ -186,13 +290,20 @@ __global__ void zero_ints(int* p, int count) {
 // cplx: aoR holds m complex orbitals interleaved (n x 2m doubles); then
 // ZC^T = |P|^2 = (Re P)^2 + (Im P)^2 with Re P from Phi against Phi and Im P
 // from Phi against i Phi, both real GEMMs over the 2m interleaved columns.
+// cplx with aoR_half (complex half-rotated, ch): ZC^T = Pocc o conj(P) and the
+// Hermitian CC^T are complex; the solve runs on the real embedding (2k rows:
+// Re X then Im X, against [[Re C, -Im C], [Im C, Re C]]), the FFT and V on
+// those 2k real rows, and Muv is assembled from the 2k x 2k Gram matrix
+// (half_complex_muv).  v_out and xi_out are then complex, interleaved.
 int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m_orb, int m_occ, bool cplx,
              const int* idx, int k, const int fft[3], const double* coulG, const thc_isdf_opts* opts,
              double* v_out, double* luv_out, double* xi_out, thc_isdf_stats* stats) {
   if (!ctx || !aoR || !idx || !fft || !coulG || !v_out || n < 1 || m_orb < 1 || k < 1 || k > n) return THC_ERR_ARG;
   if (aoR_half && m_occ < 1) return THC_ERR_ARG;
-  if (aoR_half && cplx) return THC_ERR_UNSUPPORTED;  // Pocc o conj(P) is complex there
+  const bool ch = cplx && aoR_half;
   const int m = cplx ? 2 * m_orb : m_orb;           // real row width of Phi
+  const int mo = cplx ? 2 * m_occ : m_occ;          // real row width of Phi_occ
+  const int kr = ch ? 2 * k : k;                    // rows of the real system solved
   const int prec = opts ? opts->precision : THC_PREC_FP64;
   const int refine = opts ? std::max(0, opts->refine_steps) : 0;
   const int want_luv = opts ? opts->want_luv : 1;
@@ -208,12 +319,15 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   }
   THC_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  if (want_rank && k > THC_MAX_CLUSTERS) return THC_ERR_UNSUPPORTED;
-  THC_TRY(ctx->ws.reserve(isdf_bytes(n, k, m, m_occ, cplx, fft, prec, refine, xi_out == nullptr, want_rank)));
+  if (want_rank && kr > THC_MAX_CLUSTERS) return THC_ERR_UNSUPPORTED;
+  // int32 digit-group sums of the int8 solve are exact up to 4096 rows
+  if (prec == THC_PREC_MIXED && kr > 4096) return THC_ERR_UNSUPPORTED;
+  THC_TRY(ctx->ws.reserve(isdf_bytes(n, kr, m, mo, cplx, ch, fft, prec, refine, ch || xi_out == nullptr, want_rank)));
   ctx->ws.reset();
   IsdfBufs B;
-  if (!isdf_carve(ctx->ws, n, k, m, m_occ, cplx, fft, prec, refine, xi_out, want_rank, B)) return THC_ERR_NO_MEMORY;
-  if (want_rank) THC_TRY(ensure_pinned(ctx, sizeof(double) * 2 * (size_t)k));
+  if (!isdf_carve(ctx->ws, n, kr, m, mo, cplx, ch, fft, prec, refine, ch ? nullptr : xi_out, want_rank, B))
+    return THC_ERR_NO_MEMORY;
+  if (want_rank) THC_TRY(ensure_pinned(ctx, sizeof(double) * 2 * (size_t)kr));
   const long g2 = half_spectrum_len(fft);
   hipEvent_t* ev = ctx->ev;
 
@@ -226,87 +340,106 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   //    equals the gathered columns of ZC^T bit for bit.  Then the Cholesky
   //    factor and the explicit inverse.
   hipStream_t sa = ctx->aux;
-  if (cplx) {
+  if (cplx && !ch) {
     hipLaunchKernelGGL(times_i, dim3(grid_for((long)n * m_orb)), dim3(256), 0, s, aoR, (long)n * m_orb, B.w);
     THC_HIP(hipGetLastError());
   }
   THC_HIP(hipEventRecord(ctx->fork, s));
   THC_HIP(hipStreamWaitEvent(sa, ctx->fork, 0));
-  hipLaunchKernelGGL(gather_rows, dim3(grid_for((long)k * m)), dim3(256), 0, sa, aoR, n, m, idx, k, B.phim);
-  if (aoR_half)
-    hipLaunchKernelGGL(gather_rows, dim3(grid_for((long)k * m_occ)), dim3(256), 0, sa, aoR_half, n, m_occ, idx, k,
-                       B.phio);
-  THC_HIP(hipGetLastError());
-  {
-    GemmArgs g = {};
-    g.m = k; g.n = k; g.k = m;
-    g.a = B.phim; g.lda = m; g.a_trans = 0;
-    g.b = aoR; g.ldb = m; g.b_trans = 1; g.b_rows = idx; g.b_nrows = n;
-    g.c = B.c; g.ldc = k; g.c_trans = 0;
-    g.alpha = 1.0; g.beta = 0.0; g.epi_square = aoR_half ? 0 : 1; g.batch = 1;
-    THC_TRY(gemm_f64(g, sa));
-    if (aoR_half) {
-      GemmArgs h = g;
-      h.k = m_occ;
-      h.a = B.phio; h.lda = m_occ;
-      h.b = aoR_half; h.ldb = m_occ;
-      h.epi_hadamard = 1;
-      THC_TRY(gemm_f64(h, sa));
-    }
-    if (cplx) {
-      GemmArgs h = g;
-      h.b = B.w;
-      h.epi_square = 0; h.epi_addsquare = 1;
-      THC_TRY(gemm_f64(h, sa));
-    }
-  }
-  hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * k)), dim3(256), 0, sa, B.c, B.l, (long)k * k);
-  THC_HIP(hipGetLastError());
-  THC_TRY(cholesky_f64(B.l, k, k, B.info, B.chol_w, sa));
-  THC_TRY(spd_inverse_from_chol(B.l, k, k, B.ci, B.inv_w, sa));
-  THC_HIP(hipEventRecord(ctx->join, sa));
-  // rank diagnostic (IVecs::check_rank) after the join: nothing on the main
-  // stream waits for it; its tridiagonal form lands in pinned host memory
   double* rk_d = reinterpret_cast<double*>(ctx->pinned);
-  if (want_rank) THC_TRY(sym_tridiag(B.c, k, k, rk_d, rk_d + k, B.rank_w, sa));
-  // 1. Zt = (Phi Phi_mu^T)^2, N x K row-major (Zt[r][mu] = ZC^T[mu][r]): B rows
-  //    gathered by idx, epilogue squares; this layout lets the solve read both
-  //    operands along k and keeps the 4 GB store row-contiguous
-  //    Half-rotated (setup_CZt_half): Zt = P o Pocc, P stored first, then the
-  //    occupied-orbital product multiplied into it in the epilogue.
-  //    Mixed path without refinement: nothing but the solve reads Zt, so the
-  //    GEMM's epilogue writes the solve's int8 digit planes of Zt directly
-  //    (the same values as splitting the FP64 Zt afterwards) and Zt itself is
-  //    never stored.
-  const bool fuse_split = prec == THC_PREC_MIXED && !aoR_half && !cplx && refine == 0;
-  {
-    GemmArgs g = {};
-    g.m = n; g.n = k; g.k = m;
-    g.a = aoR; g.lda = m; g.a_trans = 0;
-    g.b = aoR; g.ldb = m; g.b_trans = 1; g.b_rows = idx; g.b_nrows = n;
-    g.c = B.z; g.ldc = k; g.c_trans = 0;
-    g.alpha = 1.0; g.beta = 0.0; g.epi_square = aoR_half ? 0 : 1; g.batch = 1;
-    if (fuse_split) {
-      THC_TRY(oz_solve_prepare(k, n, idx, aoR, m, B.solve_w, g, s));
-      g.epi_square = 0;
+  // The digit epilogue runs on the one-workgroup-per-CU 128 x 128 x 32 tile, the
+  // tile the ZC^T GEMM takes anyway for inner dimensions up to 256; deeper
+  // products (M > 256) run faster on the two-per-CU tile with a separate split.
+  const bool fuse_split = prec == THC_PREC_MIXED && !aoR_half && !cplx && refine == 0 && m <= 256;
+  if (ch) {
+    // 1 + 2, complex half-rotated: the stacked ZC^T (the X buffer as scratch),
+    // then CC^T's real embedding gathered from it, its Cholesky factor and
+    // inverse, all on the main stream
+    hipLaunchKernelGGL(repeat_idx, dim3(grid_for(2L * k)), dim3(256), 0, s, idx, k, B.idx2);
+    THC_TRY(half_complex_zt(aoR, aoR_half, n, m_orb, m_occ, idx, k, B.z, B.x, B.w, B.wo, s));
+    THC_HIP(hipEventRecord(ev[1], s));
+    hipLaunchKernelGGL(half_complex_cct, dim3(grid_for((long)kr * kr)), dim3(256), 0, s, B.z, n, idx, k, B.c);
+    hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)kr * kr)), dim3(256), 0, s, B.c, B.l, (long)kr * kr);
+    THC_HIP(hipGetLastError());
+    THC_TRY(cholesky_f64(B.l, kr, kr, B.info, B.chol_w, s));
+    THC_TRY(spd_inverse_from_chol(B.l, kr, kr, B.ci, B.inv_w, s));
+    if (want_rank) THC_TRY(sym_tridiag(B.c, kr, kr, rk_d, rk_d + kr, B.rank_w, s));
+    THC_HIP(hipEventRecord(ctx->join, sa));
+  } else {
+    hipLaunchKernelGGL(gather_rows, dim3(grid_for((long)k * m)), dim3(256), 0, sa, aoR, n, m, idx, k, B.phim);
+    if (aoR_half)
+      hipLaunchKernelGGL(gather_rows, dim3(grid_for((long)k * m_occ)), dim3(256), 0, sa, aoR_half, n, m_occ, idx, k,
+                         B.phio);
+    THC_HIP(hipGetLastError());
+    {
+      GemmArgs g = {};
+      g.m = k; g.n = k; g.k = m;
+      g.a = B.phim; g.lda = m; g.a_trans = 0;
+      g.b = aoR; g.ldb = m; g.b_trans = 1; g.b_rows = idx; g.b_nrows = n;
+      g.c = B.c; g.ldc = k; g.c_trans = 0;
+      g.alpha = 1.0; g.beta = 0.0; g.epi_square = aoR_half ? 0 : 1; g.batch = 1;
+      THC_TRY(gemm_f64(g, sa));
+      if (aoR_half) {
+        GemmArgs h = g;
+        h.k = m_occ;
+        h.a = B.phio; h.lda = m_occ;
+        h.b = aoR_half; h.ldb = m_occ;
+        h.epi_hadamard = 1;
+        THC_TRY(gemm_f64(h, sa));
+      }
+      if (cplx) {
+        GemmArgs h = g;
+        h.b = B.w;
+        h.epi_square = 0; h.epi_addsquare = 1;
+        THC_TRY(gemm_f64(h, sa));
+      }
     }
-    THC_TRY(gemm_f64(g, s));
-    if (aoR_half) {
-      GemmArgs h = g;
-      h.k = m_occ;
-      h.a = aoR_half; h.lda = m_occ;
-      h.b = aoR_half; h.ldb = m_occ;
-      h.epi_hadamard = 1;
-      THC_TRY(gemm_f64(h, s));
+    hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * k)), dim3(256), 0, sa, B.c, B.l, (long)k * k);
+    THC_HIP(hipGetLastError());
+    THC_TRY(cholesky_f64(B.l, k, k, B.info, B.chol_w, sa));
+    THC_TRY(spd_inverse_from_chol(B.l, k, k, B.ci, B.inv_w, sa));
+    THC_HIP(hipEventRecord(ctx->join, sa));
+    // rank diagnostic (IVecs::check_rank) after the join: nothing on the main
+    // stream waits for it; its tridiagonal form lands in pinned host memory
+    if (want_rank) THC_TRY(sym_tridiag(B.c, k, k, rk_d, rk_d + k, B.rank_w, sa));
+    // 1. Zt = (Phi Phi_mu^T)^2, N x K row-major (Zt[r][mu] = ZC^T[mu][r]): B rows
+    //    gathered by idx, epilogue squares; this layout lets the solve read both
+    //    operands along k and keeps the 4 GB store row-contiguous
+    //    Half-rotated (setup_CZt_half): Zt = P o Pocc, P stored first, then the
+    //    occupied-orbital product multiplied into it in the epilogue.
+    //    Mixed path without refinement: nothing but the solve reads Zt, so the
+    //    GEMM's epilogue writes the solve's int8 digit planes of Zt directly
+    //    (the same values as splitting the FP64 Zt afterwards) and Zt itself is
+    //    never stored.
+    {
+      GemmArgs g = {};
+      g.m = n; g.n = k; g.k = m;
+      g.a = aoR; g.lda = m; g.a_trans = 0;
+      g.b = aoR; g.ldb = m; g.b_trans = 1; g.b_rows = idx; g.b_nrows = n;
+      g.c = B.z; g.ldc = k; g.c_trans = 0;
+      g.alpha = 1.0; g.beta = 0.0; g.epi_square = aoR_half ? 0 : 1; g.batch = 1;
+      if (fuse_split) {
+        THC_TRY(oz_solve_prepare(k, n, idx, aoR, m, B.solve_w, g, s));
+        g.epi_square = 0;
+      }
+      THC_TRY(gemm_f64(g, s));
+      if (aoR_half) {
+        GemmArgs h = g;
+        h.k = m_occ;
+        h.a = aoR_half; h.lda = m_occ;
+        h.b = aoR_half; h.ldb = m_occ;
+        h.epi_hadamard = 1;
+        THC_TRY(gemm_f64(h, s));
+      }
+      if (cplx) {  // Zt += (Im P)^2
+        GemmArgs h = g;
+        h.b = B.w;
+        h.epi_square = 0; h.epi_addsquare = 1;
+        THC_TRY(gemm_f64(h, s));
+      }
     }
-    if (cplx) {  // Zt += (Im P)^2
-      GemmArgs h = g;
-      h.b = B.w;
-      h.epi_square = 0; h.epi_addsquare = 1;
-      THC_TRY(gemm_f64(h, s));
-    }
+    THC_HIP(hipEventRecord(ev[1], s));
   }
-  THC_HIP(hipEventRecord(ev[1], s));
   // join: the solve needs (CC^T)^-1
   THC_HIP(hipStreamWaitEvent(s, ctx->join, 0));
   THC_HIP(hipEventRecord(ev[2], s));
@@ -315,14 +448,15 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   // stores it as FP32 directly: the same values, half the bytes.
   const int x_f32 = prec != THC_PREC_FP64 && xi_out == nullptr && refine == 0;
   if (prec == THC_PREC_MIXED) {
-    // exact int8 digit products of the scaled operands (ozaki.hip)
-    THC_TRY(oz_solve(B.ci, k, B.z, k, n, idx, aoR, m, aoR_half, aoR_half ? m_occ : 0, B.x, x_f32, n, B.solve_w, s,
-                     ev[8], ev[9], fuse_split ? 1 : 0));
+    // exact int8 digit products of the scaled operands (ozaki.hip); the point
+    // scales of the 2k embedded columns are those of idx, twice
+    THC_TRY(oz_solve(B.ci, kr, B.z, kr, n, ch ? B.idx2 : idx, aoR, m, aoR_half, aoR_half ? mo : 0, B.x, x_f32, n,
+                     B.solve_w, s, ev[8], ev[9], fuse_split ? 1 : 0));
   } else {
     GemmArgs g = {};
-    g.m = k; g.n = n; g.k = k;
-    g.a = B.ci; g.lda = k; g.a_trans = 0;
-    g.b = B.z; g.ldb = k; g.b_trans = 1;
+    g.m = kr; g.n = n; g.k = kr;
+    g.a = B.ci; g.lda = kr; g.a_trans = 0;
+    g.b = B.z; g.ldb = kr; g.b_trans = 1;
     g.c = B.x; g.ldc = n;
     g.alpha = 1.0; g.beta = 0.0; g.batch = 1;
     g.c_f32 = x_f32;
@@ -334,18 +468,18 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   // 3b. FP64 iterative refinement in the transposed layout:
   //     Rt = Zt - X^T C^T;  X += C^-1 Rt^T
   for (int it = 0; it < refine; ++it) {
-    hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * n)), dim3(256), 0, s, B.z, B.r, (long)k * n);
+    hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)kr * n)), dim3(256), 0, s, B.z, B.r, (long)kr * n);
     GemmArgs g = {};
-    g.m = n; g.n = k; g.k = k;
+    g.m = n; g.n = kr; g.k = kr;
     g.a = B.x; g.lda = n; g.a_trans = 1;
-    g.b = B.c; g.ldb = k; g.b_trans = 1;
-    g.c = B.r; g.ldc = k;
+    g.b = B.c; g.ldb = kr; g.b_trans = 1;
+    g.c = B.r; g.ldc = kr;
     g.alpha = -1.0; g.beta = 1.0; g.batch = 1;
     THC_TRY(gemm_f64(g, s));
     GemmArgs h = {};
-    h.m = k; h.n = n; h.k = k;
-    h.a = B.ci; h.lda = k; h.a_trans = 0;
-    h.b = B.r; h.ldb = k; h.b_trans = 1;
+    h.m = kr; h.n = n; h.k = kr;
+    h.a = B.ci; h.lda = kr; h.a_trans = 0;
+    h.b = B.r; h.ldb = kr; h.b_trans = 1;
     h.c = B.x; h.ldc = n;
     h.alpha = 1.0; h.beta = 1.0; h.batch = 1;
     THC_TRY(gemm_f64(h, s));
@@ -354,19 +488,27 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   // 4. weighted half spectrum
   //    (mixed: pass B writes V's int8 digit planes, one exponent per y-line)
   FftDigits dig = {};
-  if (prec == THC_PREC_MIXED) dig = oz_lines_layout(k, fft, B.syrk_w);
-  THC_TRY(fft_coulomb(B.x, x_f32, k, fft, coulG, B.b, prec == THC_PREC_FP64 ? 0 : 1, B.fft_w, s, ev + 12,
+  if (prec == THC_PREC_MIXED) dig = oz_lines_layout(kr, fft, B.syrk_w);
+  THC_TRY(fft_coulomb(B.x, x_f32, kr, fft, coulG, B.b, prec == THC_PREC_FP64 ? 0 : 1, B.fft_w, s, ev + 12,
                       prec == THC_PREC_MIXED ? &dig : nullptr));
   THC_HIP(hipEventRecord(ev[5], s));
-  // 5. V = B B^T, FP64 sum of the split-K partials (exact int8 digit products on the mixed path)
+  // 5. V = B B^T, FP64 sum of the split-K partials (exact int8 digit products on the mixed path);
+  //    complex half-rotated: the 2k x 2k Gram matrix, then Muv from its blocks
+  double* gram = ch ? B.v : v_out;
   if (prec == THC_PREC_MIXED) {
-    THC_TRY(oz_syrk_lines(dig, k, fft, v_out, k, B.syrk_w, s, ev[10], ev[11]));
+    THC_TRY(oz_syrk_lines(dig, kr, fft, gram, kr, B.syrk_w, s, ev[10], ev[11]));
   } else {
-    THC_TRY(syrk_splitk(B.b, k, g2, g2, prec == THC_PREC_FP64 ? 0 : 1, v_out, k, B.syrk_w, s, ev[10], ev[11]));
+    THC_TRY(syrk_splitk(B.b, kr, g2, g2, prec == THC_PREC_FP64 ? 0 : 1, gram, kr, B.syrk_w, s, ev[10], ev[11]));
+  }
+  if (ch) {
+    hipLaunchKernelGGL(half_complex_muv, dim3(grid_for((long)k * k)), dim3(256), 0, s, B.v, k, v_out);
+    if (xi_out)
+      hipLaunchKernelGGL(interleave_halves, dim3(grid_for((long)k * n)), dim3(256), 0, s, B.x, n, k, xi_out);
+    THC_HIP(hipGetLastError());
   }
   THC_HIP(hipEventRecord(ev[6], s));
   // 6. Luv = chol(V)
-  if (want_luv && luv_out) {
+  if (want_luv && luv_out && !ch) {
     hipLaunchKernelGGL(copy_f64, dim3(grid_for((long)k * k)), dim3(256), 0, s, v_out, luv_out, (long)k * k);
     THC_TRY(cholesky_f64(luv_out, k, k, B.info + 1, B.chol_w, s));
   }
@@ -378,7 +520,8 @@ int isdf_run(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, in
   double cct_smax = 0.0;
   if (want_rank) {
     THC_HIP(hipStreamSynchronize(sa));
-    tridiag_rank(rk_d, rk_d + k, k, &cct_rank, &cct_smax);
+    tridiag_rank(rk_d, rk_d + kr, kr, &cct_rank, &cct_smax);
+    if (ch) cct_rank /= 2;  // the embedding holds every singular value of CC^T twice
   }
   if (stats) {
     stats->cct_chol_info = info[0];
@@ -634,6 +777,27 @@ int thc_czt_half(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n
   return THC_OK;
 }
 
+int thc_czt_half_complex(thc_ctx* ctx, const double* aoR_ri, const double* aoR_half_ri, long n, int m, int m_occ,
+                         const int* idx, int k, double* czt_ri) {
+  if (!ctx || !aoR_ri || !aoR_half_ri || !idx || !czt_ri || n < 1 || m < 1 || m_occ < 1 || k < 1 || k > n)
+    return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  const size_t zb = (size_t)n * 2 * k;
+  THC_TRY(ctx->ws.reserve(2 * carve_bytes<double>(zb) + carve_bytes<double>((size_t)n * 2 * m) +
+                          carve_bytes<double>((size_t)n * 2 * m_occ)));
+  ctx->ws.reset();
+  double* zt = ctx->ws.take<double>(zb);
+  double* t = ctx->ws.take<double>(zb);
+  double* w = ctx->ws.take<double>((size_t)n * 2 * m);
+  double* wo = ctx->ws.take<double>((size_t)n * 2 * m_occ);
+  if (!zt || !t || !w || !wo) return THC_ERR_NO_MEMORY;
+  THC_TRY(half_complex_zt(aoR_ri, aoR_half_ri, n, m, m_occ, idx, k, zt, t, w, wo, ctx->stream));
+  hipLaunchKernelGGL(untranspose_stacked, dim3(grid_for((long)k * n)), dim3(256), 0, ctx->stream, zt, n, k, czt_ri);
+  THC_HIP(hipGetLastError());
+  THC_HIP(hipStreamSynchronize(ctx->stream));
+  return THC_OK;
+}
+
 int thc_cct(thc_ctx* ctx, const double* czt, long n, const int* idx, int k, double* cct) {
   if (!ctx || !czt || !idx || !cct || n < 1 || k < 1) return THC_ERR_ARG;
   THC_HIP(hipSetDevice(ctx->device));
@@ -803,6 +967,14 @@ int thc_isdf_half(thc_ctx* ctx, const double* aoR, const double* aoR_half, long 
                   double* muv, double* xi, thc_isdf_stats* stats) {
   if (!aoR_half) return THC_ERR_ARG;
   return isdf_run(ctx, aoR, aoR_half, n, m, m_occ, false, idx, k, fft_grid, coulG, opts, muv, nullptr, xi, stats);
+}
+
+int thc_isdf_half_complex(thc_ctx* ctx, const double* aoR_ri, const double* aoR_half_ri, long n, int m, int m_occ,
+                          const int* idx, int k, const int fft_grid[3], const double* coulG,
+                          const thc_isdf_opts* opts, double* muv_ri, double* xi_ri, thc_isdf_stats* stats) {
+  if (!aoR_half_ri) return THC_ERR_ARG;
+  return isdf_run(ctx, aoR_ri, aoR_half_ri, n, m, m_occ, true, idx, k, fft_grid, coulG, opts, muv_ri, nullptr, xi_ri,
+                  stats);
 }
 
 int thc_factorize(thc_ctx* ctx, const double* grid, const double* rho, const double* aoR, long n, int m,
