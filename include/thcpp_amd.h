@@ -70,8 +70,9 @@ enum thc_precision {
   THC_PREC_MIXED = 3  /* solve X = (CC^T)^-1 ZC^T from exact int8 MFMA products of 7 signed 7-bit
                          digits of the rho-scaled operands (28 digit products, int32 sums, FP64
                          epilogue); FP32 FFT; V from exact int8 MFMA products of 5 digits per value
-                         (per row and y-line exponents; the FFT writes the digits), summed in FP64.  Relative V error
-                         against the FP64 path about 2e-7 on 64^3 grids with N_mu 1024-2048 */
+                         (per row and y-line exponents; the FFT writes the digits), summed in FP64.
+                         Relative V error against the FP64 path about 2e-7 on 64^3 grids with N_mu
+                         1024-2048 */
 };
 
 typedef struct thc_ctx thc_ctx;
@@ -166,6 +167,14 @@ int thc_map_to_grid(thc_ctx *ctx, const double *grid, long n, const double *cent
  * Returns THC_ERR_NOT_POSDEF when k exceeds the numerical rank. */
 int thc_qrcp_points(thc_ctx *ctx, const double *aoR, const double *aoR_half, long n, int m, int m_occ, int k,
                     int *interp_idx, double *rdiag);
+
+/* thc_qrcp_points for complex orbitals: ZT[(i k), a] = conj(phi_i(r_a))
+ * phi_k(r_a) (src/qrcp.cpp:33-66 with complex aoR / aoR_half).  The pivots
+ * come from pivoted Cholesky of ZT^H ZT = conj(Pocc) o P (|P|^2 without
+ * half rotation; complex Hermitian with it).  aoR_ri: n x m complex,
+ * aoR_half_ri (nullable): n x m_occ complex, both (re, im) interleaved. */
+int thc_qrcp_points_complex(thc_ctx *ctx, const double *aoR_ri, const double *aoR_half_ri, long n, int m,
+                            int m_occ, int k, int *interp_idx, double *rdiag);
 
 /* Full weighted k-means (KMeansSolver::kernel, src/kmeans.cpp:187-264) at one
  * rank: initial centroids from host_init_idx (host, k ints) or, if NULL, from

@@ -22,7 +22,7 @@ vp = ctypes.c_void_p
 def host():
     h = ctypes.CDLL(os.path.join(PKG, "libthcpp_amd_host.so"))
     h.thc_host_write_orbital_file.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, vp, vp, vp, vp, vp, vp,
-                                              vp, ctypes.c_double, vp, ctypes.c_int, vp]
+                                              vp, ctypes.c_double, vp, ctypes.c_int, vp, vp]
     h.thc_host_read_f64.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.c_long]
     h.thc_host_read_f64.restype = ctypes.c_long
     h.thc_host_read_int.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.c_long]
@@ -51,7 +51,7 @@ def test_dropin_driver(tmp_path, precision, tol):
     hc = np.arange(s.M * s.M, dtype=np.float64).reshape(s.M, s.M)
     ne = np.array([4, 4], dtype=np.int32)
     assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(s.aoR), P(fg), P(s.coulG),
-                                         P(hc), P(ne), -2.5, None, 0, None) == 0
+                                         P(hc), P(ne), -2.5, None, 0, None, None) == 0
     inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac,
            "interpolating_points": {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}}}
     if precision:
@@ -102,7 +102,7 @@ def test_dropin_driver_half_rotated(tmp_path):
     hc = np.eye(s.M)
     ne = np.array([4, 4], dtype=np.int32)
     assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(s.aoR), P(fg), P(s.coulG),
-                                         P(hc), P(ne), 0.0, P(occ), occ.shape[1], None) == 0
+                                         P(hc), P(ne), 0.0, P(occ), occ.shape[1], None, None) == 0
     inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac, "half_rotated": True,
            "thc_half_cfac": half_cfac,
            "interpolating_points": {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}}}
@@ -148,7 +148,7 @@ def test_dropin_driver_qrcp(tmp_path):
     hc = np.eye(s.M)
     ne = np.array([4, 4], dtype=np.int32)
     assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(s.aoR), P(fg), P(s.coulG),
-                                         P(hc), P(ne), 0.0, None, 0, None) == 0
+                                         P(hc), P(ne), 0.0, None, 0, None, None) == 0
     inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac, "interpolating_points": {"qrcp": {}}}
     (tmp_path / "input.json").write_text(json.dumps(inp))
     r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,
@@ -181,7 +181,7 @@ def test_dropin_driver_complex_orbitals(tmp_path):
     hc = np.eye(s.M)
     ne = np.array([4, 4], dtype=np.int32)
     assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(re), P(fg), P(s.coulG),
-                                         P(hc), P(ne), 0.0, None, 0, P(im)) == 0
+                                         P(hc), P(ne), 0.0, None, 0, P(im), None) == 0
     inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac,
            "interpolating_points": {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}}}
     (tmp_path / "input.json").write_text(json.dumps(inp))
@@ -205,3 +205,56 @@ def test_dropin_missing_key_fails(tmp_path):
     r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode != 0 and "thc_cfac" in r.stderr
+
+
+@pytest.mark.parametrize("points", ["kmeans", "qrcp"])
+def test_dropin_driver_complex_half_rotated(tmp_path, points):
+    """Complex orbitals with half_rotated = true: the half-rotated pass runs on
+    Pocc o conj(P) (complex ZC^T and Hermitian CC^T) and writes a complex
+    HalfTransformedMuv; with qrcp the points are the pivots of the complex ZT.
+    Orbitals: complex, non-unitary combinations of the real ones (phases alone
+    cancel in Pocc o conj(P))."""
+    from oracle import qrcp_oracle as QO
+    h = host()
+    s = synthetic.make_system(reps=1, n=16, M=8, thc_cfac=3, max_it=7)
+    rng = np.random.default_rng(13)
+    c = np.eye(s.M) + 0.3 * (rng.standard_normal((s.M, s.M)) + 1j * rng.standard_normal((s.M, s.M)))
+    a = s.aoR @ c
+    occ = np.ascontiguousarray(a[:, :4])
+    half_cfac = 2
+    kh = half_cfac * s.M
+    orb = str(tmp_path / "orbitals.h5")
+    thc = str(tmp_path / "thc.h5")
+    P = lambda x: np.ascontiguousarray(x).ctypes.data_as(vp)
+    re, im = np.ascontiguousarray(a.real), np.ascontiguousarray(a.imag)
+    ore, oim = np.ascontiguousarray(occ.real), np.ascontiguousarray(occ.imag)
+    fg = np.array(s.n, dtype=np.int32)
+    hc = np.eye(s.M)
+    ne = np.array([4, 4], dtype=np.int32)
+    assert h.thc_host_write_orbital_file(orb.encode(), s.N, s.M, P(s.grid), P(s.rho), P(re), P(fg), P(s.coulG),
+                                         P(hc), P(ne), 0.0, P(ore), 4, P(im), P(oim)) == 0
+    ip = {"kmeans": {"threshold": -1.0, "max_it": s.max_it, "rng_seed": 0}} if points == "kmeans" else {"qrcp": {}}
+    inp = {"orbital_file": orb, "output_file": thc, "thc_cfac": s.thc_cfac, "half_rotated": True,
+           "thc_half_cfac": half_cfac, "interpolating_points": ip}
+    (tmp_path / "input.json").write_text(json.dumps(inp))
+    r = subprocess.run([os.path.join(PKG, "thcpp_amd.x"), str(tmp_path / "input.json")], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    idx_h = read(h, thc, "/Hamiltonian/THC/HalfTransformedFullInterpIndx", np.int32)
+    if points == "kmeans":
+        ref_idx = KO.kmeans(s.grid, s.rho, kh, s.max_it, -1.0, 0)["interp_idx"]
+    else:
+        ref_idx = QO.qrcp_points(a, kh, occ)[0]
+    np.testing.assert_array_equal(idx_h, ref_idx)
+    occo = read(h, thc, "/Hamiltonian/THC/HalfTransformedOccOrbitals").reshape(4, kh, 2)
+    np.testing.assert_array_equal(occo[:, :, 0], ore[idx_h].T)
+    np.testing.assert_array_equal(occo[:, :, 1], oim[idx_h].T)
+    muv = read(h, thc, "/Hamiltonian/THC/HalfTransformedMuv").reshape(kh, kh, 2)
+    m = muv[:, :, 0] + 1j * muv[:, :, 1]
+    m_ref = IO.isdf_half(a, occ, idx_h, s.n, s.coulG)["Muv"]
+    assert np.abs(m_ref.imag).max() > 1e-3 * np.abs(m_ref).max()
+    err = np.linalg.norm(m - m_ref) / np.linalg.norm(m_ref)
+    print("complex half-rotated drop-in (%s points): rel Muv %.3e" % (points, err))
+    assert err < 1e-8, err  # cond(CC^T) 4e6 with the k-means points
+    assert "complex orbitals" in r.stdout

@@ -52,3 +52,35 @@ def test_qrcp_rank_exhausted(ctx):
     a = _orbitals(2000, 3, 7)
     st, _, _ = ctx.qrcp_points(a, 8)
     assert st == L.THC_ERR_NOT_POSDEF
+
+
+def _complex_orbitals(n, m, seed):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m)))
+    return q
+
+
+@pytest.mark.parametrize("n,m,mo,k", [(4096, 8, 0, 30), (3375, 10, 4, 36), (20000, 12, 5, 60)])
+def test_qrcp_complex_vs_oracle(ctx, n, m, mo, k):
+    """Complex orbitals: |P|^2 pivots (mo = 0) and the complex Hermitian
+    half-rotated Gram (mo occupied orbitals) against zgeqp3 of the explicit ZT."""
+    print("complex qrcp n=%d m=%d m_occ=%d k=%d" % (n, m, mo, k), flush=True)
+    a = _complex_orbitals(n, m, n + m)
+    occ = np.ascontiguousarray(a[:, :mo]) if mo else None
+    st, idx, rd = ctx.qrcp_points_complex(a, k, occ)
+    assert st == L.THC_OK
+    ref_idx, ref_rd, _ = QO.qrcp_points(a, k, occ)
+    np.testing.assert_array_equal(idx.cpu().numpy(), ref_idx)
+    np.testing.assert_allclose(rd, ref_rd, rtol=1e-7)
+
+
+def test_qrcp_complex_real_orbitals_match_real_path(ctx):
+    a = _orbitals(5000, 9, 6)
+    occ = np.ascontiguousarray(a[:, :4])
+    for o in (None, occ):
+        st_r, idx_r, rd_r = ctx.qrcp_points(a, 25, o)
+        st_c, idx_c, rd_c = ctx.qrcp_points_complex(a.astype(np.complex128), 25,
+                                                    o.astype(np.complex128) if o is not None else None)
+        assert st_r == L.THC_OK and st_c == L.THC_OK
+        np.testing.assert_array_equal(idx_c.cpu().numpy(), idx_r.cpu().numpy())
+        np.testing.assert_allclose(rd_c, rd_r, rtol=1e-12)

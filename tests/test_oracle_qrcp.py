@@ -119,3 +119,67 @@ def test_half_rotated_z_departs_from_reference_indexing():
     assert not np.array_equal(piv_lit, piv_kr)
     # the literal matrix has rank (nelec - 1) nelec + nbasis = 14 at most
     assert np.linalg.matrix_rank(lit) <= (mo - 1) * mo + m
+
+
+# ---------------------------------------------------------------- complex orbitals
+
+def _complex_orbitals(n, m, seed):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m)))
+    return q
+
+
+def _greedy_gram_schmidt_complex(z, k):
+    r = np.array(z, dtype=np.complex128, copy=True)
+    piv, diag = [], []
+    for _ in range(k):
+        norms = np.einsum("ij,ij->j", r.conj(), r).real
+        norms[piv] = -1.0
+        p = int(np.argmax(norms))
+        piv.append(p)
+        nrm = np.sqrt(norms[p])
+        diag.append(nrm)
+        q = r[:, p] / nrm
+        r -= np.outer(q, q.conj() @ r)
+    return np.array(piv), np.array(diag)
+
+
+def _pivoted_cholesky_hermitian(g, k):
+    """Pivoted Cholesky G = L L^H with complex columns (the HIP path's
+    half-rotated algorithm) in numpy."""
+    d = np.diag(g).real.copy()
+    l = np.zeros((k, g.shape[0]), dtype=np.complex128)
+    piv = []
+    for j in range(k):
+        dd = d.copy()
+        dd[piv] = -np.inf
+        p = int(np.argmax(dd))
+        piv.append(p)
+        l[j] = (g[:, p] - l[:j].T @ l[:j, p].conj()) / np.sqrt(d[p])
+        d -= np.abs(l[j]) ** 2
+    return np.array(piv)
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_complex_qrcp_matches_greedy_gram_schmidt_and_gram_identity(half):
+    """Complex orbitals: LAPACK's pivots (zgeqp3) equal a largest-residual
+    complex Gram-Schmidt; ZT^H ZT = conj(Pocc) o P with P = Phi^* Phi^T
+    (|P|^2, real, without half rotation), and pivoted Cholesky of that Gram
+    matrix picks the same columns.  Transposing a conjugation (P o P) breaks
+    the identity."""
+    a = _complex_orbitals(420, 6, 9)
+    occ = np.ascontiguousarray(a[:, :3]) if half else None
+    k = 14
+    z = QO.z_half_matrix(a, occ) if half else QO.z_matrix(a)
+    idx, rd, piv = QO.qrcp_points(a, k, occ)
+    gpiv, gdiag = _greedy_gram_schmidt_complex(z, k)
+    np.testing.assert_array_equal(piv, gpiv)
+    np.testing.assert_allclose(rd, gdiag, rtol=1e-10)
+    p = a.conj() @ a.T
+    po = (occ.conj() @ occ.T) if half else p
+    g = z.conj().T @ z
+    np.testing.assert_allclose(g, np.conj(po) * p, rtol=0, atol=1e-14 * np.abs(g).max())
+    if not half:
+        assert np.abs(g.imag).max() < 1e-15 * np.abs(g).max()
+    assert np.abs(g - p * p).max() > 1e-3 * np.abs(g).max()
+    np.testing.assert_array_equal(_pivoted_cholesky_hermitian(g, k), piv)

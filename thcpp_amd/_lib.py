@@ -87,6 +87,7 @@ SIGNATURES = {
     "thc_isdf_complex": (_i, [_vp, _vp, _l, _i, _vp, _i, ctypes.POINTER(_i), _vp, ctypes.POINTER(IsdfOpts), _vp, _vp,
                               _vp, ctypes.POINTER(IsdfStats)]),
     "thc_qrcp_points": (_i, [_vp, _vp, _vp, _l, _i, _i, _i, _vp, _vp]),
+    "thc_qrcp_points_complex": (_i, [_vp, _vp, _vp, _l, _i, _i, _i, _vp, _vp]),
     "thc_czt_half": (_i, [_vp, _vp, _vp, _l, _i, _i, _vp, _i, _vp]),
     "thc_isdf_half": (_i, [_vp, _vp, _vp, _l, _i, _i, _vp, _i, ctypes.POINTER(_i), _vp, ctypes.POINTER(IsdfOpts),
                            _vp, _vp, ctypes.POINTER(IsdfStats)]),

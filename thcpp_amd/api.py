@@ -168,6 +168,17 @@ class Context:
                                      rd.ctypes.data_as(ctypes.c_void_p))
         return st, idx, rd
 
+    def qrcp_points_complex(self, aoR, k, aoR_half=None):
+        """qrcp_points for complex orbitals ((N, M) complex arrays)."""
+        a = self.f64(self._interleave(aoR))
+        ah = self.f64(self._interleave(aoR_half)) if aoR_half is not None else None
+        n, m = a.shape[0], a.shape[1] // 2
+        idx = torch.empty(k, dtype=torch.int32, device=self.device)
+        rd = np.zeros(k)
+        st = self._lib().thc_qrcp_points_complex(self.h, _p(a), _p(ah), n, m, ah.shape[1] // 2 if ah is not None else 0,
+                                             k, _p(idx), rd.ctypes.data_as(ctypes.c_void_p))
+        return st, idx, rd
+
     # ------------------------------------------------------------ ISDF stages
     def czt(self, aoR, idx):
         a, i = self.f64(aoR), self.i32(idx)

@@ -756,6 +756,17 @@ int thc_qrcp_points(thc_ctx* ctx, const double* aoR, const double* aoR_half, lon
   return qrcp_points(aoR, aoR_half, n, m, aoR_half ? m_occ : 0, k, interp_idx, rdiag, ctx->ws, ctx->stream);
 }
 
+int thc_qrcp_points_complex(thc_ctx* ctx, const double* aoR_ri, const double* aoR_half_ri, long n, int m, int m_occ,
+                            int k, int* interp_idx, double* rdiag) {
+  if (!ctx || !aoR_ri || !interp_idx || n < 1 || m < 1 || k < 1 || k > n) return THC_ERR_ARG;
+  if (aoR_half_ri && m_occ < 1) return THC_ERR_ARG;
+  THC_HIP(hipSetDevice(ctx->device));
+  THC_TRY(ctx->ws.reserve(qrcp_work_bytes(n, k, true)));
+  ctx->ws.reset();
+  return qrcp_points(aoR_ri, aoR_half_ri, n, 2 * m, aoR_half_ri ? 2 * m_occ : 0, k, interp_idx, rdiag, ctx->ws,
+                     ctx->stream, true);
+}
+
 int thc_czt_half(thc_ctx* ctx, const double* aoR, const double* aoR_half, long n, int m, int m_occ, const int* idx,
                  int k, double* czt) {
   if (!ctx || !aoR || !aoR_half || !idx || !czt || n < 1 || m < 1 || m_occ < 1 || k < 1) return THC_ERR_ARG;

@@ -221,9 +221,11 @@ int oz_solve_prepare(int k, long n, const int* idx, const double* phi_a, int ma,
 // First k pivots of QRCP of the orbital-product matrix (occ = nullptr: all
 // orbitals on both sides), sorted ascending into piv_out (device); rdiag_out
 // (host, nullable) gets the k pivot magnitudes |R_jj| in pivot order.
-size_t qrcp_work_bytes(long n, int k);
+// cplx: phi / occ hold complex orbitals, (re, im) interleaved; m and mo are
+// the real row widths (twice the orbital counts).
+size_t qrcp_work_bytes(long n, int k, bool cplx = false);
 int qrcp_points(const double* phi, const double* occ, long n, int m, int mo, int k, int* piv_out, double* rdiag_out,
-                Arena& ws, hipStream_t s);
+                Arena& ws, hipStream_t s, bool cplx = false);
 
 // ---- FFT (fft.hip) ----------------------------------------------------------
 size_t fft_work_bytes(int k, const int n[3], int prec);

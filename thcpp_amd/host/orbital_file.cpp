@@ -13,7 +13,8 @@
 extern "C" int thc_host_write_orbital_file(const char* path, long n, int m, const double* grid, const double* rho,
                                            const double* aoR, const int* fft_grid, const double* coulG,
                                            const double* hcore, const int* nelec, double e0,
-                                           const double* aoR_half, int m_occ, const double* aoR_im) {
+                                           const double* aoR_half, int m_occ, const double* aoR_im,
+                                           const double* aoR_half_im) {
   try {
     thc::h5::File f(path, thc::h5::File::Truncate);
     const uint64_t N = n, M = m;
@@ -28,7 +29,10 @@ extern "C" int thc_host_write_orbital_file(const char* path, long n, int m, cons
     if (aoR_half && m_occ > 0) {
       const uint64_t MO = m_occ;
       std::vector<double> zh(2 * N * MO, 0.0);
-      for (uint64_t i = 0; i < N * MO; ++i) zh[2 * i] = aoR_half[i];
+      for (uint64_t i = 0; i < N * MO; ++i) {
+        zh[2 * i] = aoR_half[i];
+        if (aoR_half_im) zh[2 * i + 1] = aoR_half_im[i];
+      }
       f.write_compound_complex("aoR_half", zh.data(), {N, MO});
     }
     f.write_int("fft_grid", fft_grid, {3, 1});

@@ -74,8 +74,7 @@ std::vector<double> real_part_checked(const std::vector<double>& z, const std::s
     im = std::max(im, std::fabs(z[2 * i + 1]));
   }
   if (im > 1e-12 * std::max(1.0, re))
-    throw std::runtime_error(name + " has imaginary parts; this path takes real orbitals only (complex orbitals "
-                             "are supported by the full, non-half-rotated k-means path)");
+    throw std::runtime_error(name + " has imaginary parts where real orbitals are required");
   return r;
 }
 
@@ -229,12 +228,18 @@ void QRCPSolver::kernel(ContextHandler::BlacsHandler& BH, std::vector<int>& inte
     const auto d = fin.dims("aoR");
     n = (long)d.at(0);
     m = (int)d.at(1);
-    aoR = real_part_checked(fin.read_complex("aoR"), "aoR");
+    aoR = fin.read_complex("aoR");
     if (half_rotate) {
       const auto dh = fin.dims("aoR_half");
       mocc = (int)dh.at(1);
-      aoR_half = real_part_checked(fin.read_complex("aoR_half"), "aoR_half");
+      aoR_half = fin.read_complex("aoR_half");
     }
+  }
+  // complex orbitals (any non-zero imaginary part) stay interleaved
+  const bool cplx = has_imag(aoR) || (half_rotate && has_imag(aoR_half));
+  if (!cplx) {
+    aoR = real_part_checked(aoR, "aoR");
+    if (half_rotate) aoR_half = real_part_checked(aoR_half, "aoR_half");
   }
   const int k = thc_cfac * m;
   std::cout << " * Constructing Z matrix." << std::endl;
@@ -245,7 +250,11 @@ void QRCPSolver::kernel(ContextHandler::BlacsHandler& BH, std::vector<int>& inte
   int* d_idx = dev_alloc<int>((size_t)k);
   DevFree fi(d_idx);
   std::vector<double> rdiag(k);
-  status_ok(thc_qrcp_points(BH.ctx, d_ao, d_half, n, m, mocc, k, d_idx, rdiag.data()), "thc_qrcp_points");
+  if (cplx)
+    status_ok(thc_qrcp_points_complex(BH.ctx, d_ao, d_half, n, m, mocc, k, d_idx, rdiag.data()),
+              "thc_qrcp_points_complex");
+  else
+    status_ok(thc_qrcp_points(BH.ctx, d_ao, d_half, n, m, mocc, k, d_idx, rdiag.data()), "thc_qrcp_points");
   interp_indxs = to_host(d_idx, (size_t)k);
   const std::string prefix = half_rotate ? "half_ix" : "full_ix";
   for (int i = 0; i < k; ++i)
@@ -361,23 +370,30 @@ void IVecs::write_orbitals_at_points(const std::vector<double>& ao, int m, const
 // Half-rotated set-up (src/interpolating_vectors.cpp:143-161): the full-set and
 // occupied-set orbitals at the points are written under the
 // HalfTransformedFull / HalfTransformedOcc prefixes, and both orbital sets go
-// to the device; ZC^T = Pocc o P itself is formed inside kernel().
+// to the device (interleaved when either has a non-zero imaginary part); ZC^T
+// = Pocc o conj(P) itself is formed inside kernel().
 void IVecs::setup_CZt_half(std::vector<int>& interp_indxs, ContextHandler::BlacsHandler& BH) {
   (void)BH;
   std::vector<double> aoR, aoR_half;
   {
     thc::h5::File fin(input_file, thc::h5::File::ReadOnly);
-    aoR = real_part_checked(fin.read_complex("aoR"), "aoR");
+    aoR = fin.read_complex("aoR");
     const auto d = fin.dims("aoR_half");
     if (d.size() < 2 || (long)d[0] != ngrid) throw std::runtime_error("aoR_half must be (N, M_occ)");
     nocc = (int)d[1];
-    aoR_half = real_part_checked(fin.read_complex("aoR_half"), "aoR_half");
+    aoR_half = fin.read_complex("aoR_half");
   }
-  write_orbitals_at_points(aoR, nbasis, interp_indxs, "HalfTransformedFull");
-  write_orbitals_at_points(aoR_half, nocc, interp_indxs, "HalfTransformedOcc");
-  ao_mu = gather_rows(aoR, nbasis, interp_indxs);
-  aoh_mu = gather_rows(aoR_half, nocc, interp_indxs);
-  std::cout << " * Constructing CZt matrix" << std::endl;
+  cplx = has_imag(aoR) || has_imag(aoR_half);
+  if (!cplx) {
+    aoR = real_part_checked(aoR, "aoR");
+    aoR_half = real_part_checked(aoR_half, "aoR_half");
+  }
+  const int w = cplx ? 2 : 1;
+  write_orbitals_at_points(aoR, nbasis, interp_indxs, "HalfTransformedFull", cplx);
+  write_orbitals_at_points(aoR_half, nocc, interp_indxs, "HalfTransformedOcc", cplx);
+  ao_mu = gather_rows(aoR, w * nbasis, interp_indxs);
+  aoh_mu = gather_rows(aoR_half, w * nocc, interp_indxs);
+  std::cout << " * Constructing CZt matrix" << (cplx ? " (complex orbitals)" : "") << std::endl;
   std::cout << " * Matrix Shape: (" << nmu << ", " << ngrid << ")" << std::endl;
   d_ao = to_dev(aoR);
   d_aoh = to_dev(aoR_half);
@@ -421,6 +437,29 @@ void IVecs::setup_CCt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandl
   DevFree fr(d_rows);
   int* d_iota = to_dev(iota);
   DevFree fi(d_iota);
+  cct_dim = k;
+  if (half_rotate && cplx) {
+    // complex Hermitian CC^T: kept as its real embedding [[Re, -Im], [Im, Re]],
+    // which holds every singular value twice (check_rank halves the count)
+    double* d_rh = to_dev(aoh_mu);
+    DevFree fh(d_rh);
+    double* d_c = dev_alloc<double>(2 * (size_t)k * k);
+    DevFree fc(d_c);
+    status_ok(thc_czt_half_complex(BH.ctx, d_rows, d_rh, k, nbasis, nocc, d_iota, k, d_c), "thc_czt_half_complex");
+    const std::vector<double> c = to_host(d_c, 2 * (size_t)k * k);
+    cct_dim = 2 * k;
+    std::vector<double> e((size_t)cct_dim * cct_dim);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) {
+        const double re = c[2 * ((size_t)i * k + j)], im = c[2 * ((size_t)i * k + j) + 1];
+        e[(size_t)i * cct_dim + j] = re;
+        e[(size_t)(k + i) * cct_dim + k + j] = re;
+        e[(size_t)i * cct_dim + k + j] = -im;
+        e[(size_t)(k + i) * cct_dim + j] = im;
+      }
+    d_cct = to_dev(e);
+    return;
+  }
   d_cct = dev_alloc<double>((size_t)k * k);
   if (half_rotate) {
     double* d_rh = to_dev(aoh_mu);
@@ -438,7 +477,8 @@ void IVecs::setup_CCt(std::vector<int>& interp_indxs, ContextHandler::BlacsHandl
 void IVecs::check_rank(ContextHandler::BlacsHandler& BH) {
   std::cout << " * Checking rank of CCt matrix." << std::endl;
   int rank = 0;
-  status_ok(thc_cct_rank(BH.ctx, d_cct, nmu, &rank, nullptr), "thc_cct_rank");
+  status_ok(thc_cct_rank(BH.ctx, d_cct, cct_dim, &rank, nullptr), "thc_cct_rank");
+  rank /= cct_dim / nmu;
   if (rank < nmu) {
     std::cout << " * WARNING: CCt matrix is rank deficient. Rank = " << rank << std::endl;
     std::cout << std::endl;
@@ -464,7 +504,8 @@ void IVecs::kernel(ContextHandler::BlacsHandler& BH) {
   }
   double* d_cg = to_dev(coulG);
   DevFree fc(d_cg);
-  double* d_v = dev_alloc<double>((size_t)k * k);
+  const bool half_cplx = half_rotate && cplx;
+  double* d_v = dev_alloc<double>((size_t)k * k * (half_cplx ? 2 : 1));  // complex Muv interleaved
   DevFree fv(d_v);
   double* d_l = half_rotate ? nullptr : dev_alloc<double>((size_t)k * k);
   DevFree fl(d_l);
@@ -472,7 +513,10 @@ void IVecs::kernel(ContextHandler::BlacsHandler& BH) {
   thc_isdf_stats st = {};
   const auto t0 = std::chrono::steady_clock::now();
   int s;
-  if (half_rotate)
+  if (half_cplx)
+    s = thc_isdf_half_complex(BH.ctx, d_ao, d_aoh, n, nbasis, nocc, d_idx, k, fft_grid.data(), d_cg, &opts, d_v,
+                              nullptr, &st);
+  else if (half_rotate)
     s = thc_isdf_half(BH.ctx, d_ao, d_aoh, n, nbasis, nocc, d_idx, k, fft_grid.data(), d_cg, &opts, d_v, nullptr,
                       &st);
   else if (cplx)
@@ -505,13 +549,15 @@ void IVecs::kernel(ContextHandler::BlacsHandler& BH) {
             << ") in " << wall << " seconds." << std::endl;
   // dump_thc_data (src/interpolating_vectors.cpp:283-328)
   if (half_rotate) {
-    // Muv = IVMG^T IVG; for real interpolating vectors IVMG = conj(IVG): the
-    // matrix returned, written full with no Cholesky
-    std::vector<double> mh = to_host(d_v, (size_t)k * k);
+    // Muv = IVMG^T IVG, written full with no Cholesky: real for real
+    // interpolating vectors (IVMG = conj(IVG)), complex symmetric otherwise
+    std::vector<double> mh = to_host(d_v, (size_t)k * k * (half_cplx ? 2 : 1));
     std::cout << " * Dumping half transformed Muv data to: " << output_file << "." << std::endl;
     std::cout << std::endl;
     std::vector<double> muv(2 * (size_t)k * k, 0.0);
-    for (size_t i = 0; i < (size_t)k * k; ++i) muv[2 * i] = mh[i];
+    if (half_cplx) muv = mh;
+    else
+      for (size_t i = 0; i < (size_t)k * k; ++i) muv[2 * i] = mh[i];
     thc::h5::File fo(output_file, thc::h5::File::ReadWrite);
     fo.ensure_group("/Hamiltonian/THC");
     fo.write_complex_as_f64("/Hamiltonian/THC/" + prefix + "Muv", muv.data(), {(uint64_t)k, (uint64_t)k}, true);

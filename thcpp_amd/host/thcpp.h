@@ -107,8 +107,9 @@ class IVecs {
   std::vector<double> ao_mu, aoh_mu;  // orbital rows at the points (host), for CC^T
   // device buffers
   double* d_ao = nullptr;   // aoR (n x m real, or n x 2m interleaved complex)
-  double* d_aoh = nullptr;  // aoR_half (n x nocc), half-rotated only
-  double* d_cct = nullptr;
+  double* d_aoh = nullptr;  // aoR_half (n x nocc, or n x 2 nocc interleaved), half-rotated only
+  double* d_cct = nullptr;  // CC^T (nmu x nmu), or the real 2 nmu x 2 nmu embedding of a complex CC^T
+  int cct_dim = 0;
   int* d_idx = nullptr;
 };
 }  // namespace InterpolatingVectors
