@@ -28,6 +28,11 @@
 // so both touch every bank pair once per 32 lanes.  The transposes between
 // transforms (z-lines to y-lines, the strided x-lines of the global layout)
 // are done on the way into LDS, so no stage reads a strided line.
+//
+// The passes are bound by their VALU work (index arithmetic of the LDS
+// stages as much as the butterflies), so cubic grids of the common edges (32,
+// 64, 96) run instantiations compiled for their size: every divisor, stride,
+// twiddle step and loop bound is a constant (64^3 FP32 pass A: 1.60 -> 1.18 ms).
 #include <algorithm>
 #include <type_traits>
 
@@ -246,13 +251,45 @@ bool fits_ip(const Plan1D& pl, int nlines, int ipv) {
 }
 constexpr int UPV = 12;  // elements per thread of the in-place unpack
 
+// The plan of a transform length known at compile time (make_plan's rule:
+// radix 8, then 4, 2, 3, 5).
+constexpr Plan1D plan_of(int n) {
+  Plan1D pl{};
+  pl.n = n;
+  pl.nst = 0;
+  int m = n;
+  const int rs[5] = {8, 4, 2, 3, 5};
+  for (int i = 0; i < 5; ++i)
+    while (m % rs[i] == 0 && pl.nst < MAXR) {
+      pl.rad[pl.nst++] = rs[i];
+      m /= rs[i];
+    }
+  return pl;
+}
+
+// fft_lines_ip for a length N fixed at compile time: the stages unrolled, so
+// every index divisor, twiddle stride and loop bound is a constant.
+template <class T, int IPV, int N, int S = 0, int NS = 1>
+__device__ __forceinline__ void fft_lines_ct(C2<T>* buf, int nlines, int ls, const C2<T>* tw) {
+  constexpr Plan1D pl = plan_of(N);
+  if constexpr (S < pl.nst) {
+    constexpr int R = pl.rad[S];
+    stage_ip<T, R, IPV>(buf, N, NS, nlines, ls, tw);
+    fft_lines_ct<T, IPV, N, S + 1, NS * R>(buf, nlines, ls, tw);
+  }
+}
+
 // x: (k, n0, n1, n2) real, FP64 or FP32 (rounded to T as it is read); t: (k,
-// n0, n1, n2h) complex.
-template <class T, int IPV, class X>  // IPV = 0: ping-pong stages
+// n0, n1, n2h) complex.  CN > 0: a cubic grid of edge CN, compiled for that
+// size (in-place stages only); 0: sizes and plans from p.
+template <class T, int IPV, class X, int CN = 0>  // IPV = 0: ping-pong stages
 __global__ __launch_bounds__(FT) void pass_a(const X* __restrict__ x, C2<T>* __restrict__ t, FftParams p) {
   constexpr bool ip = IPV > 0;
+  static_assert(CN == 0 || ip, "compile-time sizes use the in-place stages");
   extern __shared__ unsigned char smem_raw[];
-  const int n1 = p.n1, n2 = p.n2, n2h = p.n2h, lsz = p.lsz, lsy = p.lsy;
+  constexpr int CLS = (CN + 31) / 32 * 32;
+  const int n1 = CN ? CN : p.n1, n2 = CN ? CN : p.n2, n2h = CN ? CN / 2 + 1 : p.n2h;
+  const int lsz = CN ? CLS : p.lsz, lsy = CN ? CLS : p.lsy;
   const int n1p = (n1 + 1) / 2;
   const int cap = max(n1p * lsz, n2h * lsy);
   C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
@@ -262,7 +299,8 @@ __global__ __launch_bounds__(FT) void pass_a(const X* __restrict__ x, C2<T>* __r
   const long mu = blockIdx.y, ix = blockIdx.x;
   for (int j = threadIdx.x; j < n2; j += blockDim.x) twz[j] = reinterpret_cast<const C2<T>*>(p.twz)[j];
   for (int j = threadIdx.x; j < n1; j += blockDim.x) twy[j] = reinterpret_cast<const C2<T>*>(p.twy)[j];
-  const X* src = x + (mu * p.n0 + ix) * (long)n1 * n2;
+  const int n0 = CN ? CN : p.n0;
+  const X* src = x + (mu * n0 + ix) * (long)n1 * n2;
   // pack rows (2r, 2r+1) as re/im; LB loads per thread are issued before the
   // first LDS store so the plane is fetched in one or two memory round trips
   constexpr int LB = 8;
@@ -290,7 +328,8 @@ __global__ __launch_bounds__(FT) void pass_a(const X* __restrict__ x, C2<T>* __r
   // written transposed: y-line kz holds A and B of every row pair
   C2<T>* fin;
   if (ip) {
-    fft_lines_ip<T, IPV>(bufa, p.pz, n1p, lsz, twz);
+    if constexpr (CN > 0) fft_lines_ct<T, IPV, CN>(bufa, n1p, lsz, twz);
+    else fft_lines_ip<T, IPV>(bufa, p.pz, n1p, lsz, twz);
     C2<T> z1[UPV], z2[UPV];
 #pragma unroll
     for (int i = 0; i < UPV; ++i) {
@@ -313,7 +352,8 @@ __global__ __launch_bounds__(FT) void pass_a(const X* __restrict__ x, C2<T>* __r
       }
     }
     __syncthreads();
-    fft_lines_ip<T, IPV>(bufa, p.py, n2h, lsy, twy);
+    if constexpr (CN > 0) fft_lines_ct<T, IPV, CN>(bufa, n2h, lsy, twy);
+    else fft_lines_ip<T, IPV>(bufa, p.py, n2h, lsy, twy);
     fin = bufa;
   } else {
     C2<T>* res = fft_lines<T>(bufa, bufb, p.pz, n1p, lsz, twz);
@@ -331,7 +371,7 @@ __global__ __launch_bounds__(FT) void pass_a(const X* __restrict__ x, C2<T>* __r
     C2<T>* other = (h == bufa) ? bufb : bufa;
     fin = fft_lines<T>(h, other, p.py, n2h, lsy, twy);
   }
-  C2<T>* dst = t + (mu * p.n0 + ix) * (long)n1 * n2h;
+  C2<T>* dst = t + (mu * n0 + ix) * (long)n1 * n2h;
   for (int e = threadIdx.x; e < n1 * n2h; e += blockDim.x) {
     const int y = fn2h.div(e), kz = e - y * n2h;
     dst[e] = fin[swz(kz, y, lsy)];
@@ -354,12 +394,14 @@ __global__ void weight_kernel(const double* __restrict__ coulG, int n1, int n2, 
 
 // t: (k, n0, n1, n2h) complex -> out (k, n0, n1, n2h, 2) weighted by wt, or
 // (DIG) its int8 digit planes (FftDigits).
-template <class T, int IPV, bool DIG>  // IPV = 0: ping-pong stages
+template <class T, int IPV, bool DIG, int CN = 0>  // IPV = 0: ping-pong stages; CN as in pass_a
 __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const T* __restrict__ wt,
                                              T* __restrict__ out, FftParams p) {
   constexpr bool ip = IPV > 0;
+  static_assert(CN == 0 || ip, "compile-time sizes use the in-place stages");
   extern __shared__ unsigned char smem_raw[];
-  const int n0 = p.n0, n1 = p.n1, n2h = p.n2h, lsx = p.lsx;
+  const int n0 = CN ? CN : p.n0, n1 = CN ? CN : p.n1, n2h = CN ? CN / 2 + 1 : p.n2h;
+  const int lsx = CN ? (CN + 31) / 32 * 32 : p.lsx;
   C2<T>* bufa = reinterpret_cast<C2<T>*>(smem_raw);
   C2<T>* bufb = ip ? bufa : bufa + n2h * lsx;
   C2<T>* twx = bufb + n2h * lsx;  // twiddle table staged in LDS
@@ -387,7 +429,8 @@ __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const 
   }
   __syncthreads();
   C2<T>* fin = bufa;
-  if (ip) fft_lines_ip<T, IPV>(bufa, p.px, n2h, lsx, twx);
+  if constexpr (CN > 0) fft_lines_ct<T, IPV, CN>(bufa, n2h, lsx, twx);
+  else if (ip) fft_lines_ip<T, IPV>(bufa, p.px, n2h, lsx, twx);
   else fin = fft_lines<T>(bufa, bufb, p.px, n2h, lsx, twx);
   if constexpr (DIG) {
     // the line's weighted values a = v.x f, b = v.y f (rounded to T as the
@@ -489,28 +532,29 @@ size_t lds_b(int n0, int n2, bool ip) {
   return (ip ? 1 : 2) * sizeof(C2<T>) * (size_t)(n2 / 2 + 1) * line_stride(n0) + sizeof(C2<T>) * (size_t)n0;
 }
 
-template <class T, int IPV, class X>
+template <class T, int IPV, class X, int CN = 0>
 int launch_a(dim3 g, size_t la, const X* x, C2<T>* tmid, const FftParams& p, hipStream_t s) {
-  THC_HIP(hipFuncSetAttribute((const void*)pass_a<T, IPV, X>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la));
-  hipLaunchKernelGGL((pass_a<T, IPV, X>), g, dim3(FT), la, s, x, tmid, p);
+  const void* fn = (const void*)pass_a<T, IPV, X, CN>;
+  THC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la));
+  hipLaunchKernelGGL((pass_a<T, IPV, X, CN>), g, dim3(FT), la, s, x, tmid, p);
   THC_HIP(hipGetLastError());
   return THC_OK;
 }
 
-template <class T, int IPV>
+template <class T, int IPV, int CN = 0>
 int launch_b(dim3 g, size_t lb, const C2<T>* tmid, const T* wt, T* out, const FftParams& p, hipStream_t s) {
   if (p.dig_s) {
     if constexpr (std::is_same<T, float>::value) {
-      THC_HIP(hipFuncSetAttribute((const void*)pass_b<T, IPV, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lb));
-      hipLaunchKernelGGL((pass_b<T, IPV, true>), g, dim3(FT), lb, s, tmid, wt, out, p);
+      const void* fn = (const void*)pass_b<T, IPV, true, CN>;
+      THC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
+      hipLaunchKernelGGL((pass_b<T, IPV, true, CN>), g, dim3(FT), lb, s, tmid, wt, out, p);
     } else {
       return THC_ERR_ARG;  // digits come from the FP32 transform only
     }
   } else {
-    THC_HIP(hipFuncSetAttribute((const void*)pass_b<T, IPV, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lb));
-    hipLaunchKernelGGL((pass_b<T, IPV, false>), g, dim3(FT), lb, s, tmid, wt, out, p);
+    const void* fn = (const void*)pass_b<T, IPV, false, CN>;
+    THC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
+    hipLaunchKernelGGL((pass_b<T, IPV, false, CN>), g, dim3(FT), lb, s, tmid, wt, out, p);
   }
   THC_HIP(hipGetLastError());
   return THC_OK;
@@ -558,13 +602,22 @@ int run(const X* x, int k, const int n[3], const double* coulG, void* out, void*
   T* o = reinterpret_cast<T*>(out);
   p.vec_out = (reinterpret_cast<uintptr_t>(out) % (2 * sizeof(T))) == 0;
   if (pass_ev) THC_HIP(hipEventRecord(pass_ev[0], s));
-  if (ipa == 16) THC_TRY((launch_a<T, 16, X>(dim3(n[0], k), la, x, tmid, p, s)));
-  else if (ipa == 24) THC_TRY((launch_a<T, 24, X>(dim3(n[0], k), la, x, tmid, p, s)));
-  else THC_TRY((launch_a<T, 0, X>(dim3(n[0], k), la, x, tmid, p, s)));
+  // cubic grids of the common edges run kernels compiled for their size
+  const int cube = (n[0] == n[1] && n[1] == n[2]) ? n[0] : 0;
+  const dim3 ga(n[0], k), gb(n[1], k);
+  if (ipa == 16 && cube == 64) THC_TRY((launch_a<T, 16, X, 64>(ga, la, x, tmid, p, s)));
+  else if (ipa == 16 && cube == 32) THC_TRY((launch_a<T, 16, X, 32>(ga, la, x, tmid, p, s)));
+  else if (ipa == 24 && cube == 96) THC_TRY((launch_a<T, 24, X, 96>(ga, la, x, tmid, p, s)));
+  else if (ipa == 16) THC_TRY((launch_a<T, 16, X>(ga, la, x, tmid, p, s)));
+  else if (ipa == 24) THC_TRY((launch_a<T, 24, X>(ga, la, x, tmid, p, s)));
+  else THC_TRY((launch_a<T, 0, X>(ga, la, x, tmid, p, s)));
   if (pass_ev) THC_HIP(hipEventRecord(pass_ev[1], s));
-  if (ipb == 16) THC_TRY((launch_b<T, 16>(dim3(n[1], k), lb, tmid, wt, o, p, s)));
-  else if (ipb == 24) THC_TRY((launch_b<T, 24>(dim3(n[1], k), lb, tmid, wt, o, p, s)));
-  else THC_TRY((launch_b<T, 0>(dim3(n[1], k), lb, tmid, wt, o, p, s)));
+  if (ipb == 16 && cube == 64) THC_TRY((launch_b<T, 16, 64>(gb, lb, tmid, wt, o, p, s)));
+  else if (ipb == 16 && cube == 32) THC_TRY((launch_b<T, 16, 32>(gb, lb, tmid, wt, o, p, s)));
+  else if (ipb == 24 && cube == 96) THC_TRY((launch_b<T, 24, 96>(gb, lb, tmid, wt, o, p, s)));
+  else if (ipb == 16) THC_TRY((launch_b<T, 16>(gb, lb, tmid, wt, o, p, s)));
+  else if (ipb == 24) THC_TRY((launch_b<T, 24>(gb, lb, tmid, wt, o, p, s)));
+  else THC_TRY((launch_b<T, 0>(gb, lb, tmid, wt, o, p, s)));
   THC_HIP(hipGetLastError());
   if (pass_ev) THC_HIP(hipEventRecord(pass_ev[2], s));
   return THC_OK;
