@@ -451,19 +451,24 @@ __global__ __launch_bounds__(256) void sv_split_zt(const double* __restrict__ zt
   }
 }
 
-// s[r] = sqrt(sum_i a[r][i]^2 * sum_i b[r][i]^2) (b = null: s[r] = sum_i a[r][i]^2)
+// s[r] = sqrt(sum_i a[r][i]^2 * sum_i b[r][i]^2) (b = null: s[r] = sum_i a[r][i]^2).
+// One wave per row at a time: the lanes read consecutive entries of the row
+// (coalesced) and the sums are combined across the wave.
+__device__ inline double sv_row_sq(const double* __restrict__ row, int m, int lane) {
+  double v = 0.0;
+  for (int i = lane; i < m; i += 64) v = fma(row[i], row[i], v);
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
 __global__ __launch_bounds__(256) void sv_scale(const double* __restrict__ a, int ma, const double* __restrict__ b,
                                                 int mb, long n, double* __restrict__ s) {
-  for (long r = blockIdx.x * 256L + threadIdx.x; r < n; r += 256L * gridDim.x) {
-    double ra = 0.0;
-    for (int i = 0; i < ma; ++i) ra += a[r * ma + i] * a[r * ma + i];
-    if (b) {
-      double rb = 0.0;
-      for (int i = 0; i < mb; ++i) rb += b[r * mb + i] * b[r * mb + i];
-      s[r] = sqrt(ra * rb);
-    } else {
-      s[r] = ra;
-    }
+  const int lane = threadIdx.x & 63;
+  const long waves = (long)gridDim.x * 4;
+  for (long r = (long)blockIdx.x * 4 + (threadIdx.x >> 6); r < n; r += waves) {
+    const double ra = sv_row_sq(a + r * ma, ma, lane);
+    double v = ra;
+    if (b) v = sqrt(ra * sv_row_sq(b + r * mb, mb, lane));
+    if (lane == 0) s[r] = v;
   }
 }
 
