@@ -37,7 +37,7 @@ OZ_V_DIGITS = 5                      # digits per value of the int8 V (written b
 # FETCH_SIZE / WRITE_SIZE counters scaled by the same counters on a 1 GiB device copy (fetch x 2.0,
 # write x 1.0); the summary with its commands is profiles/isdf_diamond64_n64_k2048_hbm_traffic.txt.
 # Other workloads report null.
-SOLVE_TRAFFIC_BYTES = {"diamond64_n64_k2048": 18202105 * 1024 * 2.0 + 2097152 * 1024 * 1.0}
+SOLVE_TRAFFIC_BYTES = {"diamond64_n64_k2048": 18916036 * 1024 * 2.0 + 2097152 * 1024 * 1.0}
 METRIC = "ISDF factorizations/s + % roofline; mixed vs FP64 GPU at <=1e-6 rel V error"
 STAGES = ["zct", "cct_chol_inv", "solve", "fft", "V", "luv", "isdf_total", "refine", "solve_gemm_kernel",
           "V_kernel", "fft_pass_a", "fft_pass_b"]
