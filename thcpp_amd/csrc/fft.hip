@@ -133,6 +133,31 @@ __device__ inline void butterfly(C2<T> (&x)[R], C2<T> (&y)[R], int n, int ns, in
     y[6] = csub(e[2], w2);
     y[3] = cadd(e[3], w3);
     y[7] = csub(e[3], w3);
+  } else if (R == 3) {
+    // w = e^{-2 pi i / 3} = -1/2 - i sqrt(3)/2: y1,2 = x0 - t1/2 -/+ i sqrt(3)/2 t2
+    const T s3 = T(0.86602540378443864676);
+    const C2<T> t1 = cadd(x[1], x[2]), t2 = csub(x[1], x[2]);
+    const C2<T> m = C2<T>{x[0].x - T(0.5) * t1.x, x[0].y - T(0.5) * t1.y};
+    y[0] = cadd(x[0], t1);
+    y[1] = C2<T>{m.x + s3 * t2.y, m.y - s3 * t2.x};
+    y[2] = C2<T>{m.x - s3 * t2.y, m.y + s3 * t2.x};
+  } else if (R == 5) {
+    // w = e^{-2 pi i / 5} = c1 - i s1, w^2 = c2 - i s2; with a1,2 = x1 + x4, x2 + x3
+    // and b1,2 = x1 - x4, x2 - x3: y1,4 = x0 + c1 a1 + c2 a2 -/+ i (s1 b1 + s2 b2),
+    // y2,3 = x0 + c2 a1 + c1 a2 -/+ i (s2 b1 - s1 b2)
+    const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+    const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);
+    const C2<T> a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
+    const C2<T> a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
+    y[0] = cadd(x[0], cadd(a1, a2));
+    const C2<T> m1 = C2<T>{x[0].x + c1 * a1.x + c2 * a2.x, x[0].y + c1 * a1.y + c2 * a2.y};
+    const C2<T> m2 = C2<T>{x[0].x + c2 * a1.x + c1 * a2.x, x[0].y + c2 * a1.y + c1 * a2.y};
+    const C2<T> n1 = C2<T>{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y};
+    const C2<T> n2 = C2<T>{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y};
+    y[1] = C2<T>{m1.x + n1.y, m1.y - n1.x};
+    y[4] = C2<T>{m1.x - n1.y, m1.y + n1.x};
+    y[2] = C2<T>{m2.x + n2.y, m2.y - n2.x};
+    y[3] = C2<T>{m2.x - n2.y, m2.y + n2.x};
   } else {
     const int step = n / R;
 #pragma unroll
