@@ -92,7 +92,8 @@ def _half_spectrum_ref(x, n, coulg):
 
 @pytest.mark.parametrize("grid", [(8, 8, 8), (12, 10, 9), (16, 16, 16), (32, 32, 32), (24, 20, 30),
                                   (27, 25, 45), (64, 64, 64), (5, 3, 2), (1, 4, 6), (96, 96, 96), (48, 45, 40),
-                                  (24, 24, 24), (36, 36, 36), (40, 40, 40), (48, 48, 48)])
+                                  (24, 24, 24), (25, 25, 25), (27, 27, 27), (30, 30, 30), (36, 36, 36),
+                                  (40, 40, 40), (45, 45, 45), (48, 48, 48)])
 def test_fft_vectors(ctx, grid):
     from thcpp_amd import synthetic
     rng = np.random.default_rng(sum(grid))

@@ -592,14 +592,15 @@ int launch_b(dim3 g, size_t lb, const C2<T>* tmid, const T* wt, T* out, const Ff
 
 // Cubic grids whose edges have kernels compiled for their size (values per
 // thread, edge): the 32^3, 64^3 and 96^3 grids, and the 24-48 point edges of
-// small cells.  A grid of another shape, or whose per-thread budget differs
+// small cells (products of 2, 3 and 5).  A grid of another shape, or whose per-thread budget differs
 // from the one listed, runs the kernels with run-time plans.
 template <int IPV, int N>
 struct CtEdge {
   static constexpr int ipv = IPV, n = N;
 };
-using CtEdges = std::tuple<CtEdge<16, 64>, CtEdge<24, 96>, CtEdge<16, 32>, CtEdge<16, 48>, CtEdge<16, 40>,
-                           CtEdge<16, 36>, CtEdge<16, 24>>;
+using CtEdges = std::tuple<CtEdge<16, 64>, CtEdge<24, 96>, CtEdge<16, 32>, CtEdge<16, 48>, CtEdge<16, 45>,
+                           CtEdge<16, 40>, CtEdge<16, 36>, CtEdge<16, 30>, CtEdge<16, 27>, CtEdge<16, 25>,
+                           CtEdge<16, 24>>;
 
 template <class T, class X, class... E>
 bool launch_a_ct(std::tuple<E...>*, int ipa, int cube, dim3 g, size_t la, const X* x, C2<T>* tmid,
