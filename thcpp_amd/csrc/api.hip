@@ -2,6 +2,7 @@
 // kernel at one rank, src/interpolating_vectors.cpp:17-57 and 406-455) and the
 // k-means entry points (kmeans.hip).
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -1025,8 +1026,12 @@ int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isd
     sys[i].kstats = kstats[i];
     sys[i].status = kst[i];
   }
-  // 2. the ISDF stages, spread largest first over up to max_streams lanes (the
-  //    context and up to 15 more it keeps), each driven by its own host thread
+  // 2. the ISDF stages on up to max_streams lanes (the context and up to 15
+  //    more it keeps), each driven by its own host thread.  The systems are
+  //    ordered largest first and a lane takes the next one whenever its last
+  //    one is done (list scheduling: the small systems, whose Cholesky and
+  //    FFT chains are latency-bound, fill in where the lanes would otherwise
+  //    end unevenly)
   const int nl = std::min(std::min(max_streams > 0 ? max_streams : 4, nsys), 16);
   while ((int)ctx->lanes.size() < nl - 1) {
     thc_ctx* c = nullptr;
@@ -1043,20 +1048,33 @@ int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isd
     cost[i] = (double)sys[i].n * k * (k + sys[i].m);
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-  std::vector<std::vector<int>> plan(nl);
-  std::vector<double> load(nl, 0.0);
+  // the largest workspace any system of the batch needs: every lane reserves
+  // it up front where it fits, so that no lane frees and reallocates its arena
+  // in the middle of the batch (hipFree waits for the other lanes' kernels)
+  size_t maxb = 0;
+  const int prec = opts ? opts->precision : THC_PREC_FP64;
+  const int refine = opts ? std::max(0, opts->refine_steps) : 0;
+  const bool rank = opts && opts->check_rank;
   for (int i : order) {
-    const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    plan[j].push_back(i);
-    load[j] += cost[i];
+    const thc_system& q = sys[i];
+    const int k = q.thc_cfac * q.m;
+    if (prec == THC_PREC_MIXED && k > 4096) continue;  // refused by isdf_run
+    maxb = std::max(maxb, isdf_bytes(q.n, k, q.m, 0, false, false, q.fft_grid, prec, refine, true,
+                                     rank));
   }
+  std::atomic<size_t> next{0};
   std::vector<std::string> errs(nsys);
   for (int i = 0; i < nsys; ++i)
     if (sys[i].status != THC_OK && sys[i].status < 100) errs[i] = "k-means of this system failed";
   auto work = [&](int lane) {
     thc_ctx* c = lane == 0 ? ctx : ctx->lanes[lane - 1];
     const bool dev_ok = hipSetDevice(c->device) == hipSuccess;
-    for (int i : plan[lane]) {
+    // a reservation that does not fit leaves the lane to reserve per system
+    if (dev_ok && c->ws.reserve(maxb) != THC_OK) (void)hipGetLastError();
+    for (;;) {
+      const size_t t = next.fetch_add(1);
+      if (t >= order.size()) break;
+      const int i = order[t];
       thc_system& q = sys[i];
       if (!dev_ok) {
         q.status = THC_ERR_HIP;
