@@ -383,10 +383,12 @@ typedef struct thc_system {
  * system per thcpp.x invocation, src/main.cpp:18-77; this is that loop).
  * The k-means of all systems run together on the context's stream, every
  * kernel launch covering one step of every system (kstats.elapsed_ms is the
- * time of that whole batched k-means); the ISDF stages are then spread,
- * largest first, over up to max_streams HIP streams (<= 0: 4; at most 16),
- * each with its own workspace and driven by its own host thread, so that
- * small systems overlap on the GPU.  Each system's results equal
+ * time of that whole batched k-means); the ISDF stages then run on up to
+ * max_streams HIP streams (<= 0: 4; at most 16), each with its own workspace
+ * and driven by its own host thread, which takes the next system, largest
+ * first, whenever its last one is done, so that small systems overlap on the
+ * GPU.  Every stream first reserves the largest workspace of the batch when
+ * the device has room for max_streams of them.  Each system's results equal
  * thc_factorize's on the same inputs.  Returns THC_OK when every system
  * returned THC_OK, else the first error status by system order (or, without
  * errors, the first warning); per-system statuses are in systems[i].status. */
