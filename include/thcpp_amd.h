@@ -387,8 +387,8 @@ typedef struct thc_system {
  * max_streams HIP streams (<= 0: 4; at most 16), each with its own workspace
  * and driven by its own host thread, which takes the next system, largest
  * first, whenever its last one is done, so that small systems overlap on the
- * GPU.  Every stream first reserves the largest workspace of the batch when
- * the device has room for max_streams of them.  Each system's results equal
+ * GPU.  Every stream first tries to reserve the largest workspace of the
+ * batch (one whose reservation fails reserves per system instead).  Each system's results equal
  * thc_factorize's on the same inputs.  Returns THC_OK when every system
  * returned THC_OK, else the first error status by system order (or, without
  * errors, the first warning); per-system statuses are in systems[i].status. */
