@@ -1068,19 +1068,17 @@ int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isd
     if (sys[i].status != THC_OK && sys[i].status < 100) errs[i] = "k-means of this system failed";
   auto work = [&](int lane) {
     thc_ctx* c = lane == 0 ? ctx : ctx->lanes[lane - 1];
-    const bool dev_ok = hipSetDevice(c->device) == hipSuccess;
+    if (hipSetDevice(c->device) != hipSuccess) {
+      (void)hipGetLastError();
+      return;  // this lane takes no system; the others run them
+    }
     // a reservation that does not fit leaves the lane to reserve per system
-    if (dev_ok && c->ws.reserve(maxb) != THC_OK) (void)hipGetLastError();
+    if (c->ws.reserve(maxb) != THC_OK) (void)hipGetLastError();
     for (;;) {
       const size_t t = next.fetch_add(1);
       if (t >= order.size()) break;
       const int i = order[t];
       thc_system& q = sys[i];
-      if (!dev_ok) {
-        q.status = THC_ERR_HIP;
-        errs[i] = "hipSetDevice failed in a batch worker";
-        continue;
-      }
       const int k = q.thc_cfac * q.m;
       const int st2 = isdf_run(c, q.aoR, nullptr, q.n, q.m, 0, false, q.interp_idx, k, q.fft_grid, q.coulG, opts,
                                q.v, q.luv, nullptr, &q.istats);
@@ -1093,11 +1091,16 @@ int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isd
     try {
       threads.emplace_back(work, lane);
     } catch (...) {
-      work(lane);  // no thread available: run this lane's share here
+      work(lane);  // no thread available: this lane takes systems from the list here
     }
   }
   work(0);
   for (auto& t : threads) t.join();
+  // systems no lane took (every lane failed to select the device)
+  for (size_t t = next.load(); t < order.size(); ++t) {
+    sys[order[t]].status = THC_ERR_HIP;
+    errs[order[t]] = "no batch worker could select the device";
+  }
   int first_warn = THC_OK;
   for (int i = 0; i < nsys; ++i) {
     if (sys[i].status != THC_OK && sys[i].status < 100) {
