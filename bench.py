@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg (and the errors against it)")
     p.add_argument("--no-batch", action="store_true", help="skip the 32-cell batched leg and the 8-atom cell leg")
     p.add_argument("--no-variants", action="store_true", help="skip the FP32 and refinement legs")
+    p.add_argument("--batch-only", action="store_true",
+                   help="run only the mixed 32-cell batch on 4 streams (--steps batches after --warmup)")
     p.add_argument("--copy-cal", action="store_true",
                    help="only run a 1 GiB float64 device copy three times (the known-size kernel that "
                         "FETCH_SIZE / WRITE_SIZE counters are scaled against) and exit")
@@ -124,6 +126,39 @@ def rel_rows(dev, host):
     return (num / den) ** 0.5
 
 
+def batch_systems(ctx):
+    """The uneven 32-cell batch, its inputs resident on the GPU."""
+    from thcpp_amd import synthetic
+    batch = synthetic.make_batch(count=32, seed=0, max_it=20)
+    systems = [{"grid": ctx.f64(s.grid), "rho": ctx.f64(s.rho), "aoR": ctx.f64(s.aoR), "coulG": ctx.f64(s.coulG),
+                "fft_grid": s.n, "thc_cfac": s.thc_cfac, "max_it": s.max_it, "seed": s.seed} for s in batch]
+    return batch, systems
+
+
+def batch_only(steps, warmup):
+    """Only the mixed-precision 32-cell batch on 4 streams (for profiling the
+    batched path by itself): one JSON line with its time per batch."""
+    import torch
+    from thcpp_amd import _lib as L
+    from thcpp_amd.api import Context
+    ctx = Context(0)
+    batch, systems = batch_systems(ctx)
+    for _ in range(warmup):
+        ctx.factorize_batched(systems, precision=L.PREC_MIXED, max_streams=4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st, _res = ctx.factorize_batched(systems, precision=L.PREC_MIXED, max_streams=4)
+        if st != L.THC_OK:
+            raise RuntimeError("batched factorisation status %d" % st)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    print(json.dumps({"workload": "batch of 32 uneven diamond cells, 20 k-means iterations each, mixed, 4 streams",
+                      "steps": steps, "warmup": warmup, "ms_per_batch": round(dt * 1e3, 3),
+                      "systems_per_s": round(len(batch) / dt, 2)}))
+    return 0
+
+
 def batched_leg(ctx, steps, warmup):
     """The uneven 32-cell batch (side 6-14 bohr, 8-64 atoms, grids 24^3-48^3,
     N_mu = 8-10 M) through thc_factorize_batched, mixed precision, on 4
@@ -132,10 +167,7 @@ def batched_leg(ctx, steps, warmup):
     baseline of the batched speed-up."""
     import torch
     from thcpp_amd import _lib as L
-    from thcpp_amd import synthetic
-    batch = synthetic.make_batch(count=32, seed=0, max_it=20)
-    systems = [{"grid": ctx.f64(s.grid), "rho": ctx.f64(s.rho), "aoR": ctx.f64(s.aoR), "coulG": ctx.f64(s.coulG),
-                "fft_grid": s.n, "thc_cfac": s.thc_cfac, "max_it": s.max_it, "seed": s.seed} for s in batch]
+    batch, systems = batch_systems(ctx)
     out = {"workload": "batch of 32 uneven diamond cells, 20 k-means iterations each",
            "systems": len(batch), "sum_N": int(sum(s.N for s in batch)),
            "sum_N_mu": int(sum(s.K for s in batch))}
@@ -243,6 +275,8 @@ def main():
         print("bench.py measures one GPU only (the ISDF path has no multi-GPU sharding); run with --gpus 1",
               file=sys.stderr)
         return 2
+    if a.batch_only:
+        return batch_only(a.steps, a.warmup)
     import torch
     from thcpp_amd import _lib as L
     from thcpp_amd import synthetic
