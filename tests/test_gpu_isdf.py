@@ -229,12 +229,13 @@ def test_isdf_mixed_int8_cfg0(ctx, cfg0, cfg0_kmeans):
     np.testing.assert_array_equal(v, v.T)
 
 
-@pytest.mark.parametrize("n,m,k", [((15, 12, 10), 9, 37), ((40, 36, 30), 24, 150)])
+@pytest.mark.parametrize("n,m,k", [((15, 12, 10), 9, 37), ((40, 36, 30), 24, 150), ((72, 6, 80), 24, 150)])
 def test_isdf_mixed_uneven_sizes(ctx, n, m, k):
     """K not a multiple of the 128 tile (and spanning two tiles), grids with odd
-    edges, spectrum length below one 8192-column chunk and over five chunks with
-    a partial last one (2 * 40 * 36 * 16 = 46080), grid lengths that are not a
-    multiple of the 64-column solve tile; against the oracle."""
+    edges, grid lengths that are not a multiple of the 64-column solve tile, and
+    a y-line of the weighted spectrum (2 * 72 * 41 = 5904 values, 5952 with its
+    padding) longer than one chunk of the int8 V, so cut into three chunks that
+    share the line's exponent; against the oracle."""
     from thcpp_amd import synthetic
     rng = np.random.default_rng(7)
     N = int(np.prod(n))

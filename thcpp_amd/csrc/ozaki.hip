@@ -59,7 +59,7 @@ typedef __attribute__((address_space(3))) void lds_void;
 __device__ inline int oz_seg(int r, int s) { return (s + (r >> 2)) & 3; }
 
 __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S, long g2, long g2p, int k,
-                                               const int* __restrict__ E, int nchunk, int ch,
+                                               const int* __restrict__ E, int nchunk, int ch, int spl,
                                                double* __restrict__ work, long ntiles) {
   // one variable per buffer: the compiler then knows that the direct loads
   // into one buffer do not alias the fragment reads of the other, and does not
@@ -168,7 +168,9 @@ __global__ __launch_bounds__(512) void oz_syrk(const signed char* __restrict__ S
           double sum = 0.0;
 #pragma unroll
           for (int d = OZ_S - 1; d >= 0; --d) sum += ldexp((double)acc[d][i][j][v], -7 * (d + 2));
-          val = ldexp(sum, E[(long)r * nchunk + c] + E[(long)col * nchunk + c]);
+          // exponents are per (row, y-line); a y-line is spl consecutive chunks
+          const int nl = nchunk / spl, ln = c / spl;
+          val = ldexp(sum, E[(long)r * nl + ln] + E[(long)col * nl + ln]);
         }
         W[rl * OZ_T + cl] = val;
       }
@@ -241,25 +243,40 @@ int oz_line_len(const int n[3]) {
   return (int)((len + OZ_KS - 1) / OZ_KS * OZ_KS);
 }
 
-static void oz_lines_plan(int k, const int n[3], int& lp, long& g2p, long& ntiles) {
+// Chunks per y-line: the fewest that divide the line into whole k-steps of at
+// most OZ_CH bytes.  The workgroups of one XCD run the tiles of one chunk at a
+// time, and a 128-row slab of a chunk (5 planes) of up to OZ_CH = 4352 bytes,
+// 2.8 MB, stays in the XCD's 4 MB L2 for the tiles that share it; a 96-point
+// grid's 9408-byte line (6 MB slabs) did not, and V ran at 0.43 of the int8
+// peak against 0.51 for 64-point lines.
+constexpr int OZ_CH = 4352;
+static int oz_chunks_per_line(int lp) {
+  const int steps = lp / OZ_KS;
+  for (int d = 1; d <= steps; ++d)
+    if (steps % d == 0 && lp / d <= OZ_CH) return d;
+  return steps;
+}
+
+static void oz_lines_plan(int k, const int n[3], int& lp, long& g2p, long& ntiles, int& spl) {
   lp = oz_line_len(n);
   g2p = (long)n[1] * lp;
+  spl = oz_chunks_per_line(lp);
   const long nb = (k + OZ_T - 1) / OZ_T;
   ntiles = nb * (nb + 1) / 2;
 }
 
 size_t oz_lines_work_bytes(int k, const int n[3]) {
-  int lp;
+  int lp, spl;
   long g2p, ntiles;
-  oz_lines_plan(k, n, lp, g2p, ntiles);
+  oz_lines_plan(k, n, lp, g2p, ntiles, spl);
   return align_up((size_t)OZ_S * k * g2p, 256) + align_up(sizeof(int) * (size_t)k * n[1], 256) +
-         align_up(sizeof(double) * (size_t)n[1] * ntiles * OZ_T * OZ_T, 256) + 256;
+         align_up(sizeof(double) * (size_t)n[1] * spl * ntiles * OZ_T * OZ_T, 256) + 256;
 }
 
 FftDigits oz_lines_layout(int k, const int n[3], void* work) {
-  int lp;
+  int lp, spl;
   long g2p, ntiles;
-  oz_lines_plan(k, n, lp, g2p, ntiles);
+  oz_lines_plan(k, n, lp, g2p, ntiles, spl);
   char* w = reinterpret_cast<char*>(work);
   FftDigits d;
   d.s = reinterpret_cast<signed char*>(w);
@@ -274,18 +291,18 @@ int oz_syrk_lines(const FftDigits& d, int k, const int n[3], double* v, long ldv
                   hipEvent_t e0, hipEvent_t e1) {
   if (k <= 0) return THC_OK;
   static_assert((long)OZ_S * 26624 * 127 * 127 < 2147483647L, "int32 sums of a 128-edge grid's line");
-  int lp;
+  int lp, spl;
   long g2p, ntiles;
-  oz_lines_plan(k, n, lp, g2p, ntiles);
+  oz_lines_plan(k, n, lp, g2p, ntiles, spl);
   if (lp != d.lp || g2p != d.g2p || lp > 26624) return THC_ERR_ARG;
-  const int nchunk = n[1];
+  const int nchunk = n[1] * spl, ch = lp / spl;
   double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + align_up((size_t)OZ_S * k * g2p, 256) +
-                                           align_up(sizeof(int) * (size_t)k * nchunk, 256));
+                                           align_up(sizeof(int) * (size_t)k * n[1], 256));
   const long slots = (ntiles * nchunk + 7) / 8 * 8;  // every w < ntiles * nchunk gets one workgroup
   if (slots > 0x7fffffffL) return THC_ERR_ARG;
   if (e0) THC_HIP(hipEventRecord(e0, s));
   // every column of a chunk is data or zero padding: g2 = g2p
-  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, d.s, g2p, g2p, k, d.e, nchunk, lp, part,
+  hipLaunchKernelGGL(oz_syrk, dim3((unsigned)slots), dim3(512), 0, s, d.s, g2p, g2p, k, d.e, nchunk, ch, spl, part,
                      ntiles);
   THC_HIP(hipGetLastError());
   if (e1) THC_HIP(hipEventRecord(e1, s));
