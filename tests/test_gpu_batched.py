@@ -89,6 +89,25 @@ def test_batched_error_isolated(ctx, batch):
         assert torch.equal(outs[i]["V"], one["V"])
 
 
+def test_batched_isdf_error_isolated(ctx, batch):
+    """A system whose FFT grid does not hold its N points passes the batched
+    k-means and fails in its ISDF stages on a lane (THC_ERR_ARG); the lanes go
+    on taking the other systems, which equal their single factorisations."""
+    systems = [as_dict(s) for s in batch[:4]]
+    bad = dict(systems[2])
+    n = tuple(bad["fft_grid"])
+    bad["fft_grid"] = (n[0], n[1], n[2] + 1)
+    systems[2] = bad
+    st, outs = ctx.factorize_batched(systems, precision=L.PREC_MIXED, max_streams=4)
+    assert st == L.THC_ERR_ARG
+    assert outs[2]["status"] == L.THC_ERR_ARG
+    for i in (0, 1, 3):
+        s = batch[i]
+        assert outs[i]["status"] == L.THC_OK
+        one = ctx.factorize(s.grid, s.rho, s.aoR, s.thc_cfac, s.max_it, s.n, s.coulG, precision=L.PREC_MIXED)
+        assert torch.equal(outs[i]["V"], one["V"])
+
+
 def test_batched_empty(ctx):
     st, outs = ctx.factorize_batched([])
     assert st == L.THC_OK and outs == []
