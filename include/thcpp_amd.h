@@ -383,7 +383,9 @@ typedef struct thc_system {
  * system per thcpp.x invocation, src/main.cpp:18-77; this is that loop).
  * The k-means of all systems run together on the context's stream, every
  * kernel launch covering one step of every system (kstats.elapsed_ms is the
- * time of that whole batched k-means); the ISDF stages then run on up to
+ * time of that whole batched run; with max_streams = 1 the largest system's
+ * k-means runs first and the rest in a second batched run on another stream,
+ * beside the largest system's ISDF stages).  The ISDF stages run on up to
  * max_streams HIP streams (<= 0: 4; at most 16), each with its own workspace
  * and driven by its own host thread, which takes the next system, largest
  * first, whenever its last one is done, so that small systems overlap on the

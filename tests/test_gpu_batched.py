@@ -71,15 +71,18 @@ def test_batched_v_matches_oracle(ctx, batch, prec):
         assert e < e64 + (1e-6 if prec == L.PREC_MIXED else 1e-12)
 
 
-def test_batched_error_isolated(ctx, batch):
+@pytest.mark.parametrize("streams", [4, 1])
+def test_batched_error_isolated(ctx, batch, streams):
     """A system with thc_cfac = 0 (no interpolation points) fails with
     THC_ERR_ARG; the rest are still factorised and the call returns that
-    system's status."""
+    system's status.  On one stream the failing system is in the second
+    k-means run (all but the largest system), which goes on beside the
+    largest system's ISDF stages."""
     systems = [as_dict(s) for s in batch[:3]]
     bad = dict(systems[1])
     bad["thc_cfac"] = 0
     systems[1] = bad
-    st, outs = ctx.factorize_batched(systems, precision=L.PREC_FP64)
+    st, outs = ctx.factorize_batched(systems, precision=L.PREC_FP64, max_streams=streams)
     assert st == L.THC_ERR_ARG
     assert outs[1]["status"] == L.THC_ERR_ARG
     for i in (0, 2):

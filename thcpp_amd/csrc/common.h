@@ -68,6 +68,7 @@ struct thc_ctx {
   void* pinned = nullptr;  // small pinned host buffer for status read-back
   size_t pinned_bytes = 0;
   std::vector<thc_ctx*> lanes;  // extra stream contexts for batched calls (owned)
+  thc_ctx* kaux = nullptr;      // batched calls: the second k-means run (owned)
 };
 
 namespace thc {

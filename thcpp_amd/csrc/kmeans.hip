@@ -15,6 +15,7 @@
 //    dropped only if its distance bound exceeds the best bound by a relative
 //    1e-8 on squares, far above the few-ulp rounding of any distance.
 #pragma clang fp contract(off)
+#include <atomic>
 #include <cfloat>
 #include <climits>
 #include <vector>
@@ -1063,7 +1064,7 @@ __global__ void collect_m(const Job* jobs, char* out, long stride, int max_it) {
 
 }  // namespace km
 
-static int km_attrs_done = 0;
+static std::atomic<int> km_attrs_done{0};  // batched calls run two k-means at once
 static int km_set_attrs() {
   if (km_attrs_done) return THC_OK;
   THC_HIP(hipFuncSetAttribute((const void*)km::diff_control, hipFuncAttributeMaxDynamicSharedMemorySize,
