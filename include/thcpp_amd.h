@@ -116,7 +116,11 @@ typedef struct thc_isdf_stats {
  * above) or a hipStream_t to use. */
 int thc_ctx_create(int device, void *stream, thc_ctx **out);
 int thc_ctx_destroy(thc_ctx *ctx);
-/* Use HIP graphs for the k-means iteration loop (default 1). */
+/* Capture the single-system k-means iteration loop once as a HIP graph and
+ * replay it (default 0).  Measured on MI355X, replaying the graph saved no
+ * time and slowed the later work on the context's stream: the 32-cell batch
+ * on a context that had run graph-replayed k-means took 67-69 ms, 59-63 ms
+ * without graphs. */
 int thc_ctx_set_graphs(thc_ctx *ctx, int enable);
 const char *thc_status_string(int status);
 /* Text of the last HIP error seen on this thread. */
@@ -381,15 +385,15 @@ typedef struct thc_system {
 
 /* Factorise many independent systems of uneven sizes (the reference runs one
  * system per thcpp.x invocation, src/main.cpp:18-77; this is that loop).
- * The k-means of all systems run together on the context's stream, every
- * kernel launch covering one step of every system (kstats.elapsed_ms is the
- * time of that whole batched run; with max_streams = 1 the largest system's
- * k-means runs first and the rest in a second batched run on another stream,
- * beside the largest system's ISDF stages).  The ISDF stages run on up to
- * max_streams HIP streams (<= 0: 4; at most 16), each with its own workspace
- * and driven by its own host thread, which takes the next system, largest
- * first, whenever its last one is done, so that small systems overlap on the
- * GPU.  Every stream first tries to reserve the largest workspace of the
+ * The k-means run as two batched runs, every kernel launch covering one step
+ * of every system of its run (kstats.elapsed_ms is the time of the system's
+ * whole run): the max_streams largest systems on the context's stream, the
+ * rest on another stream beside their ISDF stages.  The ISDF stages run on up
+ * to max_streams HIP streams (<= 0: 4; at most 16), each with its own
+ * workspace and driven by its own host thread, which takes the next system,
+ * largest first, whenever its last one is done (a stream reaching the systems
+ * of the second k-means run waits for it), so that small systems overlap on
+ * the GPU.  Every stream first tries to reserve the largest workspace of the
  * batch (one whose reservation fails reserves per system instead).  Each system's results equal
  * thc_factorize's on the same inputs.  Returns THC_OK when every system
  * returned THC_OK, else the first error status by system order (or, without

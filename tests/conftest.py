@@ -23,9 +23,9 @@ def ctx():
 
 
 @pytest.fixture(scope="session")
-def ctx_eager():
+def ctx_graph():
     from thcpp_amd.api import Context
-    c = Context(0, graphs=False)
+    c = Context(0, graphs=True)
     yield c
     c.close()
 

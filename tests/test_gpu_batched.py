@@ -77,7 +77,8 @@ def test_batched_error_isolated(ctx, batch, streams):
     THC_ERR_ARG; the rest are still factorised and the call returns that
     system's status.  On one stream the failing system is in the second
     k-means run (all but the largest system), which goes on beside the
-    largest system's ISDF stages."""
+    largest system's ISDF stages; on four streams (three systems) there is
+    no second run."""
     systems = [as_dict(s) for s in batch[:3]]
     bad = dict(systems[1])
     bad["thc_cfac"] = 0

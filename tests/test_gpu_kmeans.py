@@ -119,11 +119,11 @@ def test_kmeans_maps_per_iteration(ctx, cfg0, cfg0_kmeans, iters):
     np.testing.assert_array_equal(out["diffs"], ref["diffs"][:iters])
 
 
-def test_kmeans_graph_vs_eager_and_replay(ctx, ctx_eager, cfg0, cfg0_kmeans):
+def test_kmeans_graph_vs_eager_and_replay(ctx, ctx_graph, cfg0, cfg0_kmeans):
     s = cfg0
-    a = ctx.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, 0)
-    b = ctx.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, 0)  # graph replay
-    c = ctx_eager.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, 0)
+    a = ctx_graph.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, 0)
+    b = ctx_graph.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, 0)  # graph replay
+    c = ctx.kmeans(s.grid, s.rho, s.K, s.max_it, -1.0, 0)
     for o in (a, b, c):
         np.testing.assert_array_equal(o["interp_idx"].cpu().numpy(), cfg0_kmeans["interp_idx"])
         np.testing.assert_array_equal(o["diffs"], cfg0_kmeans["diffs"])

@@ -46,13 +46,14 @@ def _dev(t, dtype, device):
 class Context:
     """One thc_ctx on one GPU (device index 'device')."""
 
-    def __init__(self, device: int = 0, graphs: bool = True, stream=None):
+    def __init__(self, device: int = 0, graphs: bool = False, stream=None):
         """stream: a torch.cuda.Stream or raw hipStream_t handle for the library
         to run on.  By default it is torch's current stream on the device; when
         that is the legacy default stream (handle 0) the library creates a
         blocking stream of its own, which HIP orders after torch's work there.
         The first context of a process installs the committed GEMM tuning
-        table (apply_tuning)."""
+        table (apply_tuning).  graphs: replay the single-system k-means loop
+        as a HIP graph (thc_ctx_set_graphs; off by default)."""
         if not _tuning_applied[0]:
             apply_tuning()
         self.device = torch.device("cuda", device)

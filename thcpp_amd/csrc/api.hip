@@ -1038,16 +1038,13 @@ int thc_factorize_batched(thc_ctx* ctx, int nsys, thc_system* sys, const thc_isd
     c->use_graphs = ctx->use_graphs;
     ctx->lanes.push_back(c);
   }
-  // The k-means run as one batched run (each launch covering one step of
-  // every system) on the context's stream, or, with a single lane, as two:
-  // the largest system first, the rest on an auxiliary context from a thread
-  // of its own, beside the largest system's ISDF stages (the lane waits for
-  // that second run when it reaches its systems).  Each system's k-means is
-  // the same either way.  On one lane the split shortens the 32-cell batch by
-  // 2-3 ms; on four lanes it measured 2-5 ms faster in a fresh context but
-  // 1-3 ms slower in one that had run large factorisations before, so four
-  // lanes keep the single run.
-  const int na = nl == 1 && nsys > 1 ? 1 : nsys;
+  // The k-means run as two batched runs (each launch covering one step of
+  // every system of its run): the nl largest systems on the context's stream,
+  // the rest on an auxiliary context from a thread of its own.  The lanes
+  // start on the largest systems as soon as their run is done; a lane reaching
+  // the rest waits for the second run, which has meanwhile gone on beside the
+  // first systems' ISDF stages.  Each system's k-means is the same either way.
+  const int na = nsys > nl ? nl : nsys;
   std::vector<int> ga(order.begin(), order.begin() + na), gb(order.begin() + na, order.end());
   auto run_km = [&](thc_ctx* c, const std::vector<int>& ids) -> int {
     std::vector<KmSpec> sp;

@@ -56,7 +56,7 @@ struct thc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  bool use_graphs = true;
+  bool use_graphs = false;  // k-means loop as a replayed HIP graph (thc_ctx_set_graphs)
   thc::Arena ws;       // per-call scratch
   thc::Arena kmws;     // k-means state (kept across calls for graph replay)
   KmGraphCache* kmgraph = nullptr;
