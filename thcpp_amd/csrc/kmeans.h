@@ -7,6 +7,7 @@ namespace km {
 
 constexpr int TP = 256;        // grid points per assignment tile
 constexpr int CS_TILE = 1024;  // points per counting-sort tile (index order)
+constexpr int CS_SCAN_W = 16;  // waves of a column-scan workgroup
 constexpr int MAXB = 4096;     // max bins: clusters, and Morton cells of the spatial sort
 constexpr int CAND_CAP = 512;  // candidate centroids cached in LDS per tile
 

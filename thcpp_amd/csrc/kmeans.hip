@@ -232,14 +232,16 @@ __global__ __launch_bounds__(256) void cs_hist(const int* keys, long n, int nbin
 
 // Column scan over tiles, in place: tile_count[t][b] -> exclusive offset of
 // tile t within bin b; total[b] = count of bin b.  A workgroup takes 64 bins;
-// its 4 waves scan a quarter of the tiles each, then add the earlier quarters.
+// its CS_SCAN_W waves scan consecutive runs of the tiles, then add the earlier
+// runs' totals (16 waves: two dependent load rounds per wave at 256 tiles,
+// where 4 waves needed eight).
 __device__ __forceinline__ void cs_colscan_b(int* tile_count, int ntiles, int nbins, int* total,
                                                   const State* st, int bid, int nblk) {
   if (st && st->done) return;
-  __shared__ int part[4][64];
+  __shared__ int part[CS_SCAN_W][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int b = bid * 64 + lane;
-  const int per = (ntiles + 3) / 4;
+  const int per = (ntiles + CS_SCAN_W - 1) / CS_SCAN_W;
   const int t0 = wave * per, t1 = min(ntiles, t0 + per);
   // chunks of 16 tiles: every load of a chunk is issued before its first use
   // and before the chunk's stores (on this target stores and loads share one
@@ -270,10 +272,10 @@ __device__ __forceinline__ void cs_colscan_b(int* tile_count, int ntiles, int nb
           off += c[q];
         }
     }
-    if (wave == 3) total[b] = off;
+    if (wave == CS_SCAN_W - 1) total[b] = off;
   }
 }
-__global__ __launch_bounds__(256) void cs_colscan(int* tile_count, int ntiles, int nbins, int* total,
+__global__ __launch_bounds__(64 * CS_SCAN_W) void cs_colscan(int* tile_count, int ntiles, int nbins, int* total,
                                                   const State* st) {
   cs_colscan_b(tile_count, ntiles, nbins, total, st, (int)blockIdx.x, (int)gridDim.x);
 }
@@ -815,7 +817,8 @@ static bool carve(Arena& a, const Layout& L, Bufs& b, bool full) {
 static void enqueue_sort(const Layout& L, const Bufs& b, const int* keys, int nbins, int nbits, int* out,
                          const State* st, hipStream_t s) {
   hipLaunchKernelGGL(cs_hist, dim3(L.ncs), dim3(256), 0, s, keys, L.n, nbins, b.tc, st);
-  hipLaunchKernelGGL(cs_colscan, dim3(ceil_div(nbins, 64)), dim3(256), 0, s, b.tc, L.ncs, nbins, b.total, st);
+  hipLaunchKernelGGL(cs_colscan, dim3(ceil_div(nbins, 64)), dim3(64 * CS_SCAN_W), 0, s, b.tc, L.ncs, nbins, b.total,
+                     st);
   hipLaunchKernelGGL(cs_totscan, dim3(1), dim3(1024), 0, s, b.total, nbins, b.start, st);
   hipLaunchKernelGGL(cs_scatter, dim3(L.ncs), dim3(256), 0, s, keys, L.n, nbins, nbits, b.tc, b.start, out, st);
 }
@@ -989,7 +992,8 @@ __global__ __launch_bounds__(256) void cs_hist_m(const Job* jobs, const int* off
   cs_hist_b(a.keys, q.L.n, a.nbins, q.b.tc, a.st, bid, nblk);
 }
 
-__global__ __launch_bounds__(256) void cs_colscan_m(const Job* jobs, const int* off, int nj, int mode, int it) {
+__global__ __launch_bounds__(64 * CS_SCAN_W) void cs_colscan_m(const Job* jobs, const int* off, int nj, int mode,
+                                                                 int it) {
   KM_JOB(off)
   SortArgs a;
   if (!sort_args(q, mode, it, a)) return;
@@ -1258,7 +1262,7 @@ int kmeans_run_many(thc_ctx* ctx, int nj, const KmSpec* spec, int* status, thc_k
   auto sort = [&](int mode, int it) {
     const int col = mode == 0 ? JK_COLSP : JK_COLK;
     hipLaunchKernelGGL(cs_hist_m, grid_of(JK_NCS), dim3(256), 0, s, dj, off(JK_NCS), nl, mode, it);
-    hipLaunchKernelGGL(cs_colscan_m, grid_of(col), dim3(256), 0, s, dj, off(col), nl, mode, it);
+    hipLaunchKernelGGL(cs_colscan_m, grid_of(col), dim3(64 * CS_SCAN_W), 0, s, dj, off(col), nl, mode, it);
     hipLaunchKernelGGL(cs_totscan_m, dim3(nl), dim3(1024), 0, s, dj, mode, it);
     hipLaunchKernelGGL(cs_scatter_m, grid_of(JK_NCS), dim3(256), 0, s, dj, off(JK_NCS), nl, mode, it);
   };
