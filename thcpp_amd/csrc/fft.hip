@@ -465,7 +465,10 @@ __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const 
   if constexpr (DIG) {
     // the line's weighted values a = v.x f, b = v.y f (rounded to T as the
     // stored spectrum would be); exponent: max |value| < 2^ex; then each value
-    // times 2^-ex cut into digits (x *= 128, d = trunc(x), x -= d: exact in FP64)
+    // times 2^-ex cut into digits (x *= 128, d = trunc(x), x -= d).  The loop
+    // runs in T = FP32 and is exact there: x is a power-of-two multiple of an
+    // FP32 value, |x| < 1, and only a value below 2^-126 (rounded or flushed)
+    // can lose bits, where every digit is 0 anyway (|x| 128^5 < 1).
     __shared__ double red[FT / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     // (the weighted values go back into the thread's own LDS slots for the
@@ -492,14 +495,14 @@ __global__ __launch_bounds__(FT) void pass_b(const C2<T>* __restrict__ t, const 
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
       const int kx = fn2h.div(e), kz = e - kx * n2h;
       const C2<T> v = fin[swz(kz, kx, lsx)];
-      double xa = ldexp((double)v.x, -ex), xb = ldexp((double)v.y, -ex);
+      T xa = ldexp(v.x, -ex), xb = ldexp(v.y, -ex);
 #pragma unroll
       for (int q = 0; q < OZ_DIGITS; ++q) {
-        xa *= 128.0;
-        const double da = trunc(xa);
+        xa *= T(128);
+        const T da = trunc(xa);
         xa -= da;
-        xb *= 128.0;
-        const double db = trunc(xb);
+        xb *= T(128);
+        const T db = trunc(xb);
         xb -= db;
         *reinterpret_cast<unsigned short*>(row + q * p.dig_plane + 2 * e) =
             (unsigned short)(((unsigned)(int)da & 0xffu) | (((unsigned)(int)db & 0xffu) << 8));
